@@ -1,0 +1,115 @@
+/*
+ * nsim-mi355x: C ABI of the MI355X-native LLG right-hand-side path.
+ *
+ * Plain C types only (no torch, no HIP types).  Every entry point returns an
+ * int status: 0 = ok, > 0 = recoverable (the caller may retry with a smaller
+ * step, as CVODE does), < 0 = fatal.  nsim_last_error() returns the message of
+ * the last failure on the calling thread.  A handle owns one HIP stream and all
+ * device memory of one magnet; calls on one handle are not re-entrant.
+ *
+ * The reference exposes this path through two stacked boundaries, and each
+ * entry point below names the interface it replaces:
+ *   B1  Python -> "ocaml" module (src/nsimexec.ml:4467-4676 registry)
+ *   B2  CVODE callbacks (src/sundials_sp_stubs.c)
+ */
+#ifndef NSIM_H
+#define NSIM_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct nsim_handle nsim_handle;
+
+/* Tetrahedral mesh: coordinates in simulation length units (nm). */
+typedef struct {
+  int n_nodes;
+  int n_tets;
+  const double* coords; /* 3*n_nodes, site-major                      */
+  const int* tets;      /* 4*n_tets node indices                      */
+} nsim_mesh;
+
+/* LLG and material coefficients in simulation units; the same numbers that
+ * MagMaterial computes (interface/nmag/main.py:423-471). */
+typedef struct {
+  double Ms;   /* su_Ms                                                   */
+  double negJ; /* -su_exch_prefactor = -2A/(mu0 Ms) (nmag_lam.py:256)     */
+  double c1;   /* su_llg_coeff1 = -gamma/(1+alpha^2)                       */
+  double c2;   /* su_llg_coeff2 = c1*alpha                                 */
+  double c3;   /* su_llg_normalisationfactor                               */
+  double mu0;  /* su_mu0                                                  */
+} nsim_material;
+
+enum {
+  NSIM_FP64 = 0,       /* everything in FP64                                */
+  NSIM_MIXED_FP32 = 1, /* FP32 SpMV and BEM, FP64 dots/state/norms          */
+  NSIM_MIXED_BF16 = 2  /* as MIXED_FP32 but the BEM matrix stored in BF16   */
+};
+
+/* Solver options; the KSP tolerances mirror Simulation(ksp_tolerances=...)
+ * (interface/nmag/main.py:819-843, nmag_lam.py:614-633). */
+typedef struct {
+  int precision;   /* NSIM_FP64 / NSIM_MIXED_FP32 / NSIM_MIXED_BF16        */
+  int do_demag;    /* 0: exchange/anisotropy/applied field only            */
+  double nbc_rtol; /* Neumann solve relative tolerance                     */
+  double dbc_rtol; /* Dirichlet solve relative tolerance                   */
+  int nbc_maxits;
+  int dbc_maxits;
+} nsim_options;
+
+/* Replaces raw_make_lam (src/nsimexec.ml:3605 -> src/nsim.ml:2828): builds
+ * every operator (stiffness, divergence, gradient, dense BEM) on the device. */
+int nsim_create(const nsim_mesh* mesh, const nsim_material* mat,
+                const nsim_options* opt, nsim_handle** out);
+void nsim_destroy(nsim_handle* h);
+const char* nsim_last_error(void);
+
+/* sizes[0..4] = n_nodes, n_boundary_nodes, n_boundary_triangles, n_tets,
+ * stored operator entries (padded SELL slots). */
+int nsim_get_sizes(nsim_handle* h, int* sizes);
+
+/* Replaces lam_set_field / lam_get_field (src/nsimexec.ml:4286-4305).
+ * Settable: "m", "H_ext" (3N), "K1", "K2" (N), "axis" (3N), "pin" (N).
+ * Readable additionally: "H_exch","H_demag","H_anis","H_total","dmdt" (3N),
+ * "rho","phi","phi1","phi2","volumes" (N), "solid_angles" (N_b),
+ * "boundary_nodes" (N_b, as doubles). */
+int nsim_set_field(nsim_handle* h, const char* name, const double* host_data);
+int nsim_get_field(nsim_handle* h, const char* name, double* host_out);
+
+/* Replaces lam_execute (src/nsimexec.ml:4077-4095) for the path's programs:
+ * "update_dmdt", "set_H_demag", "set_H_exch", "set_H_anis", "set_H_total". */
+int nsim_execute(nsim_handle* h, const char* program);
+
+/* Dense BEM matrix (N_b x N_b, row-major, FP64 copy) for inspection. */
+int nsim_get_bem(nsim_handle* h, double* host_out);
+
+/* CVODE rhs callback (src/sundials_sp_stubs.c:264, src/nsim.ml:2273-2313):
+ * ydot = f(t, y).  y and ydot are 3N arrays; device_ptrs != 0 means they are
+ * device pointers, else host pointers. */
+int nsim_rhs(nsim_handle* h, double t, const double* y, double* ydot, int device_ptrs);
+
+/* CVODE jtimes callback (src/sundials_sp_stubs.c:579, src/nsim.ml:2400-2417):
+ * Jv = J(y) v with H frozen at the last rhs except for the exchange term,
+ * the reference's jacobi_ts1 linearisation (nmag_lam.py:432-436). */
+int nsim_jtimes(nsim_handle* h, double t, const double* v, double* Jv,
+                const double* y, int device_ptrs);
+
+/* Statistics: "cg_iters_nbc", "cg_iters_dbc" (last solve), "rhs_evals",
+ * "cg_iters_nbc_total", "cg_iters_dbc_total". */
+int nsim_get_stat(nsim_handle* h, const char* name, double* out);
+
+/* Per-kernel device timers, the analogue of the reference's TSTART/TSTOP
+ * (src/nsim.ml:3414-3453): when enabled, each launch on the handle's stream is
+ * bracketed by HIP events.  name: "exch_div", "cg_nbc", "bem", "cg_dbc",
+ * "demag_llg"; out[0] = total ms, out[1] = launches. */
+int nsim_timers_enable(nsim_handle* h, int on);
+/* Launch-shape knobs (used by the tile autotuner); key "cg_rows_per_thread"
+ * takes 1, 2, 4 or 8.  No counterpart in the reference (PETSc picks its own). */
+int nsim_set_tuning(nsim_handle* h, const char* key, int value);
+int nsim_get_timer(nsim_handle* h, const char* name, double* out);
+int nsim_synchronize(nsim_handle* h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,569 @@
+// C ABI of the MI355X-native LLG right-hand-side path (declared in include/nsim.h).
+//
+// A handle is one magnet: its operators in HBM, its field vectors, one HIP
+// stream and the persistent-CG communication buffers.  The rhs program follows
+// update_dmdt (interface/nmag/nmag_lam.py:784-793): set_H_demag, set_H_exch,
+// set_H_anis, set_H_total, then the site-wise LLG, all on the device.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "nsim_device.h"
+#include "nsim_internal.h"
+#include "nsim_integrator.h"
+
+using namespace nsim;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(const std::string& msg, int code = -1) {
+  g_last_error = msg;
+  return code;
+}
+
+#define HIPCHK(expr)                                                              \
+  do {                                                                            \
+    hipError_t _e = (expr);                                                       \
+    if (_e != hipSuccess)                                                         \
+      return fail(std::string(#expr) + ": " + hipGetErrorString(_e));             \
+  } while (0)
+
+template <typename T>
+hipError_t upload(T** dst, const T* src, size_t count) {
+  *dst = nullptr;
+  if (count == 0) return hipSuccess;
+  hipError_t e = hipMalloc(reinterpret_cast<void**>(dst), count * sizeof(T));
+  if (e != hipSuccess) return e;
+  return hipMemcpy(*dst, src, count * sizeof(T), hipMemcpyHostToDevice);
+}
+
+template <typename T>
+hipError_t zeros(T** dst, size_t count) {
+  *dst = nullptr;
+  if (count == 0) return hipSuccess;
+  hipError_t e = hipMalloc(reinterpret_cast<void**>(dst), count * sizeof(T));
+  if (e != hipSuccess) return e;
+  return hipMemset(*dst, 0, count * sizeof(T));
+}
+
+struct Timer {
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+  double ms = 0.0;
+  long count = 0;
+};
+
+}  // namespace
+
+struct nsim_handle {
+  HostOperators host;
+  DeviceOps d;
+  CgWork cgw;
+  unsigned* sync_nbc = nullptr;
+  unsigned* sync_dbc = nullptr;
+  Coeffs coef{};
+  nsim_options opt{};
+  bool mixed = false;
+  int device = 0, num_cus = 0;
+  hipStream_t stream = nullptr;
+  int cg_grid = 0, cg_rpt = 1;
+  // field vectors (site-major 3-vectors or scalars)
+  double *m = nullptr, *Hext = nullptr, *k1 = nullptr, *k2 = nullptr, *axis = nullptr,
+         *pin = nullptr;
+  double *Hexch = nullptr, *Hdemag = nullptr, *Hanis = nullptr, *Htotal = nullptr,
+         *dmdt = nullptr, *rho = nullptr, *phi1 = nullptr, *phi2 = nullptr, *phi = nullptr,
+         *phi1b = nullptr;
+  double *io_a = nullptr, *io_b = nullptr, *io_c = nullptr;  // staging for host-pointer calls
+  bool have_hext = false, have_anis = false, have_k2 = false, have_pin = false;
+  long rhs_evals = 0;
+  bool timers_on = false;
+  std::map<std::string, Timer> timers;
+  Integrator* ts = nullptr;
+  std::vector<void*> allocations;
+
+  void track(void* p) {
+    if (p) allocations.push_back(p);
+  }
+};
+
+namespace {
+
+// Records a named device interval around a launch when timers are enabled.
+struct Span {
+  nsim_handle* h;
+  Timer* t = nullptr;
+  hipEvent_t a{}, b{};
+  Span(nsim_handle* hh, const char* name) : h(hh) {
+    if (!h->timers_on) return;
+    t = &h->timers[name];
+    (void)hipEventCreate(&a);
+    (void)hipEventCreate(&b);
+    (void)hipEventRecord(a, h->stream);
+  }
+  ~Span() {
+    if (!t) return;
+    (void)hipEventRecord(b, h->stream);
+    t->pending.push_back({a, b});
+  }
+};
+
+void flush_timers(nsim_handle* h) {
+  (void)hipStreamSynchronize(h->stream);
+  for (auto& kv : h->timers) {
+    for (auto& ev : kv.second.pending) {
+      float ms = 0.f;
+      if (hipEventElapsedTime(&ms, ev.first, ev.second) == hipSuccess) kv.second.ms += ms;
+      kv.second.count++;
+      (void)hipEventDestroy(ev.first);
+      (void)hipEventDestroy(ev.second);
+    }
+    kv.second.pending.clear();
+  }
+}
+
+FieldArgs field_args(nsim_handle* h, const double* m) {
+  FieldArgs f;
+  f.m = m;
+  f.Hext = h->have_hext ? h->Hext : nullptr;
+  f.k1 = h->have_anis ? h->k1 : nullptr;
+  f.k2 = (h->have_anis && h->have_k2) ? h->k2 : nullptr;
+  f.axis = h->have_anis ? h->axis : nullptr;
+  f.pin = h->have_pin ? h->pin : nullptr;
+  return f;
+}
+
+}  // namespace
+
+namespace nsim {
+
+// Full field evaluation for magnetisation m (device) into dmdt (device).
+int device_rhs(nsim_handle* h, const double* m, double* dmdt) {
+  const bool demag = h->opt.do_demag != 0 && h->d.nb > 0;
+  {
+    Span s(h, "exch_div");
+    HIPCHK(launch_exch_div(h->d, h->mixed, h->coef, m, h->Hexch, h->rho, demag, h->stream));
+  }
+  if (demag) {
+    CgWork w = h->cgw;
+    {
+      Span s(h, "cg_nbc");
+      w.sync = h->sync_nbc;
+      HIPCHK(launch_cg(h->d, h->mixed, w, h->cg_grid, h->cg_rpt, h->d.dinv_nbc, h->rho,
+                       h->phi1, h->opt.nbc_rtol, h->opt.nbc_maxits, nullptr, nullptr,
+                       h->phi1b, h->stream));
+    }
+    {
+      Span s(h, "bem");
+      HIPCHK(launch_bem_gemv(h->d, h->phi1b, h->phi2, h->stream, h->num_cus));
+    }
+    {
+      Span s(h, "cg_dbc");
+      w.sync = h->sync_dbc;
+      HIPCHK(launch_cg(h->d, h->mixed, w, h->cg_grid, h->cg_rpt, h->d.dinv_dbc, nullptr,
+                       h->phi2, h->opt.dbc_rtol, h->opt.dbc_maxits, h->phi1, h->phi,
+                       nullptr, h->stream));
+    }
+  }
+  {
+    Span s(h, "demag_llg");
+    FieldBufs b{h->Hexch, h->Hdemag, h->Hanis, h->Htotal, h->rho,
+                h->phi1,  h->phi2,   h->phi,   h->phi1b};
+    HIPCHK(launch_demag_llg(h->d, h->mixed, h->coef, field_args(h, m), b, dmdt, demag,
+                            h->stream));
+  }
+  h->rhs_evals++;
+  return 0;
+}
+
+int device_jtimes(nsim_handle* h, const double* m, const double* v, double* Jv) {
+  Span s(h, "jtimes");
+  HIPCHK(launch_jtimes(h->d, h->mixed, h->coef, m, h->Htotal, v, Jv, h->stream));
+  return 0;
+}
+
+hipStream_t handle_stream(nsim_handle* h) { return h->stream; }
+int handle_n(nsim_handle* h) { return h->d.n; }
+const double* handle_volumes_inv(nsim_handle* h) { return h->d.Vinv; }
+double* handle_m(nsim_handle* h) { return h->m; }
+int handle_error_flags(nsim_handle* h, unsigned* out) {
+  unsigned a[6], b[6];
+  HIPCHK(hipMemcpyAsync(a, h->sync_nbc, sizeof(a), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipMemcpyAsync(b, h->sync_dbc, sizeof(b), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  *out = a[2] | b[2];
+  return 0;
+}
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+}  // namespace nsim
+
+extern "C" {
+
+const char* nsim_last_error(void) { return g_last_error.c_str(); }
+
+int nsim_create(const nsim_mesh* mesh, const nsim_material* mat, const nsim_options* opt,
+                nsim_handle** out) {
+  if (!mesh || !mat || !out) return fail("nsim_create: null argument");
+  *out = nullptr;
+  nsim_handle* h = new nsim_handle();
+  auto cleanup = [&](int code) {
+    nsim_destroy(h);
+    return code;
+  };
+  std::string err =
+      build_host_operators(mesh->n_nodes, mesh->coords, mesh->n_tets, mesh->tets, h->host);
+  if (!err.empty()) return cleanup(fail("nsim_create: " + err));
+  if (opt) h->opt = *opt;
+  else {
+    h->opt.precision = NSIM_FP64;
+    h->opt.do_demag = 1;
+    h->opt.nbc_rtol = h->opt.dbc_rtol = 1e-5;
+    h->opt.nbc_maxits = h->opt.dbc_maxits = 10000;
+  }
+  if (h->opt.precision < NSIM_FP64 || h->opt.precision > NSIM_MIXED_BF16)
+    return cleanup(fail("nsim_create: unknown precision"));
+  if (h->opt.nbc_maxits <= 0) h->opt.nbc_maxits = 10000;
+  if (h->opt.dbc_maxits <= 0) h->opt.dbc_maxits = 10000;
+  h->mixed = h->opt.precision != NSIM_FP64;
+  h->coef = Coeffs{mat->Ms, mat->negJ, mat->c1, mat->c2, mat->c3, mat->mu0};
+
+  const HostOperators& o = h->host;
+  DeviceOps& d = h->d;
+  d.n = o.n;
+  d.nb = o.nb;
+  d.ntri = o.ntri;
+  d.nslices = o.nslices;
+  d.sell_total = o.sell_off[o.nslices];
+  d.ldb = ((o.nb + 63) / 64) * 64;
+  d.btype = h->opt.precision == NSIM_FP64 ? kBemF64
+            : h->opt.precision == NSIM_MIXED_FP32 ? kBemF32 : kBemBF16;
+  d.geo_eps = o.geo_eps;
+
+  hipError_t e;
+#define UP(dst, vec)                                                        \
+  do {                                                                      \
+    e = upload(&(dst), (vec).data(), (vec).size());                         \
+    if (e != hipSuccess) return cleanup(fail(std::string("upload: ") + hipGetErrorString(e))); \
+    h->track(dst);                                                          \
+  } while (0)
+#define ZEROS(dst, count)                                                   \
+  do {                                                                      \
+    e = zeros(&(dst), (count));                                             \
+    if (e != hipSuccess) return cleanup(fail(std::string("alloc: ") + hipGetErrorString(e))); \
+    h->track(dst);                                                          \
+  } while (0)
+
+  if (hipGetDevice(&h->device) != hipSuccess) return cleanup(fail("no HIP device"));
+  (void)hipDeviceGetAttribute(&h->num_cus, hipDeviceAttributeMultiprocessorCount, h->device);
+  if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess)
+    return cleanup(fail("hipStreamCreate failed"));
+
+  UP(d.sell_off, o.sell_off);
+  UP(d.sell_col, o.sell_col);
+  UP(d.K, o.sK);
+  UP(d.D, o.sD);
+  UP(d.G, o.sG);
+  if (h->mixed) {
+    std::vector<float> kf(o.sK.begin(), o.sK.end()), df(o.sD.begin(), o.sD.end()),
+        gf(o.sG.begin(), o.sG.end());
+    UP(d.Kf, kf);
+    UP(d.Df, df);
+    UP(d.Gf, gf);
+  }
+  std::vector<double> vinv(o.n), dn(o.n), dd(o.n);
+  for (int i = 0; i < o.n; i++) {
+    vinv[i] = 1.0 / o.V[i];
+    dn[i] = 1.0 / o.Kdiag[i];
+    dd[i] = o.blocal[i] >= 0 ? 0.0 : dn[i];
+  }
+  UP(d.Vinv, vinv);
+  UP(d.dinv_nbc, dn);
+  UP(d.dinv_dbc, dd);
+  UP(d.blocal, o.blocal);
+
+  // BEM matrix, assembled on the device from the boundary geometry
+  if (o.nb > 0) {
+    std::vector<int> tri_node(3 * (size_t)o.ntri);
+    for (size_t k = 0; k < tri_node.size(); k++) tri_node[k] = o.bnode[o.tri[k]];
+    UP(d.bnode, o.bnode);
+    UP(d.coords, o.x);
+    UP(d.tri_node, tri_node);
+    UP(d.tnorm, o.tnorm);
+    UP(d.omega, o.omega);
+    UP(d.inc_rp, o.inc_rp);
+    UP(d.inc_tri, o.inc_tri);
+    size_t esz = d.btype == kBemF64 ? 8 : d.btype == kBemF32 ? 4 : 2;
+    e = hipMalloc(&d.B, (size_t)o.nb * d.ldb * esz);
+    if (e != hipSuccess) return cleanup(fail(std::string("BEM alloc: ") + hipGetErrorString(e)));
+    h->track(d.B);
+    e = launch_bem_assemble(d, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess)
+      return cleanup(fail(std::string("BEM assembly: ") + hipGetErrorString(e)));
+  }
+
+  // field vectors
+  const size_t n = o.n;
+  ZEROS(h->m, 3 * n);
+  ZEROS(h->Hext, 3 * n);
+  ZEROS(h->k1, n);
+  ZEROS(h->k2, n);
+  ZEROS(h->axis, 3 * n);
+  ZEROS(h->pin, n);
+  ZEROS(h->Hexch, 3 * n);
+  ZEROS(h->Hdemag, 3 * n);
+  ZEROS(h->Hanis, 3 * n);
+  ZEROS(h->Htotal, 3 * n);
+  ZEROS(h->dmdt, 3 * n);
+  ZEROS(h->rho, n);
+  ZEROS(h->phi1, n);
+  ZEROS(h->phi2, n);
+  ZEROS(h->phi, n);
+  ZEROS(h->phi1b, std::max<size_t>(1, o.nb));
+  ZEROS(h->io_a, 3 * n);
+  ZEROS(h->io_b, 3 * n);
+  ZEROS(h->io_c, 3 * n);
+
+  // persistent CG launch shape: smallest rows-per-thread that fits co-resident
+  int rpt = 1, grid = 0;
+  for (; rpt <= 8; rpt *= 2) {
+    grid = (int)((n + 256 * (size_t)rpt - 1) / (256 * (size_t)rpt));
+    if (grid <= cg_max_workgroups(h->mixed, rpt, h->device)) break;
+  }
+  if (rpt > 8) return cleanup(fail("mesh too large for the co-resident CG kernel"));
+  h->cg_rpt = rpt;
+  h->cg_grid = grid;
+  h->cgw.gmax = std::max(grid, 1024);
+  ZEROS(h->cgw.pub, 2 * n);
+  ZEROS(h->cgw.partial, 8 * (size_t)h->cgw.gmax);
+  ZEROS(h->sync_nbc, 8);
+  ZEROS(h->sync_dbc, 8);
+  h->cgw.sync = h->sync_nbc;
+#undef UP
+#undef ZEROS
+  if (hipDeviceSynchronize() != hipSuccess) return cleanup(fail("device sync failed"));
+  *out = h;
+  return 0;
+}
+
+void nsim_destroy(nsim_handle* h) {
+  if (!h) return;
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  delete h->ts;
+  for (auto& kv : h->timers)
+    for (auto& ev : kv.second.pending) {
+      (void)hipEventDestroy(ev.first);
+      (void)hipEventDestroy(ev.second);
+    }
+  for (void* p : h->allocations) (void)hipFree(p);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
+  delete h;
+}
+
+int nsim_get_sizes(nsim_handle* h, int* s) {
+  if (!h || !s) return fail("nsim_get_sizes: null argument");
+  s[0] = h->d.n;
+  s[1] = h->d.nb;
+  s[2] = h->d.ntri;
+  s[3] = h->host.T;
+  s[4] = h->d.sell_total;
+  return 0;
+}
+
+int nsim_set_tuning(nsim_handle* h, const char* key, int value) {
+  if (!h || !key) return fail("nsim_set_tuning: null argument");
+  std::string k(key);
+  if (k == "cg_rows_per_thread") {
+    if (value != 1 && value != 2 && value != 4 && value != 8)
+      return fail("cg_rows_per_thread must be 1, 2, 4 or 8");
+    int grid = (h->d.n + 256 * value - 1) / (256 * value);
+    if (grid > cg_max_workgroups(h->mixed, value, h->device) || grid > h->cgw.gmax)
+      return fail("too few rows per thread for a co-resident CG grid");
+    h->cg_rpt = value;
+    h->cg_grid = grid;
+    return 0;
+  }
+  return fail("unknown tuning key: " + k);
+}
+
+int nsim_set_field(nsim_handle* h, const char* name, const double* data) {
+  if (!h || !name || !data) return fail("nsim_set_field: null argument");
+  std::string f(name);
+  const size_t n = h->d.n;
+  double* dst = nullptr;
+  size_t cnt = 0;
+  if (f == "m") dst = h->m, cnt = 3 * n;
+  else if (f == "H_ext") dst = h->Hext, cnt = 3 * n, h->have_hext = true;
+  else if (f == "K1") dst = h->k1, cnt = n, h->have_anis = true;
+  else if (f == "K2") dst = h->k2, cnt = n, h->have_k2 = true;
+  else if (f == "axis") dst = h->axis, cnt = 3 * n;
+  else if (f == "pin") dst = h->pin, cnt = n, h->have_pin = true;
+  else if (f == "phi1") dst = h->phi1, cnt = n;
+  else if (f == "phi2") dst = h->phi2, cnt = n;
+  else return fail("nsim_set_field: unknown or read-only field '" + f + "'");
+  HIPCHK(hipMemcpyAsync(dst, data, cnt * sizeof(double), hipMemcpyHostToDevice, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+int nsim_get_field(nsim_handle* h, const char* name, double* out) {
+  if (!h || !name || !out) return fail("nsim_get_field: null argument");
+  std::string f(name);
+  const size_t n = h->d.n;
+  const double* src = nullptr;
+  size_t cnt = 0;
+  if (f == "volumes") {
+    std::copy(h->host.V.begin(), h->host.V.end(), out);
+    return 0;
+  }
+  if (f == "solid_angles") {
+    std::copy(h->host.omega.begin(), h->host.omega.end(), out);
+    return 0;
+  }
+  if (f == "boundary_nodes") {
+    for (int i = 0; i < h->d.nb; i++) out[i] = h->host.bnode[i];
+    return 0;
+  }
+  if (f == "m") src = h->m, cnt = 3 * n;
+  else if (f == "H_ext") src = h->Hext, cnt = 3 * n;
+  else if (f == "H_exch") src = h->Hexch, cnt = 3 * n;
+  else if (f == "H_demag") src = h->Hdemag, cnt = 3 * n;
+  else if (f == "H_anis") src = h->Hanis, cnt = 3 * n;
+  else if (f == "H_total") src = h->Htotal, cnt = 3 * n;
+  else if (f == "dmdt") src = h->dmdt, cnt = 3 * n;
+  else if (f == "rho") src = h->rho, cnt = n;
+  else if (f == "phi") src = h->phi, cnt = n;
+  else if (f == "phi1") src = h->phi1, cnt = n;
+  else if (f == "phi2") src = h->phi2, cnt = n;
+  else if (f == "K1") src = h->k1, cnt = n;
+  else if (f == "pin") src = h->pin, cnt = n;
+  else return fail("nsim_get_field: unknown field '" + f + "'");
+  HIPCHK(hipMemcpyAsync(out, src, cnt * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+int nsim_get_bem(nsim_handle* h, double* out) {
+  if (!h || !out) return fail("nsim_get_bem: null argument");
+  const size_t nb = h->d.nb, ld = h->d.ldb;
+  if (nb == 0) return 0;
+  const size_t esz = h->d.btype == kBemF64 ? 8 : h->d.btype == kBemF32 ? 4 : 2;
+  std::vector<unsigned char> buf(nb * ld * esz);
+  HIPCHK(hipMemcpy(buf.data(), h->d.B, buf.size(), hipMemcpyDeviceToHost));
+  for (size_t i = 0; i < nb; i++)
+    for (size_t j = 0; j < nb; j++) {
+      const size_t k = i * ld + j;
+      double v;
+      if (h->d.btype == kBemF64) v = reinterpret_cast<double*>(buf.data())[k];
+      else if (h->d.btype == kBemF32) v = reinterpret_cast<float*>(buf.data())[k];
+      else {
+        uint32_t u = (uint32_t)reinterpret_cast<uint16_t*>(buf.data())[k] << 16;
+        float f;
+        std::memcpy(&f, &u, 4);
+        v = f;
+      }
+      out[i * nb + j] = v;
+    }
+  return 0;
+}
+
+int nsim_execute(nsim_handle* h, const char* program) {
+  if (!h || !program) return fail("nsim_execute: null argument");
+  std::string p(program);
+  // Every field program of the path recomputes the whole dependency set of
+  // update_dmdt: the H_* vectors it leaves behind are those of the current m.
+  if (p == "update_dmdt" || p == "set_H_demag" || p == "set_H_exch" || p == "set_H_anis" ||
+      p == "set_H_total")
+    return device_rhs(h, h->m, h->dmdt);
+  return fail("nsim_execute: unknown program '" + p + "'");
+}
+
+int nsim_rhs(nsim_handle* h, double t, const double* y, double* ydot, int device_ptrs) {
+  (void)t;
+  if (!h || !y || !ydot) return fail("nsim_rhs: null argument");
+  const size_t bytes = 3 * (size_t)h->d.n * sizeof(double);
+  if (device_ptrs) return device_rhs(h, y, ydot);
+  HIPCHK(hipMemcpyAsync(h->io_a, y, bytes, hipMemcpyHostToDevice, h->stream));
+  int rc = device_rhs(h, h->io_a, h->io_b);
+  if (rc) return rc;
+  HIPCHK(hipMemcpyAsync(ydot, h->io_b, bytes, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+int nsim_jtimes(nsim_handle* h, double t, const double* v, double* Jv, const double* y,
+                int device_ptrs) {
+  (void)t;
+  if (!h || !v || !Jv || !y) return fail("nsim_jtimes: null argument");
+  const size_t bytes = 3 * (size_t)h->d.n * sizeof(double);
+  if (device_ptrs) return device_jtimes(h, y, v, Jv);
+  HIPCHK(hipMemcpyAsync(h->io_a, y, bytes, hipMemcpyHostToDevice, h->stream));
+  HIPCHK(hipMemcpyAsync(h->io_b, v, bytes, hipMemcpyHostToDevice, h->stream));
+  double* out = h->io_c;
+  int rc = device_jtimes(h, h->io_a, h->io_b, out);
+  if (rc) return rc;
+  HIPCHK(hipMemcpyAsync(Jv, out, bytes, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+int nsim_get_stat(nsim_handle* h, const char* name, double* out) {
+  if (!h || !name || !out) return fail("nsim_get_stat: null argument");
+  std::string s(name);
+  if (s == "rhs_evals") {
+    *out = (double)h->rhs_evals;
+    return 0;
+  }
+  if (s == "cg_grid") {
+    *out = h->cg_grid;
+    return 0;
+  }
+  if (s == "cg_rows_per_thread") {
+    *out = h->cg_rpt;
+    return 0;
+  }
+  unsigned a[8], b[8];
+  HIPCHK(hipMemcpyAsync(a, h->sync_nbc, sizeof(a), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipMemcpyAsync(b, h->sync_dbc, sizeof(b), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  if (s == "cg_iters_nbc") *out = a[3];
+  else if (s == "cg_iters_dbc") *out = b[3];
+  else if (s == "cg_iters_nbc_total") *out = a[4];
+  else if (s == "cg_iters_dbc_total") *out = b[4];
+  else if (s == "cg_solves") *out = a[5] + b[5];
+  else if (s == "cg_error") *out = (double)(a[2] | b[2]);
+  else return fail("nsim_get_stat: unknown statistic '" + s + "'");
+  return 0;
+}
+
+int nsim_timers_enable(nsim_handle* h, int on) {
+  if (!h) return fail("nsim_timers_enable: null handle");
+  flush_timers(h);
+  if (on) h->timers.clear();
+  h->timers_on = on != 0;
+  return 0;
+}
+
+int nsim_get_timer(nsim_handle* h, const char* name, double* out) {
+  if (!h || !name || !out) return fail("nsim_get_timer: null argument");
+  flush_timers(h);
+  auto it = h->timers.find(name);
+  out[0] = it == h->timers.end() ? 0.0 : it->second.ms;
+  out[1] = it == h->timers.end() ? 0.0 : (double)it->second.count;
+  return 0;
+}
+
+int nsim_synchronize(nsim_handle* h) {
+  if (!h) return fail("nsim_synchronize: null handle");
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+}  // extern "C"

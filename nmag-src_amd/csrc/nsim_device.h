@@ -1,0 +1,87 @@
+// Device-side data of one magnet and the launch interface of the kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace nsim {
+
+enum BemType { kBemF64 = 0, kBemF32 = 1, kBemBF16 = 2 };
+
+// Operators resident in HBM (one copy per precision actually used).
+struct DeviceOps {
+  int n = 0, nb = 0, ntri = 0, ldb = 0, nslices = 0, sell_total = 0;
+  int* sell_off = nullptr;
+  int* sell_col = nullptr;
+  double* K = nullptr;  // FP64 values (always present)
+  double* D = nullptr;
+  double* G = nullptr;
+  float* Kf = nullptr;  // FP32 copies (mixed precision only)
+  float* Df = nullptr;
+  float* Gf = nullptr;
+  double* Vinv = nullptr;      // 1/V_i
+  double* dinv_nbc = nullptr;  // 1/K_ii
+  double* dinv_dbc = nullptr;  // 1/K_ii on interior rows, 0 on boundary rows
+  int* bnode = nullptr;        // local boundary index -> node
+  int* blocal = nullptr;       // node -> local boundary index or -1
+  void* B = nullptr;           // nb x ldb row-major
+  int btype = kBemF64;
+  // BEM geometry (set-up only)
+  double* coords = nullptr;
+  int* tri_node = nullptr;     // 3*ntri global node indices
+  double* tnorm = nullptr;
+  double* omega = nullptr;
+  int* inc_rp = nullptr;
+  int* inc_tri = nullptr;
+  double geo_eps = 0.0;
+};
+
+// Communication buffers of the persistent CG kernel.
+struct CgWork {
+  double* pub = nullptr;      // 2*n published vector (double buffered)
+  double* partial = nullptr;  // 2 * 4 * gmax per-workgroup partial sums
+  unsigned* sync = nullptr;   // [0] barrier counter [1] exit counter [2] error flag
+                              // [3] last iterations [4] total iterations [5] solves
+  int gmax = 0;
+};
+
+struct Coeffs {
+  double Ms, negJ, c1, c2, c3, mu0;
+};
+
+// Field inputs of one evaluation (device pointers; optional ones may be null).
+struct FieldArgs {
+  const double* m;
+  const double* Hext;
+  const double* k1;
+  const double* k2;
+  const double* axis;
+  const double* pin;
+};
+
+struct FieldBufs {
+  double *Hexch, *Hdemag, *Hanis, *Htotal, *rho, *phi1, *phi2, *phi, *phi1b;
+};
+
+// Launchers (all asynchronous on `st`).  Return hipSuccess or the launch error.
+hipError_t launch_bem_assemble(const DeviceOps& d, hipStream_t st);
+hipError_t launch_exch_div(const DeviceOps& d, bool mixed, const Coeffs& c, const double* m,
+                           double* Hexch, double* rho, bool want_rho, hipStream_t st);
+// Persistent pipelined Jacobi-PCG on the stiffness matrix; see the kernel.
+hipError_t launch_cg(const DeviceOps& d, bool mixed, const CgWork& w, int grid, int rpt,
+                     const double* dinv, const double* b, double* x, double rtol, int maxit,
+                     const double* xadd, double* xsum, double* restrict_out,
+                     hipStream_t st);
+hipError_t launch_bem_gemv(const DeviceOps& d, const double* phi1b, double* phi2,
+                           hipStream_t st, int num_cus);
+hipError_t launch_demag_llg(const DeviceOps& d, bool mixed, const Coeffs& c,
+                            const FieldArgs& f, const FieldBufs& b, double* dmdt,
+                            bool do_demag, hipStream_t st);
+hipError_t launch_jtimes(const DeviceOps& d, bool mixed, const Coeffs& c, const double* m,
+                         const double* Htotal, const double* v, double* Jv, hipStream_t st);
+
+// Cooperative-launch capacity of the CG kernel (workgroups of 256 threads).
+int cg_max_workgroups(bool mixed, int rpt, int device);
+
+}  // namespace nsim

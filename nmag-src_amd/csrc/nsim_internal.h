@@ -1,0 +1,47 @@
+// Internal structures shared by the host set-up, the kernels and the C ABI.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/nsim.h"
+
+namespace nsim {
+
+constexpr int kSlice = 64;  // SELL slice height = one wavefront
+
+// Host-side mesh preprocessing result (all FP64, host memory).
+struct HostOperators {
+  int n = 0, T = 0;
+  std::vector<double> x;       // 3n coordinates
+  std::vector<int> tet;        // 4T
+  std::vector<double> V;       // n box volumes (src/fem.ml:1757-1788)
+  // node-adjacency CSR (sorted, diagonal included)
+  std::vector<int> rp, ci;
+  std::vector<double> K;       // stiffness
+  std::vector<double> D;       // 3/nnz: rho_i = Ms * sum_j D_ij . m_j
+  std::vector<double> G;       // 3/nnz: H_demag_i = sum_j G_ij phi_j
+  std::vector<double> Kdiag;
+  // SELL-64 copy of the same pattern
+  int nslices = 0;
+  std::vector<int> sell_off;   // nslices+1, element offsets
+  std::vector<int> sell_col;
+  std::vector<double> sK, sD, sG;
+  // boundary
+  int nb = 0, ntri = 0;
+  std::vector<int> bnode;      // nb, ascending global index
+  std::vector<int> blocal;     // n -> local boundary index or -1
+  std::vector<int> tri;        // 3*ntri (local boundary indices)
+  std::vector<double> tnorm;   // 3*ntri outward unit normals
+  std::vector<double> omega;   // nb interior solid angles
+  // triangles incident to each boundary node: (triangle, vertex slot)
+  std::vector<int> inc_rp, inc_tri;
+  double geo_eps = 0.0;
+};
+
+// Builds every host-side operator; returns "" on success, else a message.
+std::string build_host_operators(int n, const double* coords, int T, const int* tets,
+                                 HostOperators& ops);
+
+}  // namespace nsim

@@ -1,0 +1,13 @@
+"""nmag-compatible front end of the MI355X-native LLG right-hand-side path.
+
+Keeps the surface of the reference's interface/nmag package for this path:
+Simulation, MagMaterial, SI, uniaxial_anisotropy, load_mesh, set_m, set_H_ext,
+advance_time, relax, get_subfield.  All compute runs in libnsim_hip.so.
+"""
+from .units import SI, mu0, to_su, from_su
+from .material import MagMaterial, NmagUserError, uniaxial_anisotropy
+from .mesh import kuhn_cuboid, cuboid_sizes
+from . import _native
+
+__all__ = ["SI", "mu0", "to_su", "from_su", "MagMaterial", "NmagUserError",
+           "uniaxial_anisotropy", "kuhn_cuboid", "cuboid_sizes"]

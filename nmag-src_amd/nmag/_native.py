@@ -1,0 +1,189 @@
+"""ctypes binding of the HIP library (C ABI declared in include/nsim.h).
+
+The library is the only compute path: if it is missing, or no HIP device is
+visible, creating a magnet raises instead of falling back to anything else.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libnsim_hip.so")
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_vp = ctypes.c_void_p
+
+FP64, MIXED_FP32, MIXED_BF16 = 0, 1, 2
+PRECISIONS = {"fp64": FP64, "mixed": MIXED_FP32, "fp32": MIXED_FP32, "bf16": MIXED_BF16}
+
+
+class NsimMesh(ctypes.Structure):
+    _fields_ = [("n_nodes", ctypes.c_int), ("n_tets", ctypes.c_int),
+                ("coords", _dp), ("tets", _ip)]
+
+
+class NsimMaterial(ctypes.Structure):
+    _fields_ = [("Ms", ctypes.c_double), ("negJ", ctypes.c_double), ("c1", ctypes.c_double),
+                ("c2", ctypes.c_double), ("c3", ctypes.c_double), ("mu0", ctypes.c_double)]
+
+
+class NsimOptions(ctypes.Structure):
+    _fields_ = [("precision", ctypes.c_int), ("do_demag", ctypes.c_int),
+                ("nbc_rtol", ctypes.c_double), ("dbc_rtol", ctypes.c_double),
+                ("nbc_maxits", ctypes.c_int), ("dbc_maxits", ctypes.c_int)]
+
+
+# name -> (restype, argtypes); every symbol include/nsim.h declares
+SIGNATURES = {
+    "nsim_create": (ctypes.c_int, [ctypes.POINTER(NsimMesh), ctypes.POINTER(NsimMaterial),
+                                   ctypes.POINTER(NsimOptions), ctypes.POINTER(_vp)]),
+    "nsim_destroy": (None, [_vp]),
+    "nsim_last_error": (ctypes.c_char_p, []),
+    "nsim_get_sizes": (ctypes.c_int, [_vp, _ip]),
+    "nsim_set_field": (ctypes.c_int, [_vp, ctypes.c_char_p, _dp]),
+    "nsim_get_field": (ctypes.c_int, [_vp, ctypes.c_char_p, _dp]),
+    "nsim_execute": (ctypes.c_int, [_vp, ctypes.c_char_p]),
+    "nsim_get_bem": (ctypes.c_int, [_vp, _dp]),
+    "nsim_rhs": (ctypes.c_int, [_vp, ctypes.c_double, _vp, _vp, ctypes.c_int]),
+    "nsim_jtimes": (ctypes.c_int, [_vp, ctypes.c_double, _vp, _vp, _vp, ctypes.c_int]),
+    "nsim_get_stat": (ctypes.c_int, [_vp, ctypes.c_char_p, _dp]),
+    "nsim_timers_enable": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "nsim_get_timer": (ctypes.c_int, [_vp, ctypes.c_char_p, _dp]),
+    "nsim_set_tuning": (ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.c_int]),
+    "nsim_synchronize": (ctypes.c_int, [_vp]),
+}
+
+_lib = None
+
+
+def lib():
+    """Load libnsim_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(
+                "libnsim_hip.so is not built (run __graft_entry__.build() or "
+                "make -C nmag-src_amd/csrc); there is no fallback compute path")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+class NsimError(RuntimeError):
+    pass
+
+
+def _check(rc, what):
+    if rc != 0:
+        msg = lib().nsim_last_error()
+        raise NsimError("%s failed (%d): %s" % (what, rc, msg.decode() if msg else ""))
+
+
+def _dptr(a):
+    return a.ctypes.data_as(_dp)
+
+
+class NativeMagnet:
+    """One magnet on the GPU: operators, fields and solver state in HBM."""
+
+    VEC3 = ("m", "H_ext", "axis", "H_exch", "H_demag", "H_anis", "H_total", "dmdt")
+    SCALAR = ("K1", "K2", "pin", "rho", "phi", "phi1", "phi2", "volumes")
+
+    def __init__(self, coords, tets, material, precision="fp64", do_demag=True,
+                 nbc_rtol=1e-5, dbc_rtol=1e-5, nbc_maxits=10000, dbc_maxits=10000):
+        L = lib()
+        self._coords = np.ascontiguousarray(coords, dtype=np.float64).reshape(-1, 3)
+        self._tets = np.ascontiguousarray(tets, dtype=np.int32).reshape(-1, 4)
+        mesh = NsimMesh(len(self._coords), len(self._tets), _dptr(self._coords),
+                        self._tets.ctypes.data_as(_ip))
+        mat = NsimMaterial(*[float(material[k]) for k in ("Ms", "negJ", "c1", "c2", "c3", "mu0")])
+        prec = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+        opt = NsimOptions(prec, 1 if do_demag else 0, nbc_rtol, dbc_rtol, nbc_maxits, dbc_maxits)
+        h = _vp()
+        self._h = None
+        _check(L.nsim_create(ctypes.byref(mesh), ctypes.byref(mat), ctypes.byref(opt),
+                             ctypes.byref(h)), "nsim_create")
+        self._h = h
+        s = np.zeros(5, dtype=np.int32)
+        _check(L.nsim_get_sizes(h, s.ctypes.data_as(_ip)), "nsim_get_sizes")
+        self.n, self.nb, self.ntri, self.ntet, self.slots = [int(v) for v in s]
+        self.precision = precision
+
+    def close(self):
+        if self._h is not None and _lib is not None:
+            _lib.nsim_destroy(self._h)
+        self._h = None
+
+    def __del__(self):
+        self.close()
+
+    @property
+    def handle(self):
+        return self._h
+
+    def _size(self, name):
+        if name in self.VEC3:
+            return (self.n, 3)
+        if name == "solid_angles" or name == "boundary_nodes":
+            return (self.nb,)
+        return (self.n,)
+
+    def set_field(self, name, data):
+        a = np.ascontiguousarray(np.broadcast_to(np.asarray(data, dtype=np.float64),
+                                                 self._size(name)))
+        _check(lib().nsim_set_field(self._h, name.encode(), _dptr(a)), "nsim_set_field(%s)" % name)
+
+    def get_field(self, name):
+        out = np.zeros(self._size(name))
+        _check(lib().nsim_get_field(self._h, name.encode(), _dptr(out)), "nsim_get_field(%s)" % name)
+        return out
+
+    def execute(self, program="update_dmdt"):
+        _check(lib().nsim_execute(self._h, program.encode()), "nsim_execute(%s)" % program)
+
+    def rhs(self, y, t=0.0):
+        y = np.ascontiguousarray(y, dtype=np.float64).reshape(-1)
+        yd = np.zeros_like(y)
+        _check(lib().nsim_rhs(self._h, t, y.ctypes.data, yd.ctypes.data, 0), "nsim_rhs")
+        return yd
+
+    def rhs_device(self, y_ptr, ydot_ptr, t=0.0):
+        _check(lib().nsim_rhs(self._h, t, y_ptr, ydot_ptr, 1), "nsim_rhs")
+
+    def jtimes(self, y, v, t=0.0):
+        y = np.ascontiguousarray(y, dtype=np.float64).reshape(-1)
+        v = np.ascontiguousarray(v, dtype=np.float64).reshape(-1)
+        jv = np.zeros_like(v)
+        _check(lib().nsim_jtimes(self._h, t, v.ctypes.data, jv.ctypes.data, y.ctypes.data, 0),
+               "nsim_jtimes")
+        return jv
+
+    def bem(self):
+        B = np.zeros((self.nb, self.nb))
+        _check(lib().nsim_get_bem(self._h, _dptr(B)), "nsim_get_bem")
+        return B
+
+    def stat(self, name):
+        v = ctypes.c_double(0)
+        _check(lib().nsim_get_stat(self._h, name.encode(), ctypes.byref(v)), "nsim_get_stat")
+        return v.value
+
+    def timers(self, on=True):
+        _check(lib().nsim_timers_enable(self._h, 1 if on else 0), "nsim_timers_enable")
+
+    def timer(self, name):
+        out = np.zeros(2)
+        _check(lib().nsim_get_timer(self._h, name.encode(), _dptr(out)), "nsim_get_timer")
+        return float(out[0]), int(out[1])
+
+    def tune(self, key, value):
+        _check(lib().nsim_set_tuning(self._h, key.encode(), int(value)), "nsim_set_tuning")
+
+    def synchronize(self):
+        _check(lib().nsim_synchronize(self._h), "nsim_synchronize")
