@@ -1,0 +1,282 @@
+// Dense Fredkin-Koehler boundary matrix: device assembly (Lindholm weights) and
+// the per-evaluation GEMV phi2|_b = B phi1|_b (DM*V BEM, nmag_lam.py:749).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+
+#include "nsim_dev_common.h"
+#include "nsim_device.h"
+
+namespace nsim {
+
+namespace {
+
+using namespace dev;
+
+// Lindholm weight of vertex slot `a` of triangle (p0,p1,p2) seen from observer
+// o (src/bem3d_stubs.c:413-610, identity lattice transform, min_dist 0).
+__device__ double lindholm_weight(const double* o, const double* p0, const double* p1,
+                                  const double* p2, const double* nout, double geo_eps,
+                                  int a) {
+  double s0[3], s1[3], s2[3], r0[3], r1[3], r2[3];
+  for (int d = 0; d < 3; d++) {
+    s0[d] = p2[d] - p1[d];
+    s1[d] = p0[d] - p2[d];
+    s2[d] = p1[d] - p0[d];
+    r0[d] = p0[d] - o[d];
+    r1[d] = p1[d] - o[d];
+    r2[d] = p2[d] - o[d];
+  }
+  const double r0l = sqrt(dot3(r0, r0)), r1l = sqrt(dot3(r1, r1)), r2l = sqrt(dot3(r2, r2));
+  if (r0l <= 0.0 || r1l <= 0.0 || r2l <= 0.0) return 0.0;  // observer on a vertex
+  const double s0l = sqrt(dot3(s0, s0)), s1l = sqrt(dot3(s1, s1)), s2l = sqrt(dot3(s2, s2));
+  double zv[3];
+  cross(s1, s2, zv);
+  const double zl = sqrt(dot3(zv, zv));
+  if (zl == 0.0 || s0l == 0.0 || s1l == 0.0 || s2l == 0.0) return 0.0;
+  for (int d = 0; d < 3; d++) zv[d] /= zl;
+  const double area = 0.5 * zl;
+  double xi0[3], xi1[3], xi2[3];
+  for (int d = 0; d < 3; d++) {
+    xi0[d] = s0[d] / s0l;
+    xi1[d] = s1[d] / s1l;
+    xi2[d] = s2[d] / s2l;
+  }
+  const double c01 = dot3(xi0, xi1), c12 = dot3(xi1, xi2), c20 = dot3(xi2, xi0);
+  const double lg0 = log((r1l + r2l + s0l) / (r1l + r2l - s0l));
+  const double lg1 = log((r2l + r0l + s1l) / (r2l + r0l - s1l));
+  const double lg2 = log((r0l + r1l + s2l) / (r0l + r1l - s2l));
+  const double sp01 = dot3(r0, r1), sp12 = dot3(r1, r2), sp20 = dot3(r2, r0);
+  const double num = r0l * r1l * r2l + r0l * sp12 + r1l * sp20 + r2l * sp01;
+  const double den =
+      sqrt(2.0 * (r1l * r2l + sp12) * (r2l * r0l + sp20) * (r0l * r1l + sp01));
+  if (den == 0.0) return 0.0;
+  // The reference's 2*acos(num/den)*sign(zeta) equals 2*atan2(r0.(r1 x r2), num)
+  // (den^2 = num^2 + triple^2, zeta ~ triple); atan2 keeps the small angles of
+  // distant triangles to full precision where acos near 1 loses half the digits.
+  double r1xr2[3];
+  cross(r1, r2, r1xr2);
+  const double angle = 2.0 * atan2(dot3(r0, r1xr2), num);
+  const double zeta = dot3(zv, r0);
+  double f = 1.0 / (8.0 * kPi * area);
+  if (dot3(zv, nout) < 0.0) f = -f;
+  if (fabs(zeta) < geo_eps) f = 0.0;
+  double eta[3], res;
+  if (a == 0) {
+    cross(zv, xi0, eta);
+    res = f * s0l * (angle * dot3(eta, r1) - zeta * (lg0 + c01 * lg1 + c20 * lg2));
+  } else if (a == 1) {
+    cross(zv, xi1, eta);
+    res = f * s1l * (angle * dot3(eta, r2) - zeta * (c01 * lg0 + lg1 + c12 * lg2));
+  } else {
+    cross(zv, xi2, eta);
+    res = f * s2l * (angle * dot3(eta, r0) - zeta * (c20 * lg0 + c12 * lg1 + lg2));
+  }
+  return res;
+}
+
+__device__ inline uint16_t f32_to_bf16(float f) {
+  uint32_t u = __float_as_uint(f);
+  if ((u & 0x7f800000u) == 0x7f800000u && (u & 0x7fffffu)) return 0x7fc0;
+  u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even
+  return (uint16_t)(u >> 16);
+}
+
+template <typename TB>
+__device__ inline void store_b(TB* p, double v);
+template <>
+__device__ inline void store_b<double>(double* p, double v) {
+  *p = v;
+}
+template <>
+__device__ inline void store_b<float>(float* p, double v) {
+  *p = (float)v;
+}
+template <>
+__device__ inline void store_b<uint16_t>(uint16_t* p, double v) {
+  *p = f32_to_bf16((float)v);
+}
+
+// One thread per matrix entry (i, j): B_ij = sum over the triangles incident to
+// boundary node j of j's Lindholm weight seen from node i, plus the diagonal
+// Omega_i/(4 pi) - 1 (src/bem3d.ml:938-1046).  Deterministic; writes coalesced.
+template <typename TB>
+__global__ __launch_bounds__(256) void k_bem_assemble(int nb, int ldb, int col_blocks,
+                                                      const double* __restrict__ x,
+                                                      const int* __restrict__ bnode,
+                                                      const int* __restrict__ tri_node,
+                                                      const double* __restrict__ tnorm,
+                                                      const int* __restrict__ inc_rp,
+                                                      const int* __restrict__ inc_tri,
+                                                      const double* __restrict__ omega,
+                                                      double geo_eps, TB* __restrict__ B) {
+  const int i = blockIdx.x / col_blocks;
+  const int j = (blockIdx.x % col_blocks) * 256 + threadIdx.x;
+  if (i >= nb || j >= ldb) return;
+  double v = 0.0;
+  if (j < nb) {
+    const int oi = bnode[i];
+    const double o[3] = {x[3 * oi], x[3 * oi + 1], x[3 * oi + 2]};
+    for (int k = inc_rp[j]; k < inc_rp[j + 1]; k++) {
+      const int t = inc_tri[k] / 3, a = inc_tri[k] % 3;
+      const int v0 = tri_node[3 * t], v1 = tri_node[3 * t + 1], v2 = tri_node[3 * t + 2];
+      const double p0[3] = {x[3 * v0], x[3 * v0 + 1], x[3 * v0 + 2]};
+      const double p1[3] = {x[3 * v1], x[3 * v1 + 1], x[3 * v1 + 2]};
+      const double p2[3] = {x[3 * v2], x[3 * v2 + 1], x[3 * v2 + 2]};
+      v += lindholm_weight(o, p0, p1, p2, tnorm + 3 * t, geo_eps, a);
+    }
+    if (i == j) v += omega[i] / (4.0 * kPi) - 1.0;
+  }
+  store_b<TB>(B + (size_t)i * ldb + j, v);
+}
+
+// Storage type TB of B, type TX of the LDS-staged input vector, accumulation
+// type TA.  FP32 storage is accumulated in FP64 (the kernel is HBM-bound, the
+// FP64 FMAs are free); BF16 storage uses FP32 inputs and accumulation.
+template <typename TB>
+struct BemVec;
+template <>
+struct BemVec<double> {
+  static constexpr int V = 2;  // 16 B per lane per load
+  using X = double;
+  using A = double;
+};
+template <>
+struct BemVec<float> {
+  static constexpr int V = 4;
+  using X = double;
+  using A = double;
+};
+template <>
+struct BemVec<uint16_t> {
+  static constexpr int V = 8;
+  using X = float;
+  using A = float;
+};
+
+template <typename TB, typename X>
+__device__ inline void load_row_chunk(const TB* p, X* out);
+template <>
+__device__ inline void load_row_chunk<double, double>(const double* p, double* out) {
+  const double2 v = *reinterpret_cast<const double2*>(p);
+  out[0] = v.x;
+  out[1] = v.y;
+}
+template <>
+__device__ inline void load_row_chunk<float, double>(const float* p, double* out) {
+  const float4 v = *reinterpret_cast<const float4*>(p);
+  out[0] = v.x;
+  out[1] = v.y;
+  out[2] = v.z;
+  out[3] = v.w;
+}
+template <>
+__device__ inline void load_row_chunk<uint16_t, float>(const uint16_t* p, float* out) {
+  const uint4 v = *reinterpret_cast<const uint4*>(p);
+  const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    out[2 * k] = __uint_as_float(w[k] << 16);
+    out[2 * k + 1] = __uint_as_float(w[k] & 0xffff0000u);
+  }
+}
+
+// phi2[bnode[i]] = sum_j B_ij phi1b_j.  The input vector is staged once per
+// workgroup in LDS; each wavefront streams ROWS rows of B at a time with
+// 16-byte loads per lane and reduces with cross-lane adds.  Rows are split
+// evenly over all wavefronts of the grid.
+template <typename TB, int ROWS>
+__global__ __launch_bounds__(256) void k_bem_gemv(const TB* __restrict__ B, int ldb, int nb,
+                                                  int rows_per_wave,
+                                                  const double* __restrict__ xb,
+                                                  const int* __restrict__ bnode,
+                                                  double* __restrict__ y) {
+  using X = typename BemVec<TB>::X;
+  using A = typename BemVec<TB>::A;
+  constexpr int V = BemVec<TB>::V;
+  extern __shared__ __align__(16) unsigned char smem[];
+  X* xs = reinterpret_cast<X*>(smem);
+  for (int j = threadIdx.x; j < ldb; j += 256) xs[j] = j < nb ? (X)xb[j] : (X)0;
+  __syncthreads();
+  const int wave = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const int r0 = wave * rows_per_wave;
+  const int r1 = min(r0 + rows_per_wave, nb);
+  for (int rb = r0; rb < r1; rb += ROWS) {
+    const TB* rp[ROWS];
+    A acc[ROWS];
+#pragma unroll
+    for (int k = 0; k < ROWS; k++) {
+      rp[k] = B + (size_t)min(rb + k, nb - 1) * ldb;
+      acc[k] = 0;
+    }
+#pragma unroll 4
+    for (int c = lane * V; c < ldb; c += 64 * V) {
+      X xv[V];
+#pragma unroll
+      for (int e = 0; e < V; e++) xv[e] = xs[c + e];
+#pragma unroll
+      for (int k = 0; k < ROWS; k++) {
+        X bv[V];
+        load_row_chunk<TB, X>(rp[k] + c, bv);
+#pragma unroll
+        for (int e = 0; e < V; e++) acc[k] += (A)bv[e] * (A)xv[e];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < ROWS; k++) {
+      const double t = wave_sum((double)acc[k]);
+      if (lane == 0 && rb + k < r1) y[bnode[rb + k]] = t;
+    }
+  }
+}
+
+template <typename TB>
+hipError_t gemv_typed(const DeviceOps& d, const double* phi1b, double* phi2, hipStream_t st,
+                      int num_cus) {
+  using X = typename BemVec<TB>::X;
+  constexpr int ROWS = 2;
+  const size_t lds = (size_t)d.ldb * sizeof(X);
+  if (lds > 64 * 1024) return hipErrorInvalidValue;
+  // two workgroups per CU, rows split evenly over the wavefronts
+  const int waves = 4 * 2 * (num_cus > 0 ? num_cus : 256);
+  int rpw = (d.nb + waves - 1) / waves;
+  rpw = ((rpw + ROWS - 1) / ROWS) * ROWS;
+  const int used_waves = (d.nb + rpw - 1) / rpw;
+  const int grid = (used_waves + 3) / 4;
+  k_bem_gemv<TB, ROWS><<<grid, 256, lds, st>>>((const TB*)d.B, d.ldb, d.nb, rpw, phi1b,
+                                               d.bnode, phi2);
+  return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch_bem_assemble(const DeviceOps& d, hipStream_t st) {
+  const int cb = (d.ldb + 255) / 256;
+  const long long nblk = (long long)cb * d.nb;
+  if (nblk <= 0) return hipSuccess;
+  if (nblk > 0x7fffffffLL) return hipErrorInvalidValue;
+  if (d.btype == kBemF64)
+    k_bem_assemble<double><<<(int)nblk, 256, 0, st>>>(
+        d.nb, d.ldb, cb, d.coords, d.bnode, d.tri_node, d.tnorm, d.inc_rp, d.inc_tri,
+        d.omega, d.geo_eps, (double*)d.B);
+  else if (d.btype == kBemF32)
+    k_bem_assemble<float><<<(int)nblk, 256, 0, st>>>(
+        d.nb, d.ldb, cb, d.coords, d.bnode, d.tri_node, d.tnorm, d.inc_rp, d.inc_tri,
+        d.omega, d.geo_eps, (float*)d.B);
+  else
+    k_bem_assemble<uint16_t><<<(int)nblk, 256, 0, st>>>(
+        d.nb, d.ldb, cb, d.coords, d.bnode, d.tri_node, d.tnorm, d.inc_rp, d.inc_tri,
+        d.omega, d.geo_eps, (uint16_t*)d.B);
+  return hipGetLastError();
+}
+
+hipError_t launch_bem_gemv(const DeviceOps& d, const double* phi1b, double* phi2,
+                           hipStream_t st, int num_cus) {
+  if (d.nb == 0) return hipSuccess;
+  if (d.btype == kBemF64) return gemv_typed<double>(d, phi1b, phi2, st, num_cus);
+  if (d.btype == kBemF32) return gemv_typed<float>(d, phi1b, phi2, st, num_cus);
+  return gemv_typed<uint16_t>(d, phi1b, phi2, st, num_cus);
+}
+
+}  // namespace nsim

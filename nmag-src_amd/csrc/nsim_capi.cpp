@@ -151,7 +151,7 @@ int device_rhs(nsim_handle* h, const double* m, double* dmdt) {
     HIPCHK(launch_exch_div(h->d, h->mixed, h->coef, m, h->Hexch, h->rho, demag, h->stream));
   }
   if (demag) {
-    CgWork w = h->cgw;
+    CgWork& w = h->cgw;
     {
       Span s(h, "cg_nbc");
       w.sync = h->sync_nbc;
@@ -342,7 +342,9 @@ int nsim_create(const nsim_mesh* mesh, const nsim_material* mat, const nsim_opti
   h->cg_grid = grid;
   h->cgw.gmax = std::max(grid, 1024);
   ZEROS(h->cgw.pub, 2 * n);
-  ZEROS(h->cgw.partial, 8 * (size_t)h->cgw.gmax);
+  ZEROS(h->cgw.pa, 2 * (size_t)h->cgw.gmax);
+  ZEROS(h->cgw.pb, 2 * (size_t)h->cgw.gmax);
+  ZEROS(h->cgw.tag, 2 * (size_t)h->cgw.gmax);
   ZEROS(h->sync_nbc, 8);
   ZEROS(h->sync_dbc, 8);
   h->cgw.sync = h->sync_nbc;

@@ -39,11 +39,14 @@ struct DeviceOps {
 
 // Communication buffers of the persistent CG kernel.
 struct CgWork {
-  double* pub = nullptr;      // 2*n published vector (double buffered)
-  double* partial = nullptr;  // 2 * 4 * gmax per-workgroup partial sums
-  unsigned* sync = nullptr;   // [0] barrier counter [1] exit counter [2] error flag
-                              // [3] last iterations [4] total iterations [5] solves
+  double* pub = nullptr;            // 2*n published vector (by phase parity)
+  double* pa = nullptr;             // 2*gmax per-workgroup partial sums
+  double* pb = nullptr;
+  unsigned long long* tag = nullptr;  // 2*gmax phase tags
+  unsigned* sync = nullptr;         // [2] error flag [3] last iterations
+                                    // [4] total iterations [5] solves
   int gmax = 0;
+  unsigned long long epoch = 0;     // launch counter; makes every phase tag unique
 };
 
 struct Coeffs {
@@ -69,7 +72,7 @@ hipError_t launch_bem_assemble(const DeviceOps& d, hipStream_t st);
 hipError_t launch_exch_div(const DeviceOps& d, bool mixed, const Coeffs& c, const double* m,
                            double* Hexch, double* rho, bool want_rho, hipStream_t st);
 // Persistent pipelined Jacobi-PCG on the stiffness matrix; see the kernel.
-hipError_t launch_cg(const DeviceOps& d, bool mixed, const CgWork& w, int grid, int rpt,
+hipError_t launch_cg(const DeviceOps& d, bool mixed, CgWork& w, int grid, int rpt,
                      const double* dinv, const double* b, double* x, double rtol, int maxit,
                      const double* xadd, double* xsum, double* restrict_out,
                      hipStream_t st);

@@ -45,6 +45,7 @@ def lib():
         L.orc_get_bem.argtypes = [ctypes.c_void_p, _dp]
         L.orc_kuhn_cuboid.argtypes = [ctypes.c_int] * 3 + [ctypes.c_double, _dp, _ip]
         L.orc_lindholm.argtypes = [_dp] * 5 + [ctypes.c_double, _dp]
+        L.orc_lindholm_acos.argtypes = [_dp] * 5 + [ctypes.c_double, _dp]
         L.orc_vertex_solid_angle.restype = ctypes.c_double
         L.orc_vertex_solid_angle.argtypes = [_dp] * 4
         L.orc_fields.argtypes = ([ctypes.c_void_p, ctypes.POINTER(OrcParams)] + [_dp] * 5
@@ -75,10 +76,11 @@ def kuhn_cuboid(nx, ny, nz, h):
     return coords, tets
 
 
-def lindholm(observer, p0, p1, p2, normal, geo_eps=0.0):
+def lindholm(observer, p0, p1, p2, normal, geo_eps=0.0, acos_form=False):
     out = np.zeros(3)
     args = [np.ascontiguousarray(v, dtype=np.float64) for v in (observer, p0, p1, p2, normal)]
-    lib().orc_lindholm(*[_p(a) for a in args], geo_eps, _p(out))
+    fn = lib().orc_lindholm_acos if acos_form else lib().orc_lindholm
+    fn(*[_p(a) for a in args], geo_eps, _p(out))
     return out
 
 
@@ -90,11 +92,12 @@ def vertex_solid_angle(x, p0, p1, p2):
 class OracleMesh:
     """Assembled operators of one tetrahedral mesh (FP64, host)."""
 
-    def __init__(self, coords, tets, build_bem=True):
+    def __init__(self, coords, tets, build_bem=True, acos_form=False):
         self.coords = np.ascontiguousarray(coords, dtype=np.float64)
         self.tets = np.ascontiguousarray(tets, dtype=np.int32)
         self.h = lib().orc_setup(len(self.coords), _p(self.coords), len(self.tets),
-                                 _p(self.tets, np.int32), 1 if build_bem else 0)
+                                 _p(self.tets, np.int32),
+                                 (2 if acos_form else 1) if build_bem else 0)
         s = np.zeros(5, dtype=np.int32)
         lib().orc_sizes(self.h, _p(s, np.int32))
         self.n, self.nnz, self.nb, self.ntri, self.ntet = [int(v) for v in s]

@@ -120,9 +120,15 @@ double orc_vertex_solid_angle(const double* x, const double* p0, const double* p
 }
 
 /* Lindholm linear-shape double-layer weights: src/bem3d_stubs.c:413-610
- * (identity lattice transform, grey factor 1, min_dist 0). */
-void orc_lindholm(const double* obs, const double* p0, const double* p1,
-                  const double* p2, const double* n_out, double geo_eps, double* lh) {
+ * (identity lattice transform, grey factor 1, min_dist 0).
+ * The triangle's solid angle is 2*acos(num/den)*sign(zeta) in the reference.
+ * Since den^2 = num^2 + (r0.(r1 x r2))^2 and zeta has the sign of r0.(r1 x r2),
+ * that equals 2*atan2(r0.(r1 x r2), num), which keeps full precision for the
+ * small angles of distant triangles (acos near 1 loses about half the digits).
+ * acos_form != 0 selects the reference's literal acos form. */
+static void lindholm_impl(const double* obs, const double* p0, const double* p1,
+                          const double* p2, const double* n_out, double geo_eps,
+                          double* lh, int acos_form) {
   lh[0] = lh[1] = lh[2] = 0.0;
   double s0[3], s1[3], s2[3], xi0[3], xi1[3], xi2[3], zv[3];
   sub3(s0, p2, p1); sub3(s1, p0, p2); sub3(s2, p1, p0);
@@ -152,12 +158,19 @@ void orc_lindholm(const double* obs, const double* p0, const double* p1,
   double num = r0l * r1l * r2l + r0l * sp12 + r1l * sp20 + r2l * sp01;
   double den = sqrt(2.0 * (r1l * r2l + sp12) * (r2l * r0l + sp20) * (r0l * r1l + sp01));
   if (den == 0.0) return;
-  double q = num / den;
-  if (q < -1.0) q = -1.0;
-  if (q > 1.0) q = 1.0;
-  double angle = 2.0 * acos(q);
   double zeta = dot3(zv, r0);
-  if (zeta < 0.0) angle = -angle;
+  double angle;
+  if (acos_form) {
+    double q = num / den;
+    if (q < -1.0) q = -1.0;
+    if (q > 1.0) q = 1.0;
+    angle = 2.0 * acos(q);
+    if (zeta < 0.0) angle = -angle;
+  } else {
+    double r1xr2[3];
+    cross3(r1xr2, r1, r2);
+    angle = 2.0 * atan2(dot3(r0, r1xr2), num);
+  }
   double f = 1.0 / (8.0 * PI * area);
   if (dot3(zv, n_out) < 0.0) f = -f;
   if (fabs(zeta) < geo_eps) f = 0.0;
@@ -167,6 +180,16 @@ void orc_lindholm(const double* obs, const double* p0, const double* p1,
   lh[0] = f * s0l * (angle * dot3(eta0, r1) - zeta * g0);
   lh[1] = f * s1l * (angle * dot3(eta1, r2) - zeta * g1);
   lh[2] = f * s2l * (angle * dot3(eta2, r0) - zeta * g2);
+}
+
+void orc_lindholm(const double* obs, const double* p0, const double* p1,
+                  const double* p2, const double* n_out, double geo_eps, double* lh) {
+  lindholm_impl(obs, p0, p1, p2, n_out, geo_eps, lh, 0);
+}
+
+void orc_lindholm_acos(const double* obs, const double* p0, const double* p1,
+                       const double* p2, const double* n_out, double geo_eps, double* lh) {
+  lindholm_impl(obs, p0, p1, p2, n_out, geo_eps, lh, 1);
 }
 
 /* ------------------------------------------------------------- set-up */
@@ -202,7 +225,7 @@ static int csr_find(const orc_mesh* m, int i, int j) {
   return -1;
 }
 
-static void build_bem(orc_mesh* m) {
+static void build_bem(orc_mesh* m, int acos_form) {
   int nb = m->nb;
   m->B = (double*)calloc((size_t)nb * nb, sizeof(double));
   /* Row fill over every surface triangle (src/bem3d.ml:938-1046); rows are
@@ -214,8 +237,8 @@ static void build_bem(orc_mesh* m) {
     for (int t = 0; t < m->ntri; t++) {
       const int* tv = m->tri + 3 * t;
       double lh[3];
-      orc_lindholm(xo, m->x + 3 * m->bnode[tv[0]], m->x + 3 * m->bnode[tv[1]],
-                   m->x + 3 * m->bnode[tv[2]], m->tnorm + 3 * t, m->geo_eps, lh);
+      lindholm_impl(xo, m->x + 3 * m->bnode[tv[0]], m->x + 3 * m->bnode[tv[1]],
+                    m->x + 3 * m->bnode[tv[2]], m->tnorm + 3 * t, m->geo_eps, lh, acos_form);
       row[tv[0]] += lh[0];
       row[tv[1]] += lh[1];
       row[tv[2]] += lh[2];
@@ -349,7 +372,7 @@ orc_mesh* orc_setup(int n, const double* coords, int T, const int* tets, int bui
                                              coords + 3 * o[1], coords + 3 * o[2]);
     }
   }
-  if (build_b) build_bem(m);
+  if (build_b) build_bem(m, build_b == 2);
   return m;
 }
 

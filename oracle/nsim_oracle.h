@@ -45,6 +45,8 @@ typedef struct {
 /* Structured cuboid, every cube split into 6 Kuhn tetrahedra. */
 int orc_kuhn_cuboid(int nx, int ny, int nz, double h, double* coords, int* tets);
 
+/* build_bem: 0 none, 1 dense BEM (atan2 solid angle), 2 dense BEM with the
+ * reference's literal acos solid-angle form. */
 orc_mesh* orc_setup(int n_nodes, const double* coords, int n_tets, const int* tets,
                     int build_bem);
 void orc_free(orc_mesh* m);
@@ -57,10 +59,14 @@ void orc_get_boundary(const orc_mesh* m, int* bnodes, int* tris, double* normals
                       double* solid_angles);
 void orc_get_bem(const orc_mesh* m, double* B);
 
-/* Lindholm weights of one triangle seen from one observer (reference formula). */
+/* Lindholm weights of one triangle seen from one observer (reference formula;
+ * _acos keeps the reference's literal acos solid-angle form). */
 void orc_lindholm(const double* observer, const double* p0, const double* p1,
                   const double* p2, const double* outward_normal, double geo_eps,
                   double* lh3);
+void orc_lindholm_acos(const double* observer, const double* p0, const double* p1,
+                       const double* p2, const double* outward_normal, double geo_eps,
+                       double* lh3);
 double orc_vertex_solid_angle(const double* x, const double* p0, const double* p1,
                               const double* p2);
 

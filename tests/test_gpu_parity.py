@@ -84,10 +84,12 @@ def test_rhs_host_pointers_and_jtimes():
     assert rel_err(jv, jref) < 1e-10
 
 
-@pytest.mark.parametrize("precision,tol_exch,tol_demag", [("fp32", 1e-6, 1e-5),
-                                                           ("bf16", 1e-6, 2e-2)])
+@pytest.mark.parametrize("precision,tol_exch,tol_demag", [("fp32", 1e-7, 1e-6),
+                                                           ("bf16", 1e-7, 2e-2)])
 def test_mixed_precision_field_error(precision, tol_exch, tol_demag):
-    _, _, orc, gpu, mat = make_pair(12, 12, 4, 2.5, precision=precision, rtol=1e-7)
+    """FP32 (BF16) operator storage: the per-field error against the FP64 oracle
+    is that of rounding the stored operators; solves run to 1e-10."""
+    _, _, orc, gpu, mat = make_pair(12, 12, 4, 2.5, precision=precision, rtol=1e-10)
     m = random_m(orc.n, 1)
     ref = orc.fields(orc_params(mat), m, rtol=1e-13)
     got = _fields_gpu(gpu, m)

@@ -195,6 +195,22 @@ def test_bem_row_sums(dims):
     assert np.allclose(B.sum(axis=1), -1.0, atol=1e-10)
 
 
+def test_atan2_solid_angle_form_is_the_reference_acos_form_without_its_noise():
+    """The literal acos form of the reference and the atan2 form used here are
+    the same function; the acos form carries ~1e-9 absolute noise on the
+    40x20x2 mesh (acos near 1), the atan2 form keeps B 1 = -1 to ~1e-14."""
+    c, t = oracle.kuhn_cuboid(40, 20, 2, 2.5)
+    A = oracle.OracleMesh(c, t).bem()
+    R = oracle.OracleMesh(c, t, acos_form=True).bem()
+    assert np.abs(A - R).max() < 5e-9
+    assert np.abs(A.sum(axis=1) + 1).max() < 1e-13
+    x, p0, p1, p2 = (np.array([5.0, 3.0, 40.0]), np.zeros(3), np.array([2.0, 0.0, 0.0]),
+                     np.array([0.0, 2.0, 0.0]))
+    n = np.array([0.0, 0.0, 1.0])
+    ref = _quad_double_layer(x, p0, p1, p2, n)
+    assert np.allclose(oracle.lindholm(x, p0, p1, p2, n), ref, rtol=1e-10)
+
+
 def test_uniform_cuboid_demag_factor_against_aharoni():
     M = oracle.OracleMesh(*oracle.kuhn_cuboid(12, 12, 4, 2.5))
     mat = material()
