@@ -114,7 +114,7 @@ def run_gpu(prec, coords, tets, mat, Hext, states, warmup, steps, rtol, torch):
         "cg_iters_nbc": (it1[0] - it0[0]) / steps,
         "cg_iters_dbc": (it1[1] - it0[1]) / steps,
         "kernel_ms": kern,
-        "nb": g.nb, "n": g.n,
+        "nb": g.nb, "n": g.n, "nnz": g.nnz,
         "cg_grid": g.stat("cg_grid"),
     }
     g.close()
@@ -188,19 +188,28 @@ def main():
     err = float(np.max(np.abs(outs["mixed"] - outs["fp64"])) / np.max(np.abs(outs["fp64"])))
 
     mixed = results["mixed"]
-    nb = mixed["nb"]
+    nb, n, nnz = mixed["nb"], mixed["n"], mixed["nnz"]
     kern = mixed["kernel_ms"]
     dominant = max(kern, key=kern.get)
-    if dominant == "bem":
-        alg = gemv_bytes(nb, 4)
-    else:
-        alg = None
-    roof = None
-    if alg is not None:
-        ach = alg / (kern["bem"] * 1e-3) / 1e9
-        roof = {"bound": "hbm", "kernel": "k_bem_gemv<float>", "achieved": round(ach, 1),
+
+    def roofline(kernel, alg_bytes, ms):
+        ach = alg_bytes / (ms * 1e-3) / 1e9
+        return {"bound": "hbm", "kernel": kernel, "achieved": round(ach, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
-                "traffic": None}
+                "traffic": None, "alg_bytes_per_launch": int(alg_bytes)}
+
+    # per-iteration algorithmic bytes of one PCG iteration (SpMV over the
+    # stiffness pattern with FP32 values + 4-byte columns, row pointers, and
+    # about eight FP64 vector passes), times the measured iterations per solve
+    cg_iter_bytes = 8 * nnz + 4 * (n + 1) + 64 * n
+    rooflines = {
+        "cg_nbc": roofline("k_cg<float> Neumann solve",
+                           cg_iter_bytes * max(mixed["cg_iters_nbc"], 1), kern["cg_nbc"]),
+        "cg_dbc": roofline("k_cg<float> Dirichlet solve",
+                           cg_iter_bytes * max(mixed["cg_iters_dbc"], 1), kern["cg_dbc"]),
+        "bem": roofline("k_bem_gemv<float>", gemv_bytes(nb, 4), kern["bem"]),
+    }
+    roof = rooflines.get(dominant)
     bem64 = results["fp64"]["kernel_ms"]["bem"]
     bem_fp64_gbs = gemv_bytes(nb, 8) / (bem64 * 1e-3) / 1e9
 
@@ -242,6 +251,7 @@ def main():
         "mixed_dmdt_max_rel_err": err,
         "dominant_kernel": dominant,
         "roofline": roof,
+        "roofline_all": rooflines,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

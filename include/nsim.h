@@ -64,8 +64,8 @@ int nsim_create(const nsim_mesh* mesh, const nsim_material* mat,
 void nsim_destroy(nsim_handle* h);
 const char* nsim_last_error(void);
 
-/* sizes[0..4] = n_nodes, n_boundary_nodes, n_boundary_triangles, n_tets,
- * stored operator entries (padded SELL slots). */
+/* sizes[0..5] = n_nodes, n_boundary_nodes, n_boundary_triangles, n_tets,
+ * stored operator entries (padded SELL slots), stiffness non-zeros. */
 int nsim_get_sizes(nsim_handle* h, int* sizes);
 
 /* Replaces lam_set_field / lam_get_field (src/nsimexec.ml:4286-4305).

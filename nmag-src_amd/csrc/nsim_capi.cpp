@@ -73,7 +73,7 @@ struct nsim_handle {
   bool mixed = false;
   int device = 0, num_cus = 0;
   hipStream_t stream = nullptr;
-  int cg_grid = 0, cg_rpt = 1;
+  int cg_grid = 0, cg_rpt = 1, cg_maxw = 0;
   // field vectors (site-major 3-vectors or scalars)
   double *m = nullptr, *Hext = nullptr, *k1 = nullptr, *k2 = nullptr, *axis = nullptr,
          *pin = nullptr;
@@ -155,7 +155,7 @@ int device_rhs(nsim_handle* h, const double* m, double* dmdt) {
     {
       Span s(h, "cg_nbc");
       w.sync = h->sync_nbc;
-      HIPCHK(launch_cg(h->d, h->mixed, w, h->cg_grid, h->cg_rpt, h->d.dinv_nbc, h->rho,
+      HIPCHK(launch_cg(h->d, h->mixed, w, h->cg_grid, h->cg_rpt, h->cg_maxw, h->d.dinv_nbc, h->rho,
                        h->phi1, h->opt.nbc_rtol, h->opt.nbc_maxits, nullptr, nullptr,
                        h->phi1b, h->stream));
     }
@@ -166,7 +166,7 @@ int device_rhs(nsim_handle* h, const double* m, double* dmdt) {
     {
       Span s(h, "cg_dbc");
       w.sync = h->sync_dbc;
-      HIPCHK(launch_cg(h->d, h->mixed, w, h->cg_grid, h->cg_rpt, h->d.dinv_dbc, nullptr,
+      HIPCHK(launch_cg(h->d, h->mixed, w, h->cg_grid, h->cg_rpt, h->cg_maxw, h->d.dinv_dbc, nullptr,
                        h->phi2, h->opt.dbc_rtol, h->opt.dbc_maxits, h->phi1, h->phi,
                        nullptr, h->stream));
     }
@@ -241,6 +241,8 @@ int nsim_create(const nsim_mesh* mesh, const nsim_material* mat, const nsim_opti
   d.ntri = o.ntri;
   d.nslices = o.nslices;
   d.sell_total = o.sell_off[o.nslices];
+  for (int sl = 0; sl < o.nslices; sl++)
+    d.max_width = std::max(d.max_width, (o.sell_off[sl + 1] - o.sell_off[sl]) / kSlice);
   d.ldb = ((o.nb + 63) / 64) * 64;
   d.btype = h->opt.precision == NSIM_FP64 ? kBemF64
             : h->opt.precision == NSIM_MIXED_FP32 ? kBemF32 : kBemBF16;
@@ -332,18 +334,19 @@ int nsim_create(const nsim_mesh* mesh, const nsim_material* mat, const nsim_opti
   ZEROS(h->io_c, 3 * n);
 
   // persistent CG launch shape: smallest rows-per-thread that fits co-resident
-  int rpt = 1, grid = 0;
+  int rpt = 1, grid = 0, maxw = 0;
   for (; rpt <= 8; rpt *= 2) {
     grid = (int)((n + 256 * (size_t)rpt - 1) / (256 * (size_t)rpt));
-    if (grid <= cg_max_workgroups(h->mixed, rpt, h->device)) break;
+    maxw = cg_slot_variant(d.max_width, rpt);
+    if (grid <= cg_max_workgroups(h->mixed, rpt, maxw, h->device)) break;
   }
   if (rpt > 8) return cleanup(fail("mesh too large for the co-resident CG kernel"));
   h->cg_rpt = rpt;
+  h->cg_maxw = maxw;
   h->cg_grid = grid;
   h->cgw.gmax = std::max(grid, 1024);
   ZEROS(h->cgw.pub, 2 * n);
-  ZEROS(h->cgw.pa, 2 * (size_t)h->cgw.gmax);
-  ZEROS(h->cgw.pb, 2 * (size_t)h->cgw.gmax);
+  ZEROS(h->cgw.part, 4 * (size_t)h->cgw.gmax);
   ZEROS(h->cgw.tag, 2 * (size_t)h->cgw.gmax);
   ZEROS(h->sync_nbc, 8);
   ZEROS(h->sync_dbc, 8);
@@ -376,6 +379,7 @@ int nsim_get_sizes(nsim_handle* h, int* s) {
   s[2] = h->d.ntri;
   s[3] = h->host.T;
   s[4] = h->d.sell_total;
+  s[5] = h->host.rp[h->d.n];
   return 0;
 }
 
@@ -386,9 +390,11 @@ int nsim_set_tuning(nsim_handle* h, const char* key, int value) {
     if (value != 1 && value != 2 && value != 4 && value != 8)
       return fail("cg_rows_per_thread must be 1, 2, 4 or 8");
     int grid = (h->d.n + 256 * value - 1) / (256 * value);
-    if (grid > cg_max_workgroups(h->mixed, value, h->device) || grid > h->cgw.gmax)
+    int maxw = cg_slot_variant(h->d.max_width, value);
+    if (grid > cg_max_workgroups(h->mixed, value, maxw, h->device) || grid > h->cgw.gmax)
       return fail("too few rows per thread for a co-resident CG grid");
     h->cg_rpt = value;
+    h->cg_maxw = maxw;
     h->cg_grid = grid;
     return 0;
   }
@@ -529,6 +535,10 @@ int nsim_get_stat(nsim_handle* h, const char* name, double* out) {
   }
   if (s == "cg_rows_per_thread") {
     *out = h->cg_rpt;
+    return 0;
+  }
+  if (s == "cg_register_slots") {
+    *out = h->cg_maxw;
     return 0;
   }
   unsigned a[8], b[8];

@@ -12,6 +12,7 @@ enum BemType { kBemF64 = 0, kBemF32 = 1, kBemBF16 = 2 };
 // Operators resident in HBM (one copy per precision actually used).
 struct DeviceOps {
   int n = 0, nb = 0, ntri = 0, ldb = 0, nslices = 0, sell_total = 0;
+  int max_width = 0;  // widest SELL slice (slots per row)
   int* sell_off = nullptr;
   int* sell_col = nullptr;
   double* K = nullptr;  // FP64 values (always present)
@@ -40,8 +41,7 @@ struct DeviceOps {
 // Communication buffers of the persistent CG kernel.
 struct CgWork {
   double* pub = nullptr;            // 2*n published vector (by phase parity)
-  double* pa = nullptr;             // 2*gmax per-workgroup partial sums
-  double* pb = nullptr;
+  double* part = nullptr;           // 2*gmax*2 per-workgroup partial sums
   unsigned long long* tag = nullptr;  // 2*gmax phase tags
   unsigned* sync = nullptr;         // [2] error flag [3] last iterations
                                     // [4] total iterations [5] solves
@@ -72,7 +72,7 @@ hipError_t launch_bem_assemble(const DeviceOps& d, hipStream_t st);
 hipError_t launch_exch_div(const DeviceOps& d, bool mixed, const Coeffs& c, const double* m,
                            double* Hexch, double* rho, bool want_rho, hipStream_t st);
 // Persistent pipelined Jacobi-PCG on the stiffness matrix; see the kernel.
-hipError_t launch_cg(const DeviceOps& d, bool mixed, CgWork& w, int grid, int rpt,
+hipError_t launch_cg(const DeviceOps& d, bool mixed, CgWork& w, int grid, int rpt, int maxw,
                      const double* dinv, const double* b, double* x, double rtol, int maxit,
                      const double* xadd, double* xsum, double* restrict_out,
                      hipStream_t st);
@@ -84,7 +84,10 @@ hipError_t launch_demag_llg(const DeviceOps& d, bool mixed, const Coeffs& c,
 hipError_t launch_jtimes(const DeviceOps& d, bool mixed, const Coeffs& c, const double* m,
                          const double* Htotal, const double* v, double* Jv, hipStream_t st);
 
+// Register-slot width of the CG kernel for an operator whose widest slice has
+// `width` slots at `rpt` rows per thread (0: slots streamed from memory).
+int cg_slot_variant(int width, int rpt);
 // Cooperative-launch capacity of the CG kernel (workgroups of 256 threads).
-int cg_max_workgroups(bool mixed, int rpt, int device);
+int cg_max_workgroups(bool mixed, int rpt, int maxw, int device);
 
 }  // namespace nsim

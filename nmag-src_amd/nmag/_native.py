@@ -110,9 +110,10 @@ class NativeMagnet:
         _check(L.nsim_create(ctypes.byref(mesh), ctypes.byref(mat), ctypes.byref(opt),
                              ctypes.byref(h)), "nsim_create")
         self._h = h
-        s = np.zeros(5, dtype=np.int32)
+        s = np.zeros(6, dtype=np.int32)
         _check(L.nsim_get_sizes(h, s.ctypes.data_as(_ip)), "nsim_get_sizes")
-        self.n, self.nb, self.ntri, self.ntet, self.slots = [int(v) for v in s]
+        self.n, self.nb, self.ntri, self.ntet, self.slots = [int(v) for v in s[:5]]
+        self.nnz = int(s[5])
         self.precision = precision
 
     def close(self):
