@@ -94,8 +94,37 @@ int nsim_rhs(nsim_handle* h, double t, const double* y, double* ydot, int device
 int nsim_jtimes(nsim_handle* h, double t, const double* v, double* Jv,
                 const double* y, int device_ptrs);
 
+/* CVODE psetup callback (src/sundials_sp_stubs.c:352, src/nsim.ml:2419-2461):
+ * builds M = I - gamma J_block, the 3x3-per-node block diagonal of the
+ * jacobi_ts1 Jacobian at y with H_total of the last rhs; *jcur = 1. */
+int nsim_psetup(nsim_handle* h, double t, const double* y, const double* fy, int jok,
+                int* jcur, double gamma, int device_ptrs);
+/* CVODE psolve callback (src/sundials_sp_stubs.c:473, src/nsim.ml:2465-2484):
+ * z = M^-1 r.  Only left preconditioning (lr == 1) is accepted, as in the
+ * reference (src/nsim.ml:2469-2472). */
+int nsim_psolve(nsim_handle* h, double t, const double* y, const double* fy, const double* r,
+                double* z, double gamma, double delta, int lr, int device_ptrs);
+
+/* Replaces lam_ts_init (src/nsimexec.ml:4113-4128 -> src/nsim.ml:2485-2536):
+ * (re)initialises the device BDF integrator from the handle's m at t0 with
+ * scalar tolerances (CVodeReInit semantics: history is discarded).
+ * max_order <= 0 keeps 2, krylov_max <= 0 keeps 300 (main.py:838-839). */
+int nsim_ts_init(nsim_handle* h, double t0, double rtol, double atol, int max_order,
+                 int krylov_max);
+/* Replaces lam_ts_advance (src/nsimexec.ml:4130-4146 -> src/nsim.ml:2537-2580):
+ * integrates towards t_final, at most max_steps steps when max_steps > 0.
+ * exact_tstop != 0 never steps past t_final (CV_NORMAL_TSTOP); otherwise the
+ * solution is interpolated back to t_final.  The handle's m receives y at
+ * *t_reached.  Returns > 0 on step-size underflow / repeated failures. */
+int nsim_advance(nsim_handle* h, double t_final, int max_steps, int exact_tstop,
+                 double* t_reached);
+
 /* Statistics: "cg_iters_nbc", "cg_iters_dbc" (last solve), "rhs_evals",
- * "cg_iters_nbc_total", "cg_iters_dbc_total". */
+ * "cg_iters_nbc_total", "cg_iters_dbc_total", "cg_error"; and of the
+ * integrator (cvode_get_stats, src/sundials_sp_stubs.c:1409-1528):
+ * "ts_steps", "ts_rhs", "ts_newton", "ts_linear", "ts_err_fails",
+ * "ts_conv_fails", "ts_h", "ts_order", "ts_t", "ts_max_dm" (max_i |m_i(t1) -
+ * m_i(t0)| over the last nsim_advance, as mwe_norm_dist_fun). */
 int nsim_get_stat(nsim_handle* h, const char* name, double* out);
 
 /* Per-kernel device timers, the analogue of the reference's TSTART/TSTOP

@@ -17,6 +17,7 @@
 #include "nsim_device.h"
 #include "nsim_internal.h"
 #include "nsim_integrator.h"
+#include "nsim_vec.h"
 
 using namespace nsim;
 
@@ -86,6 +87,9 @@ struct nsim_handle {
   bool timers_on = false;
   std::map<std::string, Timer> timers;
   Integrator* ts = nullptr;
+  TsOptions ts_opt;
+  double* kappa = nullptr;  // negJ K_ii / V_i: dH_exch_i/dm_i
+  double* pc_Minv = nullptr;  // block-Jacobi preconditioner of nsim_psetup
   std::vector<void*> allocations;
 
   void track(void* p) {
@@ -190,8 +194,10 @@ int device_jtimes(nsim_handle* h, const double* m, const double* v, double* Jv) 
 
 hipStream_t handle_stream(nsim_handle* h) { return h->stream; }
 int handle_n(nsim_handle* h) { return h->d.n; }
-const double* handle_volumes_inv(nsim_handle* h) { return h->d.Vinv; }
 double* handle_m(nsim_handle* h) { return h->m; }
+const double* handle_Htotal(nsim_handle* h) { return h->Htotal; }
+const double* handle_kappa(nsim_handle* h) { return h->kappa; }
+const Coeffs& handle_coeffs(nsim_handle* h) { return h->coef; }
 int handle_error_flags(nsim_handle* h, unsigned* out) {
   unsigned a[6], b[6];
   HIPCHK(hipMemcpyAsync(a, h->sync_nbc, sizeof(a), hipMemcpyDeviceToHost, h->stream));
@@ -289,6 +295,9 @@ int nsim_create(const nsim_mesh* mesh, const nsim_material* mat, const nsim_opti
   UP(d.dinv_nbc, dn);
   UP(d.dinv_dbc, dd);
   UP(d.blocal, o.blocal);
+  std::vector<double> kap(o.n);
+  for (int i = 0; i < o.n; i++) kap[i] = h->coef.negJ * o.Kdiag[i] / o.V[i];
+  UP(h->kappa, kap);
 
   // BEM matrix, assembled on the device from the boundary geometry
   if (o.nb > 0) {
@@ -537,6 +546,22 @@ int nsim_get_stat(nsim_handle* h, const char* name, double* out) {
     *out = h->cg_rpt;
     return 0;
   }
+  if (s.rfind("ts_", 0) == 0) {
+    if (!h->ts) return fail("nsim_get_stat: timestepper not initialised");
+    const TsStats& t = h->ts->stats();
+    if (s == "ts_steps") *out = (double)t.steps;
+    else if (s == "ts_rhs") *out = (double)t.rhs;
+    else if (s == "ts_newton") *out = (double)t.newton;
+    else if (s == "ts_linear") *out = (double)t.linear;
+    else if (s == "ts_err_fails") *out = (double)t.err_fails;
+    else if (s == "ts_conv_fails") *out = (double)t.conv_fails;
+    else if (s == "ts_h") *out = t.h_last;
+    else if (s == "ts_order") *out = t.q_last;
+    else if (s == "ts_t") *out = t.t;
+    else if (s == "ts_max_dm") *out = t.max_dm;
+    else return fail("nsim_get_stat: unknown statistic '" + s + "'");
+    return 0;
+  }
   if (s == "cg_register_slots") {
     *out = h->cg_maxw;
     return 0;
@@ -569,6 +594,83 @@ int nsim_get_timer(nsim_handle* h, const char* name, double* out) {
   auto it = h->timers.find(name);
   out[0] = it == h->timers.end() ? 0.0 : it->second.ms;
   out[1] = it == h->timers.end() ? 0.0 : (double)it->second.count;
+  return 0;
+}
+
+int nsim_ts_init(nsim_handle* h, double t0, double rtol, double atol, int max_order,
+                 int krylov_max) {
+  if (!h) return fail("nsim_ts_init: null handle");
+  if (!(rtol >= 0.0) || !(atol > 0.0)) return fail("nsim_ts_init: need rtol >= 0, atol > 0");
+  TsOptions o = h->ts_opt;
+  o.rtol = rtol;
+  o.atol = atol;
+  if (max_order > 0) o.max_order = max_order;
+  if (krylov_max > 0) o.krylov_max = krylov_max;
+  if (!h->ts || o.krylov_max != h->ts_opt.krylov_max) {
+    delete h->ts;
+    h->ts = new Integrator(h, o);
+  } else {
+    h->ts->options() = o;
+  }
+  h->ts_opt = o;
+  return h->ts->init(t0);
+}
+
+int nsim_advance(nsim_handle* h, double t_final, int max_steps, int exact_tstop,
+                 double* t_reached) {
+  if (!h || !t_reached) return fail("nsim_advance: null argument");
+  if (!h->ts) return fail("nsim_advance: call nsim_ts_init first");
+  const int rc = h->ts->advance(t_final, max_steps, exact_tstop != 0, t_reached);
+  if (rc) return rc;
+  unsigned flags = 0;
+  if (handle_error_flags(h, &flags) == 0 && flags != 0)
+    return fail("nsim_advance: a Poisson solve reported an error (flag " +
+                std::to_string(flags) + ")");
+  return 0;
+}
+
+int nsim_psetup(nsim_handle* h, double t, const double* y, const double* fy, int jok,
+                int* jcur, double gamma, int device_ptrs) {
+  (void)t;
+  (void)fy;
+  (void)jok;
+  if (!h || !y) return fail("nsim_psetup: null argument");
+  const size_t n = h->d.n;
+  if (!h->pc_Minv) {
+    HIPCHK(hipMalloc(&h->pc_Minv, 9 * n * sizeof(double)));
+    h->track(h->pc_Minv);
+  }
+  const double* yd = y;
+  if (!device_ptrs) {
+    HIPCHK(hipMemcpyAsync(h->io_a, y, 3 * n * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    yd = h->io_a;
+  }
+  // H_total of the most recent rhs (CVODE calls psetup after f(t, y))
+  HIPCHK(bjac_setup(h->stream, (int)n, h->coef, gamma, yd, h->Htotal, h->kappa, h->pc_Minv));
+  if (jcur) *jcur = 1;
+  if (!device_ptrs) HIPCHK(hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+int nsim_psolve(nsim_handle* h, double t, const double* y, const double* fy, const double* r,
+                double* z, double gamma, double delta, int lr, int device_ptrs) {
+  (void)t;
+  (void)y;
+  (void)fy;
+  (void)gamma;
+  (void)delta;
+  if (!h || !r || !z) return fail("nsim_psolve: null argument");
+  if (lr != 1) return fail("nsim_psolve: only left preconditioning (lr == 1) is supported");
+  if (!h->pc_Minv) return fail("nsim_psolve: call nsim_psetup first");
+  const size_t n = h->d.n;
+  if (device_ptrs) {
+    HIPCHK(bjac_solve(h->stream, (int)n, h->pc_Minv, r, z));
+    return 0;
+  }
+  HIPCHK(hipMemcpyAsync(h->io_a, r, 3 * n * sizeof(double), hipMemcpyHostToDevice, h->stream));
+  HIPCHK(bjac_solve(h->stream, (int)n, h->pc_Minv, h->io_a, h->io_b));
+  HIPCHK(hipMemcpyAsync(z, h->io_b, 3 * n * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
   return 0;
 }
 

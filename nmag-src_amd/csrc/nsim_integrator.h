@@ -1,27 +1,90 @@
-// Device-resident time integrator (BDF with Newton-GMRES); filled in below the
-// rhs path.
+// Device-resident time integrator: variable-step BDF (order <= 2) with a
+// Newton iteration whose linear systems (I - gamma J) delta = -G are solved by
+// scaled GMRES with the analytic jacobi_ts1 J.v and a 3x3 block-Jacobi right
+// preconditioner.  It stands in for CVODE (BDF + Newton + SPGMR) behind the
+// reference's timestepper (src/nsim.ml:2485-2580, src/sundials_sp_stubs.c:700-1308).
 #pragma once
 
 #include <hip/hip_runtime.h>
 
 #include <string>
+#include <vector>
 
 struct nsim_handle;
 
 namespace nsim {
-
-struct Integrator {
-  virtual ~Integrator() = default;
-};
 
 // Entry points of the rhs path used by the integrator.
 int device_rhs(nsim_handle* h, const double* m, double* dmdt);
 int device_jtimes(nsim_handle* h, const double* m, const double* v, double* Jv);
 hipStream_t handle_stream(nsim_handle* h);
 int handle_n(nsim_handle* h);
-const double* handle_volumes_inv(nsim_handle* h);
 double* handle_m(nsim_handle* h);
+const double* handle_Htotal(nsim_handle* h);
+const double* handle_kappa(nsim_handle* h);
+struct Coeffs;
+const Coeffs& handle_coeffs(nsim_handle* h);
 int handle_error_flags(nsim_handle* h, unsigned* out);
 void set_error(const std::string& msg);
+
+struct TsOptions {
+  double rtol = 1e-6, atol = 1e-6;  // scalar tolerances (main.py:724-725)
+  int max_order = 2;                // timestepper_max_order (main.py:838)
+  int krylov_max = 300;             // timestepper_krylov_max (main.py:839)
+  int precond = 1;                  // 0 none, 1 block Jacobi of I - gamma J
+};
+
+struct TsStats {
+  long steps = 0, rhs = 0, newton = 0, linear = 0, err_fails = 0, conv_fails = 0;
+  double h_last = 0.0, t = 0.0, max_dm = 0.0;
+  int q_last = 0;
+};
+
+class Integrator {
+ public:
+  Integrator(nsim_handle* h, const TsOptions& o);
+  ~Integrator();
+  // Takes y(t0) from the handle's m; forgets all history (CVodeReInit).
+  int init(double t0);
+  // Integrates towards t_final (at most max_steps steps if max_steps > 0).
+  // exact_tstop: never step past t_final; else step past and interpolate.
+  // The handle's m receives y at *t_reached.
+  int advance(double t_final, int max_steps, bool exact_tstop, double* t_reached);
+  const TsStats& stats() const { return st_; }
+  TsOptions& options() { return opt_; }
+
+ private:
+  int step_attempt(double h, int q, bool* accepted, double* err_out);
+  int gmres(const double* r, double* x, int* iters);
+  int apply_A(const double* z, double* w);
+  double wrms(const double* x);
+  int reduce(int nparts, int k, std::vector<double>* out);
+  int predictor(double h, int q, double* yp, double* E_p);
+  int check(hipError_t e, const char* what);
+
+  nsim_handle* h_;
+  TsOptions opt_;
+  TsStats st_;
+  hipStream_t s_;
+  int n3_ = 0, nn_ = 0;
+  bool ok_ = false;
+  // state: y_n at t_n and up to two previous points
+  double *yn_ = nullptr, *ynm1_ = nullptr, *ynm2_ = nullptr;
+  double tn_ = 0.0, tnm1_ = 0.0, tnm2_ = 0.0;
+  int nhist_ = 0;   // valid previous points
+  double hnext_ = 0.0;
+  int qnext_ = 1;
+  double *fn_ = nullptr;    // f(y_n) (first step's predictor)
+  double *yp_ = nullptr, *ycur_ = nullptr, *avec_ = nullptr, *f_ = nullptr, *res_ = nullptr,
+         *del_ = nullptr, *tmp_ = nullptr, *tmp2_ = nullptr, *ewt_ = nullptr, *ystart_ = nullptr;
+  double* V_ = nullptr;     // (maxl+1) Krylov vectors
+  double* Minv_ = nullptr;  // 9 per node
+  double* coef_ = nullptr;  // device coefficients for multi-axpy
+  double* part_ = nullptr;  // reduction partials
+  std::vector<double> host_part_;
+  double gamma_ = 0.0;      // current Newton matrix coefficient
+  double gamma_pc_ = -1.0;  // gamma the preconditioner was set up for
+  std::vector<void*> allocs_;
+};
 
 }  // namespace nsim

@@ -1,0 +1,33 @@
+// Launchers of the integrator's vector kernels (nsim_vec.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+
+#include "nsim_device.h"
+
+namespace nsim {
+
+constexpr int kReduceBlocks = 240;  // upper bound on partials of one reduction
+
+hipError_t vec_lincomb(hipStream_t st, int n, double c0, const double* x0, double c1,
+                       const double* x1, double c2, const double* x2, double c3,
+                       const double* x3, double* y);
+hipError_t vec_mul(hipStream_t st, int n, const double* x, const double* w, bool inverse,
+                   double* y);
+hipError_t vec_ewt(hipStream_t st, int n, const double* y, double rtol, double atol,
+                   double* ewt);
+hipError_t vec_wsq_partials(hipStream_t st, int n, const double* x, const double* w,
+                            double* part, int* nparts);
+hipError_t vec_multidot_partials(hipStream_t st, int n, int k, const double* V, size_t ld,
+                                 const double* y, double* part, int* nparts);
+hipError_t vec_axpy_multi(hipStream_t st, int n, int k, const double* V, size_t ld,
+                          const double* c_dev, double* y);
+hipError_t vec_maxdist_partials(hipStream_t st, int nn, const double* a, const double* b,
+                                double* part, int* nparts);
+hipError_t bjac_setup(hipStream_t st, int nn, const Coeffs& c, double gamma, const double* m,
+                      const double* H, const double* kappa, double* Minv);
+hipError_t bjac_solve(hipStream_t st, int nn, const double* Minv, const double* r, double* z);
+
+}  // namespace nsim

@@ -1,0 +1,232 @@
+// N_Vector operations of the device-resident time integrator, and the 3x3
+// block-Jacobi preconditioner of I - gamma J.
+//
+// Reductions write one partial per workgroup (fixed grid, fixed order) and the
+// host adds them, so every decision of the step controller is reproducible.
+// CVODE's parallel N_Vector (N_VLinearSum, N_VWrmsNorm, N_VDotProd ...) is what
+// these replace for the reference's timestepper (src/sundials_sp_stubs.c).
+#include <hip/hip_runtime.h>
+
+#include "nsim_dev_common.h"
+#include "nsim_device.h"
+#include "nsim_vec.h"
+
+namespace nsim {
+
+namespace {
+
+using namespace dev;
+
+__device__ inline double block_sum(double v, double* sh) {
+  double dummy = 0.0;
+  block_sum2(v, dummy, sh);
+  return v;
+}
+
+__global__ __launch_bounds__(256) void k_lincomb(int n, double c0, const double* x0, double c1,
+                                                 const double* x1, double c2, const double* x2,
+                                                 double c3, const double* x3, double* y) {
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+    double v = c0 * x0[i];
+    if (x1) v += c1 * x1[i];
+    if (x2) v += c2 * x2[i];
+    if (x3) v += c3 * x3[i];
+    y[i] = v;
+  }
+}
+
+// y = x * w (inverse: y = x / w)
+__global__ __launch_bounds__(256) void k_vmul(int n, const double* x, const double* w,
+                                              int inverse, double* y) {
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256)
+    y[i] = inverse ? x[i] / w[i] : x[i] * w[i];
+}
+
+__global__ __launch_bounds__(256) void k_ewt(int n, const double* y, double rtol, double atol,
+                                             double* ewt) {
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256)
+    ewt[i] = 1.0 / (rtol * fabs(y[i]) + atol);
+}
+
+// part[b] = sum over this block's elements of (x_i * w_i)^2 (w may be null: 1)
+__global__ __launch_bounds__(256) void k_wsq(int n, const double* x, const double* w,
+                                             double* part) {
+  __shared__ double sh[8];
+  double s = 0.0;
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+    const double v = w ? x[i] * w[i] : x[i];
+    s += v * v;
+  }
+  s = block_sum(s, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// part[j * G + b] = sum_i V_j[i] * y[i] for j < k (k <= 8), V_j = V + j*ld
+__global__ __launch_bounds__(256) void k_multidot(int n, int k, const double* V, size_t ld,
+                                                  const double* y, double* part) {
+  __shared__ double sh[8];
+  double s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+    const double yi = y[i];
+#pragma unroll
+    for (int j = 0; j < 8; j++)
+      if (j < k) s[j] += V[j * ld + i] * yi;
+  }
+  for (int j = 0; j < k; j++) {
+    const double t = block_sum(s[j], sh);
+    if (threadIdx.x == 0) part[j * gridDim.x + blockIdx.x] = t;
+  }
+}
+
+// y += sum_j c[j] V_j  (c in device memory, k <= 64)
+__global__ __launch_bounds__(256) void k_axpy_multi(int n, int k, const double* V, size_t ld,
+                                                    const double* c, double* y) {
+  __shared__ double cs[64];
+  if (threadIdx.x < k) cs[threadIdx.x] = c[threadIdx.x];
+  __syncthreads();
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+    double v = y[i];
+    for (int j = 0; j < k; j++) v += cs[j] * V[j * ld + i];
+    y[i] = v;
+  }
+}
+
+// part[b] = max over this block's nodes of |a_i - b_i| (3-vectors)
+__global__ __launch_bounds__(256) void k_maxdist(int nn, const double* a, const double* b,
+                                                 double* part) {
+  __shared__ double sh[4];
+  double mx = 0.0;
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < nn; i += gridDim.x * 256) {
+    const double dx = a[3 * i] - b[3 * i], dy = a[3 * i + 1] - b[3 * i + 1],
+                 dz = a[3 * i + 2] - b[3 * i + 2];
+    mx = fmax(mx, sqrt(dx * dx + dy * dy + dz * dz));
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o, 64));
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = mx;
+  __syncthreads();
+  if (threadIdx.x == 0) part[blockIdx.x] = fmax(fmax(sh[0], sh[1]), fmax(sh[2], sh[3]));
+}
+
+// Per node: M_i = I - gamma J_ii with J_ii the 3x3 diagonal block of the
+// jacobi_ts1 linearisation (H frozen, dH_i/dm_i = kappa_i I, kappa_i =
+// negJ K_ii / V_i); stores M_i^-1 (row-major, 9 doubles).
+__global__ __launch_bounds__(256) void k_bjac_setup(int nn, Coeffs c, double gamma,
+                                                    const double* m, const double* H,
+                                                    const double* kappa, double* Minv) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= nn) return;
+  const double mi[3] = {m[3 * i], m[3 * i + 1], m[3 * i + 2]};
+  const double Hi[3] = {H[3 * i], H[3 * i + 1], H[3 * i + 2]};
+  const double kp = kappa[i];
+  double M[3][3];
+  for (int col = 0; col < 3; col++) {
+    double v[3] = {0.0, 0.0, 0.0};
+    v[col] = 1.0;
+    const double h[3] = {kp * v[0], kp * v[1], kp * v[2]};
+    double vxH[3], mxh[3], mxH[3], vxmxH[3], mxvxH[3], mxmxh[3];
+    cross(v, Hi, vxH);
+    cross(mi, h, mxh);
+    cross(mi, Hi, mxH);
+    cross(v, mxH, vxmxH);
+    cross(mi, vxH, mxvxH);
+    cross(mi, mxh, mxmxh);
+    for (int r = 0; r < 3; r++) {
+      const double J = c.c1 * (vxH[r] + mxh[r]) + c.c2 * (vxmxH[r] + mxvxH[r] + mxmxh[r]);
+      M[r][col] = (r == col ? 1.0 : 0.0) - gamma * J;
+    }
+  }
+  const double det = M[0][0] * (M[1][1] * M[2][2] - M[1][2] * M[2][1]) -
+                     M[0][1] * (M[1][0] * M[2][2] - M[1][2] * M[2][0]) +
+                     M[0][2] * (M[1][0] * M[2][1] - M[1][1] * M[2][0]);
+  const double id = 1.0 / det;
+  double* o = Minv + 9 * i;
+  o[0] = (M[1][1] * M[2][2] - M[1][2] * M[2][1]) * id;
+  o[1] = (M[0][2] * M[2][1] - M[0][1] * M[2][2]) * id;
+  o[2] = (M[0][1] * M[1][2] - M[0][2] * M[1][1]) * id;
+  o[3] = (M[1][2] * M[2][0] - M[1][0] * M[2][2]) * id;
+  o[4] = (M[0][0] * M[2][2] - M[0][2] * M[2][0]) * id;
+  o[5] = (M[0][2] * M[1][0] - M[0][0] * M[1][2]) * id;
+  o[6] = (M[1][0] * M[2][1] - M[1][1] * M[2][0]) * id;
+  o[7] = (M[0][1] * M[2][0] - M[0][0] * M[2][1]) * id;
+  o[8] = (M[0][0] * M[1][1] - M[0][1] * M[1][0]) * id;
+}
+
+__global__ __launch_bounds__(256) void k_bjac_solve(int nn, const double* Minv, const double* r,
+                                                    double* z) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= nn) return;
+  const double* o = Minv + 9 * i;
+  const double a = r[3 * i], b = r[3 * i + 1], d = r[3 * i + 2];
+  z[3 * i] = o[0] * a + o[1] * b + o[2] * d;
+  z[3 * i + 1] = o[3] * a + o[4] * b + o[5] * d;
+  z[3 * i + 2] = o[6] * a + o[7] * b + o[8] * d;
+}
+
+inline int grid_for(int n) {
+  const int g = (n + 255) / 256;
+  return g < kReduceBlocks ? g : kReduceBlocks;
+}
+
+}  // namespace
+
+hipError_t vec_lincomb(hipStream_t st, int n, double c0, const double* x0, double c1,
+                       const double* x1, double c2, const double* x2, double c3,
+                       const double* x3, double* y) {
+  k_lincomb<<<grid_for(n), 256, 0, st>>>(n, c0, x0, c1, x1, c2, x2, c3, x3, y);
+  return hipGetLastError();
+}
+
+hipError_t vec_mul(hipStream_t st, int n, const double* x, const double* w, bool inverse,
+                   double* y) {
+  k_vmul<<<grid_for(n), 256, 0, st>>>(n, x, w, inverse ? 1 : 0, y);
+  return hipGetLastError();
+}
+
+hipError_t vec_ewt(hipStream_t st, int n, const double* y, double rtol, double atol,
+                   double* ewt) {
+  k_ewt<<<grid_for(n), 256, 0, st>>>(n, y, rtol, atol, ewt);
+  return hipGetLastError();
+}
+
+hipError_t vec_wsq_partials(hipStream_t st, int n, const double* x, const double* w,
+                            double* part, int* nparts) {
+  *nparts = grid_for(n);
+  k_wsq<<<*nparts, 256, 0, st>>>(n, x, w, part);
+  return hipGetLastError();
+}
+
+hipError_t vec_multidot_partials(hipStream_t st, int n, int k, const double* V, size_t ld,
+                                 const double* y, double* part, int* nparts) {
+  if (k < 1 || k > 8) return hipErrorInvalidValue;
+  *nparts = grid_for(n);
+  k_multidot<<<*nparts, 256, 0, st>>>(n, k, V, ld, y, part);
+  return hipGetLastError();
+}
+
+hipError_t vec_axpy_multi(hipStream_t st, int n, int k, const double* V, size_t ld,
+                          const double* c_dev, double* y) {
+  if (k < 1 || k > 64) return hipErrorInvalidValue;
+  k_axpy_multi<<<grid_for(n), 256, 0, st>>>(n, k, V, ld, c_dev, y);
+  return hipGetLastError();
+}
+
+hipError_t vec_maxdist_partials(hipStream_t st, int nn, const double* a, const double* b,
+                                double* part, int* nparts) {
+  *nparts = grid_for(nn);
+  k_maxdist<<<*nparts, 256, 0, st>>>(nn, a, b, part);
+  return hipGetLastError();
+}
+
+hipError_t bjac_setup(hipStream_t st, int nn, const Coeffs& c, double gamma, const double* m,
+                      const double* H, const double* kappa, double* Minv) {
+  k_bjac_setup<<<(nn + 255) / 256, 256, 0, st>>>(nn, c, gamma, m, H, kappa, Minv);
+  return hipGetLastError();
+}
+
+hipError_t bjac_solve(hipStream_t st, int nn, const double* Minv, const double* r, double* z) {
+  k_bjac_solve<<<(nn + 255) / 256, 256, 0, st>>>(nn, Minv, r, z);
+  return hipGetLastError();
+}
+
+}  // namespace nsim

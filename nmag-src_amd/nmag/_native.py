@@ -53,6 +53,14 @@ SIGNATURES = {
     "nsim_get_timer": (ctypes.c_int, [_vp, ctypes.c_char_p, _dp]),
     "nsim_set_tuning": (ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.c_int]),
     "nsim_synchronize": (ctypes.c_int, [_vp]),
+    "nsim_psetup": (ctypes.c_int, [_vp, ctypes.c_double, _vp, _vp, ctypes.c_int,
+                                   ctypes.POINTER(ctypes.c_int), ctypes.c_double, ctypes.c_int]),
+    "nsim_psolve": (ctypes.c_int, [_vp, ctypes.c_double, _vp, _vp, _vp, _vp, ctypes.c_double,
+                                   ctypes.c_double, ctypes.c_int, ctypes.c_int]),
+    "nsim_ts_init": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                    ctypes.c_int, ctypes.c_int]),
+    "nsim_advance": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                    ctypes.POINTER(ctypes.c_double)]),
 }
 
 _lib = None
@@ -188,3 +196,28 @@ class NativeMagnet:
 
     def synchronize(self):
         _check(lib().nsim_synchronize(self._h), "nsim_synchronize")
+
+    def psetup(self, y, gamma, t=0.0):
+        y = np.ascontiguousarray(y, dtype=np.float64).reshape(-1)
+        jcur = ctypes.c_int(0)
+        _check(lib().nsim_psetup(self._h, t, y.ctypes.data, None, 0, ctypes.byref(jcur), gamma, 0),
+               "nsim_psetup")
+        return jcur.value
+
+    def psolve(self, r, gamma=0.0, t=0.0, lr=1):
+        r = np.ascontiguousarray(r, dtype=np.float64).reshape(-1)
+        z = np.zeros_like(r)
+        _check(lib().nsim_psolve(self._h, t, None, None, r.ctypes.data, z.ctypes.data, gamma, 0.0,
+                                 lr, 0), "nsim_psolve")
+        return z
+
+    def ts_init(self, t0=0.0, rtol=1e-6, atol=1e-6, max_order=2, krylov_max=300):
+        _check(lib().nsim_ts_init(self._h, t0, rtol, atol, max_order, krylov_max), "nsim_ts_init")
+
+    def advance(self, t_final, max_steps=-1, exact_tstop=True):
+        """Integrate towards t_final (simulation time units); returns t reached."""
+        tr = ctypes.c_double(0)
+        rc = lib().nsim_advance(self._h, t_final, int(max_steps), 1 if exact_tstop else 0,
+                                ctypes.byref(tr))
+        _check(rc, "nsim_advance")
+        return tr.value

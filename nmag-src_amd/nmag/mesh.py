@@ -35,6 +35,41 @@ def kuhn_cuboid(nx, ny, nz, h, origin=(0.0, 0.0, 0.0)):
     return coords, tets.reshape(-1, 4).astype(np.int32)
 
 
+def read_nmesh(path):
+    """Read an nmesh ASCII file (format version 1.0, src/mesh.ml:6426-6616).
+
+    Returns (coords (N,3), tets (T,4) int32, regions (T,) int32)."""
+    with open(path) as f:
+        lines = [ln for ln in f if not ln.startswith("#") and ln.strip()]
+    pos = 0
+    n = int(lines[pos].split()[0]); pos += 1
+    coords = np.array([[float(v) for v in lines[pos + i].split()] for i in range(n)])
+    pos += n
+    t = int(lines[pos].split()[0]); pos += 1
+    rows = np.array([[int(v) for v in lines[pos + i].split()] for i in range(t)], dtype=np.int64)
+    if coords.ndim != 2 or coords.shape[1] != 3 or rows.shape[1] != 5:
+        raise ValueError("%s: only 3-d tetrahedral nmesh files are supported" % path)
+    return coords, rows[:, 1:5].astype(np.int32), rows[:, 0].astype(np.int32)
+
+
+def write_nmesh(path, coords, tets, regions=None):
+    """Write an nmesh ASCII file (version 1.0) without surface/periodic data."""
+    coords = np.asarray(coords, dtype=np.float64)
+    tets = np.asarray(tets)
+    regions = np.ones(len(tets), dtype=int) if regions is None else np.asarray(regions)
+    with open(path, "w") as f:
+        f.write("# PYFEM mesh file version 1.0\n")
+        f.write("# dim = 3 \t nodes = %d \t simplices = %d \t surfaces = 0 \t periodic = 0\n"
+                % (len(coords), len(tets)))
+        f.write("%d\n" % len(coords))
+        for c in coords:
+            f.write("".join("\t%25.12f " % v for v in c) + "\n")
+        f.write("%d\n" % len(tets))
+        for r, t in zip(regions, tets):
+            f.write("\t%d " % r + "".join("%d\t" % v for v in t) + "\n")
+        f.write("0\n0\n")
+
+
 def cuboid_sizes(nx, ny, nz):
     """(nodes, tets, boundary nodes, boundary triangles) of a Kuhn cuboid."""
     n = (nx + 1) * (ny + 1) * (nz + 1)

@@ -1,0 +1,141 @@
+"""The device BDF/Newton-GMRES integrator: closed-form macrospin dynamics,
+trajectories against the oracle's rhs integrated by SciPy (DOP853, tight
+tolerances), CVODE-like stop semantics, and the psetup/psolve blocks."""
+import math
+
+import numpy as np
+import pytest
+from scipy.integrate import solve_ivp
+
+import oracle
+from helpers import material, orc_params, random_m, rel_err
+from nmag import kuhn_cuboid
+from nmag._native import NativeMagnet
+
+pytestmark = pytest.mark.gpu
+
+
+def _magnet(nx, ny, nz, h, mat, rtol=1e-11, do_demag=True):
+    coords, tets = kuhn_cuboid(nx, ny, nz, h)
+    g = NativeMagnet(coords, tets, mat.su_coeffs(), do_demag=do_demag, nbc_rtol=rtol,
+                     dbc_rtol=rtol, nbc_maxits=100000, dbc_maxits=100000)
+    return coords, tets, g
+
+
+def test_macrospin_precession_and_damping_closed_form():
+    alpha, H = 0.1, 0.1
+    mat = material(alpha=alpha)
+    _, _, g = _magnet(1, 1, 1, 2.5, mat, do_demag=False)
+    th0, ph0 = 1.2, 0.3
+    m0 = [math.sin(th0) * math.cos(ph0), math.sin(th0) * math.sin(ph0), math.cos(th0)]
+    g.set_field("m", np.tile(m0, (g.n, 1)))
+    g.set_field("H_ext", np.tile([0.0, 0.0, H], (g.n, 1)))
+    g.ts_init(0.0, rtol=1e-9, atol=1e-9)
+    gp = mat.su_llg_gamma_G / (1 + alpha ** 2)
+    for T in (20.0, 100.0):
+        assert g.advance(T) == pytest.approx(T, abs=1e-12)
+        m = g.get_field("m")
+        th = 2 * math.atan(math.tan(th0 / 2) * math.exp(-alpha * gp * H * T))
+        ph = ph0 + gp * H * T
+        ref = [math.sin(th) * math.cos(ph), math.sin(th) * math.sin(ph), math.cos(th)]
+        assert np.abs(m - ref).max() < 2e-6
+    assert g.stat("ts_steps") > 10 and g.stat("ts_order") == 2
+
+
+def test_trajectory_matches_oracle_rhs_integrated_by_scipy():
+    mat = material(alpha=0.3)
+    coords, tets, g = _magnet(6, 5, 3, 2.5, mat)
+    x, y = coords[:, 0], coords[:, 1]
+    m0 = np.stack([np.ones_like(x), 0.4 * np.sin(x / 7.0), 0.3 * np.cos(y / 5.0)], axis=1)
+    m0 /= np.linalg.norm(m0, axis=1, keepdims=True)
+    Hx = np.tile([0.0, 0.02, 0.0], (len(coords), 1))
+    g.set_field("m", m0)
+    g.set_field("H_ext", Hx)
+    g.ts_init(0.0, rtol=1e-9, atol=1e-9)
+    orc = oracle.OracleMesh(coords, tets)
+    P = orc_params(mat)
+    state = {"p1": None, "p2": None}
+
+    def f(t, yv):
+        r = orc.fields(P, yv.reshape(-1, 3), H_ext=Hx, rtol=1e-13, phi1=state["p1"],
+                       phi2=state["p2"])
+        state["p1"], state["p2"] = r["phi1"], r["phi2"]
+        return r["dmdt"].reshape(-1)
+
+    T = 10.0
+    sol = solve_ivp(f, (0.0, T), m0.reshape(-1), method="DOP853", rtol=1e-10, atol=1e-12)
+    assert sol.success
+    g.advance(T)
+    m = g.get_field("m")
+    ref = sol.y[:, -1].reshape(-1, 3)
+    assert np.abs(m - ref).max() < 2e-6
+    V = orc.volumes()
+    avg = lambda a: (a * V[:, None]).sum(0) / V.sum()
+    assert np.abs(avg(m) - avg(ref)).max() < 1e-6
+
+
+def test_stop_semantics_and_statistics():
+    mat = material(alpha=0.5)
+    _, _, g = _magnet(4, 3, 2, 2.5, mat, rtol=1e-8)
+    g.set_field("m", random_m(g.n, 3) * 0.2 + np.array([1.0, 0.0, 0.0]))
+    g.ts_init(0.0, rtol=1e-6, atol=1e-6)
+    t = g.advance(5.0, exact_tstop=True)
+    assert t == pytest.approx(5.0, abs=1e-12)
+    s0 = g.stat("ts_steps")
+    t2 = g.advance(1e6, max_steps=3)
+    assert g.stat("ts_steps") - s0 == 3 and 5.0 < t2 < 1e6
+    # non-exact stop: interpolated back to the requested time
+    t3 = g.advance(t2 + 0.37, exact_tstop=False)
+    assert t3 == pytest.approx(t2 + 0.37, abs=1e-12)
+    assert g.stat("ts_max_dm") > 0
+    for k in ("ts_rhs", "ts_newton", "ts_linear"):
+        assert g.stat(k) > 0
+
+
+def test_relaxation_decreases_energy_and_converges():
+    mat = material(alpha=0.5)
+    coords, tets, g = _magnet(6, 5, 3, 2.5, mat, rtol=1e-8)
+    g.set_field("m", np.tile([1.0, 1.0, 0.3], (g.n, 1)) / math.sqrt(2.09))
+    g.ts_init(0.0, rtol=1e-6, atol=1e-6)
+    V = g.get_field("volumes")
+
+    def energy():
+        g.execute()
+        m = g.get_field("m")
+        return -(V[:, None] * m * (0.5 * (g.get_field("H_exch") + g.get_field("H_demag")))).sum()
+
+    e = [energy()]
+    t = 0.0
+    for _ in range(6):
+        t = g.advance(t + 50.0)
+        e.append(energy())
+    assert all(b <= a + 1e-12 * abs(a) for a, b in zip(e, e[1:]))
+    # relaxed along the long (x) axis of the 15 x 12.5 x 7.5 nm particle
+    mx = (g.get_field("m")[:, 0] * V).sum() / V.sum()
+    assert mx > 0.98
+
+
+def test_psetup_psolve_blocks_match_the_jacobian_diagonal():
+    mat = material(alpha=0.2)
+    coords, tets, g = _magnet(2, 2, 1, 2.5, mat)
+    m = random_m(g.n, 31)
+    H = np.tile([0.03, -0.01, 0.02], (g.n, 1))
+    g.set_field("H_ext", H)
+    g.rhs(m.reshape(-1))  # H_total at m
+    Htot = g.get_field("H_total")
+    gamma = 0.7
+    g.psetup(m.reshape(-1), gamma)
+    r = random_m(g.n, 32)
+    z = g.psolve(r.reshape(-1), gamma).reshape(-1, 3)
+    orc = oracle.OracleMesh(coords, tets, build_bem=False)
+    P = orc_params(mat)
+    for i in range(g.n):
+        J = np.zeros((3, 3))
+        for c in range(3):
+            v = np.zeros((g.n, 3))
+            v[i, c] = 1.0
+            J[:, c] = orc.jtimes(P, m, Htot, v)[i]
+        M = np.eye(3) - gamma * J
+        assert np.allclose(M @ z[i], r[i], rtol=1e-12, atol=1e-12)
+    with pytest.raises(Exception):
+        g.psolve(r.reshape(-1), gamma, lr=2)
