@@ -74,7 +74,9 @@ struct nsim_handle {
   bool mixed = false;
   int device = 0, num_cus = 0;
   hipStream_t stream = nullptr;
-  int cg_grid = 0, cg_rpt = 1, cg_maxw = 0;
+  CgDevice cg;                  // the CG's reordered operator
+  std::vector<void*> cg_allocs; // its device buffers
+  bool cg_coarse = true;        // two-level preconditioner on
   // field vectors (site-major 3-vectors or scalars)
   double *m = nullptr, *Hext = nullptr, *k1 = nullptr, *k2 = nullptr, *axis = nullptr,
          *pin = nullptr;
@@ -132,6 +134,45 @@ void flush_timers(nsim_handle* h) {
   }
 }
 
+// (Re)builds the CG operator with `rpt` rows per thread (parts of 256*rpt
+// nodes) and uploads it; fails if the parts do not fit one co-resident grid.
+int setup_cg(nsim_handle* h, int rpt) {
+  CgOperator op;
+  std::string err = build_cg_operator(h->host, 256 * rpt, op);
+  if (!err.empty()) return fail(err);
+  const int maxw = cg_slot_variant(op.max_width, rpt);
+  if (op.parts > 1024 || op.parts > cg_max_workgroups(h->mixed, rpt, maxw, h->device))
+    return fail("CG parts do not fit one co-resident grid at " + std::to_string(rpt) +
+                " rows per thread");
+  for (void* p : h->cg_allocs) (void)hipFree(p);
+  h->cg_allocs.clear();
+  CgDevice c;
+  c.parts = op.parts;
+  c.npad = op.npad;
+  c.rpt = rpt;
+  c.cap_shift = rpt == 1 ? 8 : rpt == 2 ? 9 : 10;
+  c.agg_shift = op.agg_shift;
+  c.naggs = op.naggs;
+  c.maxw = maxw;
+  c.max_width = op.max_width;
+  auto up = [&](auto** dst, const auto& vec) {
+    hipError_t e = upload(dst, vec.data(), vec.size());
+    if (*dst) h->cg_allocs.push_back(*dst);
+    return e == hipSuccess;
+  };
+  bool ok = up(&c.sell_off, op.sell_off) && up(&c.sell_col, op.sell_col) && up(&c.K, op.sK) &&
+            up(&c.p2o, op.p2o) && up(&c.dinv_nbc, op.dinv_nbc) &&
+            up(&c.dinv_dbc, op.dinv_dbc) && up(&c.ainv_nbc, op.ainv_nbc) &&
+            up(&c.ainv_dbc, op.ainv_dbc);
+  if (ok && h->mixed) {
+    std::vector<float> kf(op.sK.begin(), op.sK.end());
+    ok = up(&c.Kf, kf);
+  }
+  if (!ok) return fail("CG operator upload failed");
+  h->cg = c;
+  return 0;
+}
+
 FieldArgs field_args(nsim_handle* h, const double* m) {
   FieldArgs f;
   f.m = m;
@@ -159,9 +200,9 @@ int device_rhs(nsim_handle* h, const double* m, double* dmdt) {
     {
       Span s(h, "cg_nbc");
       w.sync = h->sync_nbc;
-      HIPCHK(launch_cg(h->d, h->mixed, w, h->cg_grid, h->cg_rpt, h->cg_maxw, h->d.dinv_nbc, h->rho,
-                       h->phi1, h->opt.nbc_rtol, h->opt.nbc_maxits, nullptr, nullptr,
-                       h->phi1b, h->stream));
+      HIPCHK(launch_cg(h->cg, h->mixed, w, false, h->cg_coarse, h->rho, h->phi1,
+                       h->opt.nbc_rtol, h->opt.nbc_maxits, nullptr, nullptr, h->phi1b,
+                       h->d.blocal, h->stream));
     }
     {
       Span s(h, "bem");
@@ -170,9 +211,9 @@ int device_rhs(nsim_handle* h, const double* m, double* dmdt) {
     {
       Span s(h, "cg_dbc");
       w.sync = h->sync_dbc;
-      HIPCHK(launch_cg(h->d, h->mixed, w, h->cg_grid, h->cg_rpt, h->cg_maxw, h->d.dinv_dbc, nullptr,
-                       h->phi2, h->opt.dbc_rtol, h->opt.dbc_maxits, h->phi1, h->phi,
-                       nullptr, h->stream));
+      HIPCHK(launch_cg(h->cg, h->mixed, w, true, h->cg_coarse, nullptr, h->phi2,
+                       h->opt.dbc_rtol, h->opt.dbc_maxits, h->phi1, h->phi, nullptr,
+                       h->d.blocal, h->stream));
     }
   }
   {
@@ -342,20 +383,14 @@ int nsim_create(const nsim_mesh* mesh, const nsim_material* mat, const nsim_opti
   ZEROS(h->io_b, 3 * n);
   ZEROS(h->io_c, 3 * n);
 
-  // persistent CG launch shape: smallest rows-per-thread that fits co-resident
-  int rpt = 1, grid = 0, maxw = 0;
-  for (; rpt <= 8; rpt *= 2) {
-    grid = (int)((n + 256 * (size_t)rpt - 1) / (256 * (size_t)rpt));
-    maxw = cg_slot_variant(d.max_width, rpt);
-    if (grid <= cg_max_workgroups(h->mixed, rpt, maxw, h->device)) break;
-  }
-  if (rpt > 8) return cleanup(fail("mesh too large for the co-resident CG kernel"));
-  h->cg_rpt = rpt;
-  h->cg_maxw = maxw;
-  h->cg_grid = grid;
-  h->cgw.gmax = std::max(grid, 1024);
-  ZEROS(h->cgw.pub, 2 * n);
-  ZEROS(h->cgw.part, 4 * (size_t)h->cgw.gmax);
+  // persistent CG: smallest rows-per-thread whose parts fit one co-resident grid
+  int rc = -1;
+  for (int rpt = 1; rpt <= 4 && rc != 0; rpt *= 2) rc = setup_cg(h, rpt);
+  if (rc != 0) return cleanup(fail("CG set-up failed: " + g_last_error));
+  h->cgw.gmax = 1024;
+  // every layout pads by less than one part of at most 1024 rows
+  ZEROS(h->cgw.pub, 2 * (n + 1024));
+  ZEROS(h->cgw.part, 8 * (size_t)h->cgw.gmax);
   ZEROS(h->cgw.tag, 2 * (size_t)h->cgw.gmax);
   ZEROS(h->sync_nbc, 8);
   ZEROS(h->sync_dbc, 8);
@@ -377,6 +412,7 @@ void nsim_destroy(nsim_handle* h) {
       (void)hipEventDestroy(ev.second);
     }
   for (void* p : h->allocations) (void)hipFree(p);
+  for (void* p : h->cg_allocs) (void)hipFree(p);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
 }
@@ -396,15 +432,13 @@ int nsim_set_tuning(nsim_handle* h, const char* key, int value) {
   if (!h || !key) return fail("nsim_set_tuning: null argument");
   std::string k(key);
   if (k == "cg_rows_per_thread") {
-    if (value != 1 && value != 2 && value != 4 && value != 8)
-      return fail("cg_rows_per_thread must be 1, 2, 4 or 8");
-    int grid = (h->d.n + 256 * value - 1) / (256 * value);
-    int maxw = cg_slot_variant(h->d.max_width, value);
-    if (grid > cg_max_workgroups(h->mixed, value, maxw, h->device) || grid > h->cgw.gmax)
-      return fail("too few rows per thread for a co-resident CG grid");
-    h->cg_rpt = value;
-    h->cg_maxw = maxw;
-    h->cg_grid = grid;
+    if (value != 1 && value != 2 && value != 4)
+      return fail("cg_rows_per_thread must be 1, 2 or 4");
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return setup_cg(h, value);
+  }
+  if (k == "cg_coarse") {
+    h->cg_coarse = value != 0;
     return 0;
   }
   return fail("unknown tuning key: " + k);
@@ -539,11 +573,15 @@ int nsim_get_stat(nsim_handle* h, const char* name, double* out) {
     return 0;
   }
   if (s == "cg_grid") {
-    *out = h->cg_grid;
+    *out = h->cg.parts;
     return 0;
   }
   if (s == "cg_rows_per_thread") {
-    *out = h->cg_rpt;
+    *out = h->cg.rpt;
+    return 0;
+  }
+  if (s == "cg_aggregates") {
+    *out = h->cg_coarse ? h->cg.naggs : 0;
     return 0;
   }
   if (s.rfind("ts_", 0) == 0) {
@@ -563,7 +601,7 @@ int nsim_get_stat(nsim_handle* h, const char* name, double* out) {
     return 0;
   }
   if (s == "cg_register_slots") {
-    *out = h->cg_maxw;
+    *out = h->cg.maxw;
     return 0;
   }
   unsigned a[8], b[8];

@@ -1,16 +1,20 @@
-// Persistent pipelined Jacobi-preconditioned CG for the two Poisson solves of
-// the Fredkin-Koehler demag (solve_neg_laplace_phi and solve_laplace_DBC,
+// Persistent pipelined two-level preconditioned CG for the two Poisson solves
+// of the Fredkin-Koehler demag (solve_neg_laplace_phi and solve_laplace_DBC,
 // interface/nmag/nmag_lam.py:614-633, 744-754).
 //
-// One cooperative launch runs the whole solve.  Workgroups exchange data once
-// per iteration: each publishes its slice of the preconditioned vector that the
-// next SpMV gathers and its two partial inner products, waits for those stores
-// to complete, then publishes a tag; everyone polls the tags of all
-// workgroups.  Exchanged data is written with agent-coherent (sc1) stores and
-// read with sc1 buffer loads, so no L2 write-back or invalidation and no
-// contended atomic is on the critical path, and the gathers of one row are all
-// in flight at once.  Each row's operator slots stay in registers for the
-// whole solve.
+// Rows are reordered so that each workgroup owns one spatially compact part of
+// the mesh (recursive coordinate bisection); consecutive parts form up to 64
+// coarse aggregates.  The preconditioner is additive two-level:
+//     M = D^-1 + P A_c^-1 P^T,   A_c = P^T K P,
+// with P the piecewise-constant prolongation from the aggregates.  P^T w is a
+// per-workgroup partial sum, so it travels with the inner products: one
+// launch runs the whole solve and every iteration costs a single grid-wide
+// exchange.  Each workgroup publishes its slice of D^-1 w (which the next SpMV
+// gathers) and its partial sums, waits for those stores to complete, then
+// publishes a tag; everyone polls the tags of all workgroups.  Exchanged data
+// is written with agent-coherent (sc1) stores and read with sc1 buffer loads,
+// so no L2 write-back or invalidation and no contended atomic is on the
+// critical path, and all gathers of a row are in flight at once.
 #include <hip/hip_runtime.h>
 
 #include "nsim_dev_common.h"
@@ -22,25 +26,42 @@ namespace {
 
 using namespace dev;
 
+constexpr int kMaxAggs = 64;
+constexpr int kMaxParts = 1024;
+
 template <typename TV>
 struct CgArgs {
-  int n, gmax, maxit;
+  int n;           // padded rows (parts * cap)
+  int gmax, maxit;
+  int cap_shift;   // log2(rows per part)
+  int agg_shift;   // log2(parts per aggregate)
+  int naggs;       // 0: Jacobi only
   const int* off;
   const int* col;
   const TV* val;
-  const double* dinv;  // 0 marks a fixed (Dirichlet) row
-  const double* b;     // may be null (zero load)
-  double* x;           // in: warm start (+ Dirichlet values), out: solution
+  const int* p2o;      // row -> original node, -1 on padding rows
+  const double* dinv;  // 0 on padding and fixed (Dirichlet) rows
+  const double* ainv;  // naggs x naggs coarse inverse
+  const double* b;     // original order, may be null (zero load)
+  double* x;           // original order; in: warm start (+ Dirichlet values), out: solution
   double* pub;         // 2n published vector (by phase parity)
-  double* part;        // 2 * gmax * 2 partial sums (by phase parity)
+  double* part;        // 2 * gmax * 4 partial sums (by phase parity)
   unsigned long long* tag;  // 2*gmax phase tags
   unsigned* sync;           // [2] error flag, [3] last its, [4] total its, [5] solves
   unsigned long long epoch; // unique per launch
   double rtol2;
-  const double* xadd;       // optional: xsum = x + xadd
+  const double* xadd;       // optional: xsum = x + xadd (original order)
   double* xsum;
   double* rout;             // optional: rout[blocal[i]] = x_i on boundary nodes
   const int* blocal;
+};
+
+struct Shared {
+  double red[8];
+  double ainv[kMaxAggs * kMaxAggs];
+  double cpart[kMaxParts];
+  double cin[kMaxAggs];
+  double zc[kMaxAggs];
 };
 
 constexpr unsigned kSpinLimit = 1u << 20;
@@ -58,10 +79,13 @@ __device__ inline double load_coh(__amdgpu_buffer_rsrc_t r, int idx) {
 }
 
 // Row slots of one thread: kept in registers (MAXW > 0) or streamed (MAXW == 0).
+// `free_mask` bit k marks a slot whose column is an unknown (receives the
+// coarse correction).
 template <typename TV, int MAXW>
 struct RowSlots {
   int col[MAXW > 0 ? MAXW : 1];
   TV val[MAXW > 0 ? MAXW : 1];
+  unsigned long long free_mask;
   int base, width, lane;
 
   __device__ void load(const CgArgs<TV>& a, int row) {
@@ -69,6 +93,7 @@ struct RowSlots {
     lane = row & 63;
     base = a.off[s];
     width = (a.off[s + 1] - base) >> 6;
+    free_mask = 0;
     if constexpr (MAXW > 0) {
 #pragma unroll
       for (int k = 0; k < MAXW; k++) {
@@ -76,6 +101,7 @@ struct RowSlots {
           const int e = base + (k << 6) + lane;
           col[k] = a.col[e];
           val[k] = a.val[e];
+          if (a.dinv[col[k]] != 0.0) free_mask |= 1ull << k;
         } else {
           col[k] = row;
           val[k] = (TV)0;
@@ -84,15 +110,21 @@ struct RowSlots {
     }
   }
 
-  // sum_j K_ij v_j with v read device-coherently from r at offset `shift`.
-  __device__ double gather(const CgArgs<TV>& a, __amdgpu_buffer_rsrc_t r, int shift) const {
+  // sum_j K_ij (v_j + [j free] zc[agg(j)]), v read device-coherently at `shift`
+  __device__ double gather(const CgArgs<TV>& a, __amdgpu_buffer_rsrc_t r, int shift,
+                           const double* zc, bool coarse) const {
+    const int sh = a.cap_shift + a.agg_shift;
     double acc = 0.0;
     if constexpr (MAXW > 0) {
       double v[MAXW];
 #pragma unroll
       for (int k = 0; k < MAXW; k++) v[k] = load_coh(r, shift + col[k]);
 #pragma unroll
-      for (int k = 0; k < MAXW; k++) acc += (double)val[k] * v[k];
+      for (int k = 0; k < MAXW; k++) {
+        double t = v[k];
+        if (coarse && ((free_mask >> k) & 1ull)) t += zc[col[k] >> sh];
+        acc += (double)val[k] * t;
+      }
     } else {
       for (int k0 = 0; k0 < width; k0 += 8) {
         int c[8];
@@ -107,28 +139,38 @@ struct RowSlots {
 #pragma unroll
         for (int k = 0; k < 8; k++) v[k] = load_coh(r, shift + c[k]);
 #pragma unroll
-        for (int k = 0; k < 8; k++) acc += w8[k] * v[k];
+        for (int k = 0; k < 8; k++) {
+          double t = v[k];
+          if (coarse && w8[k] != 0.0 && a.dinv[c[k]] != 0.0) t += zc[c[k] >> sh];
+          acc += w8[k] * t;
+        }
       }
     }
     return acc;
   }
 };
 
-// Publishes this workgroup's partial sums (s1, s2) of phase `ph` and replaces
-// them with the grid totals.  Every thread must have issued its sc1 stores of
-// the phase's published vector before the call.  Returns false if the exchange
-// timed out or another workgroup reported an error; the spin is bounded so that
-// every workgroup leaves the solver and the grid drains.
+// Publishes this workgroup's partial sums (s1, s2, coarse restriction c and,
+// in the first phase, cb) of phase `ph`; returns the grid totals of s1, s2 and
+// leaves in sm.zc the coarse correction A_c^-1 (sum of the c's per aggregate).
+// Every thread must have issued its sc1
+// stores of the phase's published vector before the call.  Returns false if
+// the exchange timed out or another workgroup reported an error; the spin is
+// bounded so that every workgroup leaves the solver and the grid drains.
 template <typename TV>
 __device__ bool exchange(const CgArgs<TV>& a, __amdgpu_buffer_rsrc_t rpart, unsigned ph,
-                         double& s1, double& s2, double* sh) {
+                         double& s1, double& s2, double c, double cb, Shared& sm) {
   wait_all_memory();
-  block_sum2(s1, s2, sh);
+  block_sum2(s1, s2, sm.red);
+  block_sum2(c, cb, sm.red);
   const int par = ph & 1, G = gridDim.x;
   const unsigned long long tg = (a.epoch << 24) | ph;
   if (threadIdx.x == 0) {
-    st_coh(&a.part[2 * (par * a.gmax + blockIdx.x)], s1);
-    st_coh(&a.part[2 * (par * a.gmax + blockIdx.x) + 1], s2);
+    double* slot = &a.part[4 * (par * a.gmax + blockIdx.x)];
+    st_coh(slot, s1);
+    st_coh(slot + 1, s2);
+    st_coh(slot + 2, c);
+    st_coh(slot + 3, cb);
     wait_all_memory();
     st_coh(&a.tag[par * a.gmax + blockIdx.x], tg);
   }
@@ -144,84 +186,141 @@ __device__ bool exchange(const CgArgs<TV>& a, __amdgpu_buffer_rsrc_t rpart, unsi
       }
     }
     if (!bad) {
-      const int i = 2 * (par * a.gmax + t);
+      const int i = 4 * (par * a.gmax + t);
       const double p1 = load_coh(rpart, i), p2 = load_coh(rpart, i + 1);
+      const double p3 = load_coh(rpart, i + 2);
       v1 += p1;
       v2 += p2;
+      sm.cpart[t] = p3;
     }
   }
   if (__syncthreads_or(bad)) return false;
-  block_sum2(v1, v2, sh);
+  block_sum2(v1, v2, sm.red);
   s1 = v1;
   s2 = v2;
+  if (a.naggs > 0) {
+    const int na = a.naggs;
+    if (threadIdx.x < na) {
+      double sacc = 0.0;
+      const int g0 = threadIdx.x << a.agg_shift;
+      const int g1 = min(G, (threadIdx.x + 1) << a.agg_shift);
+      for (int g = g0; g < g1; g++) sacc += sm.cpart[g];
+      sm.cin[threadIdx.x] = sacc;
+    }
+    __syncthreads();
+    if (threadIdx.x < na) {
+      double z = 0.0;
+      for (int b = 0; b < na; b++) z += sm.ainv[threadIdx.x * na + b] * sm.cin[b];
+      sm.zc[threadIdx.x] = z;
+    }
+    __syncthreads();
+  }
   return true;
 }
 
-// Pipelined Jacobi-PCG (Ghysels & Vanroose 2014): the inner products of
-// iteration i and the vector m_i = D^-1 w_i gathered by the next SpMV travel in
-// the same exchange, so an iteration costs one grid-wide round trip.  Each
-// thread owns RPT rows and keeps their recurrence vectors in registers.  Stops
-// when (r,u) <= rtol^2 (b_eff, D^-1 b_eff), b_eff = b - K x_fixed: the KSP rtol
+// Reads the fourth partial (the load's coarse restriction) of phase 1 into
+// sm.cin and returns cb^T A_c^-1 cb.
+template <typename TV>
+__device__ double load_norm_coarse(const CgArgs<TV>& a, __amdgpu_buffer_rsrc_t rpart,
+                                   Shared& sm) {
+  const int G = gridDim.x, na = a.naggs;
+  for (int t = threadIdx.x; t < G; t += 256) sm.cpart[t] = load_coh(rpart, 4 * (1 * a.gmax + t) + 3);
+  __syncthreads();
+  if (threadIdx.x < na) {
+    double sacc = 0.0;
+    const int g0 = threadIdx.x << a.agg_shift;
+    const int g1 = min(G, (threadIdx.x + 1) << a.agg_shift);
+    for (int g = g0; g < g1; g++) sacc += sm.cpart[g];
+    sm.cin[threadIdx.x] = sacc;
+  }
+  __syncthreads();
+  double q = 0.0, dummy = 0.0;
+  if (threadIdx.x < na) {
+    double z = 0.0;
+    for (int b = 0; b < na; b++) z += sm.ainv[threadIdx.x * na + b] * sm.cin[b];
+    q = z * sm.cin[threadIdx.x];
+  }
+  block_sum2(q, dummy, sm.red);
+  return q;
+}
+
+// Pipelined PCG (Ghysels & Vanroose 2014) with the two-level M: the inner
+// products of iteration i and the vector m_i = M w_i gathered by the next SpMV
+// travel in the same exchange.  Each thread owns RPT rows of its workgroup's
+// part and keeps their recurrence vectors in registers.  Stops when
+// (r, M r) <= rtol^2 (b_eff, M b_eff), b_eff = b - K x_fixed: the KSP rtol
 // measured against the load vector.
 template <typename TV, int RPT, int MAXW>
 __global__ __launch_bounds__(256) void k_cg(CgArgs<TV> a) {
-  __shared__ double sh[8];
-  const int G = gridDim.x, stride = G * 256, n = a.n;
+  __shared__ Shared sm;
+  const int n = a.n;
+  const bool coarse = a.naggs > 0;
   const __amdgpu_buffer_rsrc_t rpub = make_rsrc(a.pub, 2u * (unsigned)n * 8u);
-  const __amdgpu_buffer_rsrc_t rpart = make_rsrc(a.part, 4u * (unsigned)a.gmax * 8u);
+  const __amdgpu_buffer_rsrc_t rpart = make_rsrc(a.part, 8u * (unsigned)a.gmax * 8u);
+  for (int i = threadIdx.x; i < a.naggs * a.naggs; i += 256) sm.ainv[i] = a.ainv[i];
+  const int agg_own = blockIdx.x >> a.agg_shift;
   RowSlots<TV, MAXW> rs[RPT];
   double x[RPT], r[RPT], u[RPT], w[RPT], z[RPT], q[RPT], s[RPT], p[RPT], di[RPT], mk[RPT];
-  double lbn = 0.0, dummy = 0.0;
-  // phase 1: r0 = b - K x0, u0 = D^-1 r0 (published), |b_eff|
+  int orig[RPT];
+  double lbn = 0.0, dummy = 0.0, cr = 0.0, cb = 0.0;
+  // phase 1: r0 = b - K x0, publish D^-1 r0 and the coarse restrictions of r0, b_eff
 #pragma unroll
   for (int k = 0; k < RPT; k++) {
-    const int row = blockIdx.x * 256 + threadIdx.x + k * stride;
+    const int row = (blockIdx.x * RPT + k) * 256 + threadIdx.x;
     x[k] = r[k] = u[k] = w[k] = z[k] = q[k] = s[k] = p[k] = di[k] = mk[k] = 0.0;
+    orig[k] = row < n ? a.p2o[row] : -1;
     rs[k].load(a, row < n ? row : n - 1);
-    if (row < n) {
+    if (orig[k] >= 0) {
       di[k] = a.dinv[row];
-      x[k] = a.x[row];
+      x[k] = a.x[orig[k]];
       if (di[k] != 0.0) {
         double ax = 0.0, af = 0.0;
         for (int t = 0; t < rs[k].width; t++) {
           const int e = rs[k].base + (t << 6) + rs[k].lane;
           const int j = a.col[e];
-          const double pr = (double)a.val[e] * a.x[j];
+          const double pr = (double)a.val[e] * a.x[a.p2o[j]];
           ax += pr;
           if (a.dinv[j] == 0.0) af += pr;
         }
-        const double bi = a.b ? a.b[row] : 0.0;
+        const double bi = a.b ? a.b[orig[k]] : 0.0;
         r[k] = bi - ax;
         const double be = bi - af;
         lbn += be * be * di[k];
+        cr += r[k];
+        cb += be;
       }
-      u[k] = di[k] * r[k];
-      st_coh(&a.pub[1 * n + row], u[k]);
     }
+    if (row < n) st_coh(&a.pub[1 * n + row], di[k] * r[k]);
   }
   int it = 0;
-  const bool alive = exchange(a, rpart, 1u, lbn, dummy, sh);
+  const bool alive = exchange(a, rpart, 1u, lbn, dummy, cr, cb, sm);
   if (alive) {
-    const double stop = a.rtol2 * lbn;
+    const double bnorm = lbn + (coarse ? load_norm_coarse(a, rpart, sm) : 0.0);
+    const double stop = a.rtol2 * bnorm;
+    const double zown = coarse ? sm.zc[agg_own] : 0.0;
 #pragma unroll
     for (int k = 0; k < RPT; k++) {
-      const int row = blockIdx.x * 256 + threadIdx.x + k * stride;
-      if (row < n && di[k] != 0.0) w[k] = rs[k].gather(a, rpub, n);
+      const int row = (blockIdx.x * RPT + k) * 256 + threadIdx.x;
+      if (di[k] != 0.0) {
+        u[k] = di[k] * r[k] + zown;
+        w[k] = rs[k].gather(a, rpub, n, sm.zc, coarse);
+      }
+      (void)row;
     }
     double gam_old = 1.0, alpha_old = 1.0;
     for (;; it++) {
       const unsigned ph = (unsigned)it + 2u;
       const int par = ph & 1;
-      double gam = 0.0, del = 0.0;
+      double gam = 0.0, del = 0.0, cw = 0.0;
 #pragma unroll
       for (int k = 0; k < RPT; k++) {
-        const int row = blockIdx.x * 256 + threadIdx.x + k * stride;
+        const int row = (blockIdx.x * RPT + k) * 256 + threadIdx.x;
         gam += r[k] * u[k];
         del += w[k] * u[k];
-        mk[k] = di[k] * w[k];
-        if (row < n) st_coh(&a.pub[par * n + row], mk[k]);
+        if (di[k] != 0.0) cw += w[k];
+        if (row < n) st_coh(&a.pub[par * n + row], di[k] * w[k]);
       }
-      if (!exchange(a, rpart, ph, gam, del, sh)) break;
+      if (!exchange(a, rpart, ph, gam, del, cw, 0.0, sm)) break;
       if (!(gam > stop) || gam <= 1e-300 || it >= a.maxit) break;
       if (!isfinite(gam) || !isfinite(del)) {
         if (threadIdx.x == 0) st_coh(&a.sync[2], 2u);
@@ -229,11 +328,16 @@ __global__ __launch_bounds__(256) void k_cg(CgArgs<TV> a) {
       }
       const double beta = it > 0 ? gam / gam_old : 0.0;
       const double alpha = it > 0 ? gam / (del - beta * gam / alpha_old) : gam / del;
+      const double zw = coarse ? sm.zc[agg_own] : 0.0;
 #pragma unroll
       for (int k = 0; k < RPT; k++) {
-        const int row = blockIdx.x * 256 + threadIdx.x + k * stride;
         double nk = 0.0;
-        if (row < n && di[k] != 0.0) nk = rs[k].gather(a, rpub, par * n);
+        if (di[k] != 0.0) {
+          mk[k] = di[k] * w[k] + zw;
+          nk = rs[k].gather(a, rpub, par * n, sm.zc, coarse);
+        } else {
+          mk[k] = 0.0;
+        }
         z[k] = nk + beta * z[k];
         q[k] = mk[k] + beta * q[k];
         s[k] = w[k] + beta * s[k];
@@ -249,12 +353,12 @@ __global__ __launch_bounds__(256) void k_cg(CgArgs<TV> a) {
   }
 #pragma unroll
   for (int k = 0; k < RPT; k++) {
-    const int row = blockIdx.x * 256 + threadIdx.x + k * stride;
-    if (row < n) {
-      a.x[row] = x[k];
-      if (a.xsum) a.xsum[row] = x[k] + a.xadd[row];
+    const int o = orig[k];
+    if (o >= 0) {
+      a.x[o] = x[k];
+      if (a.xsum) a.xsum[o] = x[k] + a.xadd[o];
       if (a.rout) {
-        const int lb = a.blocal[row];
+        const int lb = a.blocal[o];
         if (lb >= 0) a.rout[lb] = x[k];
       }
     }
@@ -266,8 +370,6 @@ __global__ __launch_bounds__(256) void k_cg(CgArgs<TV> a) {
   }
 }
 
-// The kernel variants: (rows per thread, register slots).  Register slots need
-// every slice of the operator to be at most MAXW wide.
 template <typename TV, int RPT, int MAXW>
 const void* kernel_ptr() {
   return reinterpret_cast<const void*>(&k_cg<TV, RPT, MAXW>);
@@ -285,14 +387,12 @@ const void* select_kernel(int rpt, int maxw) {
     if (rpt == 1) return kernel_ptr<TV, 1, 0>();
     if (rpt == 2) return kernel_ptr<TV, 2, 0>();
     if (rpt == 4) return kernel_ptr<TV, 4, 0>();
-    if (rpt == 8) return kernel_ptr<TV, 8, 0>();
   }
   return nullptr;
 }
 
 }  // namespace
 
-// Register-slot width used for an operator whose widest slice has `width` slots.
 int cg_slot_variant(int width, int rpt) {
   if (rpt <= 2 && width <= 16) return 16;
   if (rpt <= 2 && width <= 32) return 32;
@@ -309,26 +409,32 @@ int cg_max_workgroups(bool mixed, int rpt, int maxw, int device) {
   return per_cu * cus;
 }
 
-hipError_t launch_cg(const DeviceOps& d, bool mixed, CgWork& w, int grid, int rpt, int maxw,
-                     const double* dinv, const double* b, double* x, double rtol, int maxit,
-                     const double* xadd, double* xsum, double* restrict_out,
+hipError_t launch_cg(const CgDevice& c, bool mixed, CgWork& w, bool dirichlet, bool coarse,
+                     const double* b, double* x, double rtol, int maxit, const double* xadd,
+                     double* xsum, double* restrict_out, const int* blocal,
                      hipStream_t st) {
-  if (grid <= 0 || grid > w.gmax || (long long)grid * 256 * rpt < d.n || maxit >= (1 << 23) ||
-      (maxw > 0 && d.max_width > maxw) || 2LL * d.n * 8 >= 0x7fffffffLL)
+  const int grid = c.parts;
+  if (grid <= 0 || grid > w.gmax || grid > kMaxParts || maxit >= (1 << 23) ||
+      (c.maxw > 0 && c.max_width > c.maxw) || c.naggs > kMaxAggs ||
+      2LL * c.npad * 8 >= 0x7fffffffLL || (256 << 0) * c.rpt != (1 << c.cap_shift))
     return hipErrorInvalidConfiguration;
   const unsigned long long epoch = ++w.epoch;
-  const void* k = mixed ? select_kernel<float>(rpt, maxw) : select_kernel<double>(rpt, maxw);
+  const void* k = mixed ? select_kernel<float>(c.rpt, c.maxw)
+                        : select_kernel<double>(c.rpt, c.maxw);
   if (!k) return hipErrorInvalidConfiguration;
+  const double* dinv = dirichlet ? c.dinv_dbc : c.dinv_nbc;
+  const double* ainv = dirichlet ? c.ainv_dbc : c.ainv_nbc;
+  const int naggs = coarse ? c.naggs : 0;
   if (mixed) {
-    CgArgs<float> a{d.n,   w.gmax, maxit, d.sell_off, d.sell_col, d.Kf,        dinv,
-                    b,     x,      w.pub, w.part,     w.tag,      w.sync,      epoch,
-                    rtol * rtol, xadd, xsum, restrict_out, d.blocal};
+    CgArgs<float> a{c.npad, w.gmax, maxit, c.cap_shift, c.agg_shift, naggs, c.sell_off,
+                    c.sell_col, c.Kf, c.p2o, dinv, ainv, b, x, w.pub, w.part, w.tag, w.sync,
+                    epoch, rtol * rtol, xadd, xsum, restrict_out, blocal};
     void* args[] = {&a};
     return hipLaunchCooperativeKernel(k, dim3(grid), dim3(256), args, 0, st);
   }
-  CgArgs<double> a{d.n,   w.gmax, maxit, d.sell_off, d.sell_col, d.K,         dinv,
-                   b,     x,      w.pub, w.part,     w.tag,      w.sync,      epoch,
-                   rtol * rtol, xadd, xsum, restrict_out, d.blocal};
+  CgArgs<double> a{c.npad, w.gmax, maxit, c.cap_shift, c.agg_shift, naggs, c.sell_off,
+                   c.sell_col, c.K, c.p2o, dinv, ainv, b, x, w.pub, w.part, w.tag, w.sync,
+                   epoch, rtol * rtol, xadd, xsum, restrict_out, blocal};
   void* args[] = {&a};
   return hipLaunchCooperativeKernel(k, dim3(grid), dim3(256), args, 0, st);
 }

@@ -71,10 +71,23 @@ struct FieldBufs {
 hipError_t launch_bem_assemble(const DeviceOps& d, hipStream_t st);
 hipError_t launch_exch_div(const DeviceOps& d, bool mixed, const Coeffs& c, const double* m,
                            double* Hexch, double* rho, bool want_rho, hipStream_t st);
-// Persistent pipelined Jacobi-PCG on the stiffness matrix; see the kernel.
-hipError_t launch_cg(const DeviceOps& d, bool mixed, CgWork& w, int grid, int rpt, int maxw,
-                     const double* dinv, const double* b, double* x, double rtol, int maxit,
-                     const double* xadd, double* xsum, double* restrict_out,
+// The CG's reordered stiffness operator in HBM (see CgOperator).
+struct CgDevice {
+  int parts = 0, npad = 0, rpt = 1, cap_shift = 8, agg_shift = 0, naggs = 0;
+  int maxw = 0, max_width = 0;
+  int* sell_off = nullptr;
+  int* sell_col = nullptr;
+  double* K = nullptr;
+  float* Kf = nullptr;
+  int* p2o = nullptr;
+  double *dinv_nbc = nullptr, *dinv_dbc = nullptr;
+  double *ainv_nbc = nullptr, *ainv_dbc = nullptr;
+};
+// Persistent pipelined two-level PCG on the stiffness matrix; see the kernel.
+// b, x, xadd, xsum are in the original node order.
+hipError_t launch_cg(const CgDevice& c, bool mixed, CgWork& w, bool dirichlet, bool coarse,
+                     const double* b, double* x, double rtol, int maxit, const double* xadd,
+                     double* xsum, double* restrict_out, const int* blocal,
                      hipStream_t st);
 hipError_t launch_bem_gemv(const DeviceOps& d, const double* phi1b, double* phi2,
                            hipStream_t st, int num_cus);

@@ -123,6 +123,8 @@ int Integrator::init(double t0) {
   hnext_ = 0.0;
   qnext_ = 1;
   gamma_pc_ = -1.0;
+  crate_ = 1.0;
+  gamma_crate_ = 0.0;
   st_ = TsStats();
   st_.t = t0;
   return 0;
@@ -267,7 +269,12 @@ int Integrator::step_attempt(double h, int q, bool* accepted, double* err_out) {
   HTRY(vec_lincomb(s_, n3_, -gamma_ * dl[1], Y[1], q >= 2 ? -gamma_ * dl[2] : 0.0,
                    q >= 2 ? Y[2] : nullptr, 0, nullptr, 0, nullptr, avec_));
   HTRY(hipMemcpyAsync(ycur_, yp_, sizeof(double) * n3_, hipMemcpyDeviceToDevice, s_));
-  double del_old = 0.0, crate = 1.0;
+  // The convergence-rate estimate carries over between steps (as CVODE's
+  // crate does) unless gamma changed a lot, so a small first correction can
+  // be accepted without a second rhs evaluation.
+  if (gamma_crate_ <= 0.0 || std::fabs(gamma_ / gamma_crate_ - 1.0) > 0.3) crate_ = 1.0;
+  gamma_crate_ = gamma_;
+  double del_old = 0.0, crate = crate_;
   bool converged = false;
   for (int k = 0; k < kMaxCor; k++) {
     TRY(device_rhs(h_, ycur_, f_));
@@ -291,8 +298,10 @@ int Integrator::step_attempt(double h, int q, bool* accepted, double* err_out) {
     if (k > 0 && d > 2.0 * del_old) break;
     del_old = d;
   }
+  crate_ = crate;
   if (!converged) {
     st_.conv_fails++;
+    crate_ = 1.0;
     hnext_ = 0.25 * h;
     return 0;
   }

@@ -83,6 +83,8 @@ class Integrator {
   double* part_ = nullptr;  // reduction partials
   std::vector<double> host_part_;
   double gamma_ = 0.0;      // current Newton matrix coefficient
+  double crate_ = 1.0;      // Newton convergence-rate estimate
+  double gamma_crate_ = 0.0;  // gamma at which crate_ was measured
   double gamma_pc_ = -1.0;  // gamma the preconditioner was set up for
   std::vector<void*> allocs_;
 };

@@ -44,4 +44,22 @@ struct HostOperators {
 std::string build_host_operators(int n, const double* coords, int T, const int* tets,
                                  HostOperators& ops);
 
+// The stiffness operator in the CG's own row order: the nodes are split by
+// recursive coordinate bisection into `parts` spatially compact parts of at
+// most `cap` nodes; part g occupies rows [g*cap, g*cap + size_g) and the rest
+// of its rows are inactive padding.  Consecutive parts are merged in groups of
+// 2^agg_shift into at most 64 coarse aggregates; ainv_* are the inverses of the
+// Galerkin coarse matrices P^T K P (Neumann: regularised on the constants;
+// Dirichlet: free rows and columns only).
+struct CgOperator {
+  int cap = 256, parts = 0, npad = 0, nslices = 0, max_width = 0;
+  int agg_shift = 0, naggs = 0;
+  std::vector<int> p2o;        // npad: original node or -1
+  std::vector<int> sell_off, sell_col;
+  std::vector<double> sK;
+  std::vector<double> dinv_nbc, dinv_dbc;  // npad, 0 on inactive / fixed rows
+  std::vector<double> ainv_nbc, ainv_dbc;  // naggs x naggs
+};
+std::string build_cg_operator(const HostOperators& o, int cap, CgOperator& cg);
+
 }  // namespace nsim
