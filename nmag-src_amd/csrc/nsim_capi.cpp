@@ -614,6 +614,7 @@ int nsim_get_stat(nsim_handle* h, const char* name, double* out) {
   else if (s == "cg_iters_dbc_total") *out = b[4];
   else if (s == "cg_solves") *out = a[5] + b[5];
   else if (s == "cg_error") *out = (double)(a[2] | b[2]);
+  else if (s == "cg_unconverged") *out = (double)(a[6] + b[6]);
   else return fail("nsim_get_stat: unknown statistic '" + s + "'");
   return 0;
 }

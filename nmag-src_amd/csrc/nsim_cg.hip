@@ -47,7 +47,8 @@ struct CgArgs {
   double* pub;         // 2n published vector (by phase parity)
   double* part;        // 2 * gmax * 4 partial sums (by phase parity)
   unsigned long long* tag;  // 2*gmax phase tags
-  unsigned* sync;           // [2] error flag, [3] last its, [4] total its, [5] solves
+  unsigned* sync;           // [2] error flag, [3] last its, [4] total its, [5] solves,
+                            // [6] solves stopped at maxit
   unsigned long long epoch; // unique per launch
   double rtol2;
   const double* xadd;       // optional: xsum = x + xadd (original order)
@@ -293,6 +294,7 @@ __global__ __launch_bounds__(256) void k_cg(CgArgs<TV> a) {
     if (row < n) st_coh(&a.pub[1 * n + row], di[k] * r[k]);
   }
   int it = 0;
+  unsigned unconverged = 0u;
   const bool alive = exchange(a, rpart, 1u, lbn, dummy, cr, cb, sm);
   if (alive) {
     const double bnorm = lbn + (coarse ? load_norm_coarse(a, rpart, sm) : 0.0);
@@ -321,7 +323,11 @@ __global__ __launch_bounds__(256) void k_cg(CgArgs<TV> a) {
         if (row < n) st_coh(&a.pub[par * n + row], di[k] * w[k]);
       }
       if (!exchange(a, rpart, ph, gam, del, cw, 0.0, sm)) break;
-      if (!(gam > stop) || gam <= 1e-300 || it >= a.maxit) break;
+      if (!(gam > stop) || gam <= 1e-300) break;
+      if (it >= a.maxit) {
+        unconverged = 1u;
+        break;
+      }
       if (!isfinite(gam) || !isfinite(del)) {
         if (threadIdx.x == 0) st_coh(&a.sync[2], 2u);
         break;
@@ -367,6 +373,7 @@ __global__ __launch_bounds__(256) void k_cg(CgArgs<TV> a) {
     a.sync[3] = (unsigned)it;
     a.sync[4] += (unsigned)it;
     a.sync[5] += 1u;
+    a.sync[6] += unconverged;
   }
 }
 

@@ -154,9 +154,15 @@ std::string build_cg_operator(const HostOperators& o, int cap, CgOperator& cg) {
       if (fi && o.blocal[j] < 0) Ad[a * na + b] += o.K[k];
     }
   }
-  double tr = 0.0;
+  // The Neumann coarse matrix is singular along the constants e.  With
+  // tau > 0, (A + (tau/na) e e^T)^-1 = A^+ + e e^T / (tau na), so subtracting
+  // 1/(tau na) from every entry gives the pseudo-inverse: no coarse
+  // correction along the constants, whatever tau.  tau is kept at the scale of
+  // the operator (never the roundoff-sized trace of a single aggregate).
+  double tr = 0.0, kd = 0.0;
   for (int a = 0; a < na; a++) tr += An[a * na + a];
-  const double tau = tr / na;  // lifts the constant null space of the Neumann problem
+  for (int i = 0; i < n; i++) kd += o.Kdiag[i];
+  const double tau = std::max(tr / na, kd / n);
   for (int a = 0; a < na; a++)
     for (int b = 0; b < na; b++) An[a * na + b] += tau / na;
   for (int a = 0; a < na; a++)
@@ -165,6 +171,7 @@ std::string build_cg_operator(const HostOperators& o, int cap, CgOperator& cg) {
       Ad[a * na + a] = 1.0;
     }
   if (!spd_inverse(An, na)) return "Neumann coarse matrix is not positive definite";
+  for (double& v : An) v -= 1.0 / (tau * na);
   if (!spd_inverse(Ad, na)) return "Dirichlet coarse matrix is not positive definite";
   for (int a = 0; a < na; a++)
     if (!has_free[a]) Ad[a * na + a] = 0.0;  // no coarse correction there

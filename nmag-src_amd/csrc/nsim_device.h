@@ -45,6 +45,7 @@ struct CgWork {
   unsigned long long* tag = nullptr;  // 2*gmax phase tags
   unsigned* sync = nullptr;         // [2] error flag [3] last iterations
                                     // [4] total iterations [5] solves
+                                    // [6] solves stopped at maxit
   int gmax = 0;
   unsigned long long epoch = 0;     // launch counter; makes every phase tag unique
 };

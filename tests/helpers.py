@@ -39,5 +39,5 @@ def make_pair(nx, ny, nz, h, mat=None, precision="fp64", rtol=1e-13, build_bem=T
     coords, tets = kuhn_cuboid(nx, ny, nz, h)
     orc = oracle.OracleMesh(coords, tets, build_bem=build_bem)
     gpu = NativeMagnet(coords, tets, mat.su_coeffs(), precision=precision, nbc_rtol=rtol,
-                       dbc_rtol=rtol, nbc_maxits=100000, dbc_maxits=100000)
+                       dbc_rtol=rtol, nbc_maxits=20000, dbc_maxits=20000)
     return coords, tets, orc, gpu, mat

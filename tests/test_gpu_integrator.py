@@ -18,7 +18,7 @@ pytestmark = pytest.mark.gpu
 def _magnet(nx, ny, nz, h, mat, rtol=1e-11, do_demag=True):
     coords, tets = kuhn_cuboid(nx, ny, nz, h)
     g = NativeMagnet(coords, tets, mat.su_coeffs(), do_demag=do_demag, nbc_rtol=rtol,
-                     dbc_rtol=rtol, nbc_maxits=100000, dbc_maxits=100000)
+                     dbc_rtol=rtol, nbc_maxits=5000, dbc_maxits=5000)
     return coords, tets, g
 
 
@@ -74,6 +74,7 @@ def test_trajectory_matches_oracle_rhs_integrated_by_scipy():
     sol = solve_ivp(f, (0.0, T), m0.reshape(-1), method="DOP853", rtol=1e-10, atol=1e-12)
     assert sol.success
     g.advance(T)
+    assert g.stat("cg_unconverged") == 0
     m = g.get_field("m")
     ref = sol.y[:, -1].reshape(-1, 3)
     assert np.abs(m - ref).max() < 2e-6

@@ -106,7 +106,27 @@ def test_warm_start_reduces_iterations():
     gpu.execute()
     warm = gpu.stat("cg_iters_nbc")
     assert cold > 5 and warm <= 1
-    assert gpu.stat("cg_error") == 0
+    assert gpu.stat("cg_error") == 0 and gpu.stat("cg_unconverged") == 0
+
+
+@pytest.mark.parametrize("dims", [(1, 1, 1, 2.5), (6, 5, 3, 2.5), (40, 20, 2, 2.5)])
+def test_two_level_preconditioner_matches_jacobi_solution(dims):
+    """The coarse correction changes the iteration, not the solution; on one
+    part (a single aggregate) it must vanish for the Neumann problem."""
+    nx, ny, nz, h = dims
+    _, _, orc, gpu, mat = make_pair(nx, ny, nz, h, rtol=1e-12)
+    m = random_m(orc.n, 13)
+    a = _fields_gpu(gpu, m)
+    its2 = gpu.stat("cg_iters_nbc")
+    gpu.tune("cg_coarse", 0)
+    gpu.set_field("phi1", np.zeros(orc.n))
+    gpu.set_field("phi2", np.zeros(orc.n))
+    b = _fields_gpu(gpu, m)
+    its1 = gpu.stat("cg_iters_nbc")
+    assert rel_err(a["H_demag"], b["H_demag"]) < 1e-10
+    assert gpu.stat("cg_unconverged") == 0
+    if gpu.stat("cg_grid") > 2:
+        assert its2 < its1
 
 
 def test_demag_linear_and_uniform_cuboid_factors():
