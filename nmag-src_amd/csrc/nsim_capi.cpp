@@ -294,6 +294,10 @@ int nsim_create(const nsim_mesh* mesh, const nsim_material* mat, const nsim_opti
   d.btype = h->opt.precision == NSIM_FP64 ? kBemF64
             : h->opt.precision == NSIM_MIXED_FP32 ? kBemF32 : kBemBF16;
   d.geo_eps = o.geo_eps;
+  if (h->opt.do_demag && (size_t)d.ldb * (d.btype == kBemBF16 ? 4 : 8) > 64 * 1024)
+    return cleanup(fail("nsim_create: " + std::to_string(o.nb) +
+                        " boundary nodes exceed the BEM GEMV's LDS staging (8192 at FP64/FP32 "
+                        "storage, 16384 at BF16)"));
 
   hipError_t e;
 #define UP(dst, vec)                                                        \
