@@ -64,6 +64,22 @@ int nsim_create(const nsim_mesh* mesh, const nsim_material* mat,
 void nsim_destroy(nsim_handle* h);
 const char* nsim_last_error(void);
 
+/* Batched form of nsim_create: n_magnets independent magnets (meshes[k],
+ * mats[k]) in one handle.  Replaces one raw_make_lam per magnet
+ * (src/nsimexec.ml:3605), the "batched variants take an instance-offset
+ * table" form of the boundary.  The magnets' nodes are concatenated in order:
+ * every field vector of the handle is the concatenation of the magnets'
+ * vectors, and one rhs evaluation evaluates every magnet (grouped SpMV, one
+ * persistent CG over all magnets, one grouped BEM GEMV with a dense matrix per
+ * magnet).  The CG stopping test is on the residual of the whole batch. */
+int nsim_create_batch(int n_magnets, const nsim_mesh* meshes, const nsim_material* mats,
+                      const nsim_options* opt, nsim_handle** out);
+/* Instance-offset table of a handle: magnet k owns nodes
+ * [node_off[k], node_off[k+1]) and boundary nodes [boundary_off[k],
+ * boundary_off[k+1]); both arrays have n_magnets + 1 entries (either may be
+ * null).  A handle from nsim_create is a batch of one. */
+int nsim_get_batch_layout(nsim_handle* h, int* node_off, int* boundary_off);
+
 /* sizes[0..5] = n_nodes, n_boundary_nodes, n_boundary_triangles, n_tets,
  * stored operator entries (padded SELL slots), stiffness non-zeros. */
 int nsim_get_sizes(nsim_handle* h, int* sizes);
@@ -80,7 +96,8 @@ int nsim_get_field(nsim_handle* h, const char* name, double* host_out);
  * "update_dmdt", "set_H_demag", "set_H_exch", "set_H_anis", "set_H_total". */
 int nsim_execute(nsim_handle* h, const char* program);
 
-/* Dense BEM matrix (N_b x N_b, row-major, FP64 copy) for inspection. */
+/* Dense BEM matrix (N_b x N_b, row-major, FP64 copy) for inspection; for a
+ * batch, the magnets' matrices one after another (each N_b,k x N_b,k). */
 int nsim_get_bem(nsim_handle* h, double* host_out);
 
 /* CVODE rhs callback (src/sundials_sp_stubs.c:264, src/nsim.ml:2273-2313):

@@ -182,25 +182,38 @@ __device__ inline void load_row_chunk<uint16_t, float>(const uint16_t* p, float*
   }
 }
 
-// phi2[bnode[i]] = sum_j B_ij phi1b_j.  The input vector is staged once per
-// workgroup in LDS; each wavefront streams ROWS rows of B at a time with
-// 16-byte loads per lane and reduces with cross-lane adds.  Rows are split
-// evenly over all wavefronts of the grid.
+// phi2[bnode[i]] = sum_j B_ij phi1b_j over a block-diagonal set of dense
+// matrices (one per independent magnet of a batch; one for a single magnet).
+// A table maps every workgroup to a segment and a run of its rows.  The
+// segment's input vector is staged once per workgroup in LDS; each wavefront
+// streams ROWS rows of B at a time with 16-byte loads per lane and reduces with
+// cross-lane adds.
 template <typename TB, int ROWS>
-__global__ __launch_bounds__(256) void k_bem_gemv(const TB* __restrict__ B, int ldb, int nb,
+__global__ __launch_bounds__(256) void k_bem_gemv(const TB* __restrict__ Bbase,
+                                                  const long long* __restrict__ seg_Boff,
+                                                  const int* __restrict__ seg_ldb,
+                                                  const int* __restrict__ seg_nb,
+                                                  const int* __restrict__ seg_boff,
+                                                  const int* __restrict__ wg_seg,
+                                                  const int* __restrict__ wg_row0,
                                                   int rows_per_wave,
-                                                  const double* __restrict__ xb,
-                                                  const int* __restrict__ bnode,
+                                                  const double* __restrict__ xball,
+                                                  const int* __restrict__ bnodeall,
                                                   double* __restrict__ y) {
   using X = typename BemVec<TB>::X;
   using A = typename BemVec<TB>::A;
   constexpr int V = BemVec<TB>::V;
   extern __shared__ __align__(16) unsigned char smem[];
   X* xs = reinterpret_cast<X*>(smem);
+  const int sg = wg_seg[blockIdx.x];
+  const int ldb = seg_ldb[sg], nb = seg_nb[sg], boff = seg_boff[sg];
+  const TB* B = Bbase + seg_Boff[sg];
+  const double* xb = xball + boff;
+  const int* bnode = bnodeall + boff;
   for (int j = threadIdx.x; j < ldb; j += 256) xs[j] = j < nb ? (X)xb[j] : (X)0;
   __syncthreads();
-  const int wave = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
-  const int r0 = wave * rows_per_wave;
+  const int lane = threadIdx.x & 63;
+  const int r0 = wg_row0[blockIdx.x] + (threadIdx.x >> 6) * rows_per_wave;
   const int r1 = min(r0 + rows_per_wave, nb);
   for (int rb = r0; rb < r1; rb += ROWS) {
     const TB* rp[ROWS];
@@ -231,52 +244,75 @@ __global__ __launch_bounds__(256) void k_bem_gemv(const TB* __restrict__ B, int 
   }
 }
 
+constexpr int kGemvRows = 2;
+
 template <typename TB>
-hipError_t gemv_typed(const DeviceOps& d, const double* phi1b, double* phi2, hipStream_t st,
-                      int num_cus) {
+hipError_t gemv_typed(const DeviceOps& d, const double* phi1b, double* phi2, hipStream_t st) {
   using X = typename BemVec<TB>::X;
-  constexpr int ROWS = 2;
-  const size_t lds = (size_t)d.ldb * sizeof(X);
+  const size_t lds = (size_t)d.max_ldb * sizeof(X);
   if (lds > 64 * 1024) return hipErrorInvalidValue;
-  // two workgroups per CU, rows split evenly over the wavefronts
-  const int waves = 4 * 2 * (num_cus > 0 ? num_cus : 256);
-  int rpw = (d.nb + waves - 1) / waves;
-  rpw = ((rpw + ROWS - 1) / ROWS) * ROWS;
-  const int used_waves = (d.nb + rpw - 1) / rpw;
-  const int grid = (used_waves + 3) / 4;
-  k_bem_gemv<TB, ROWS><<<grid, 256, lds, st>>>((const TB*)d.B, d.ldb, d.nb, rpw, phi1b,
-                                               d.bnode, phi2);
+  k_bem_gemv<TB, kGemvRows><<<d.gemv_grid, 256, lds, st>>>(
+      (const TB*)d.B, d.seg_Boff, d.seg_ldb, d.seg_nb, d.seg_boff, d.wg_seg, d.wg_row0,
+      d.gemv_rpw, phi1b, d.bnode, phi2);
   return hipGetLastError();
+}
+
+template <typename TB>
+hipError_t assemble_typed(const DeviceOps& d, hipStream_t st) {
+  for (size_t s = 0; s < d.h_seg_nb.size(); s++) {
+    const int nb = d.h_seg_nb[s], ldb = d.h_seg_ldb[s], bo = d.h_seg_boff[s];
+    const int cb = (ldb + 255) / 256;
+    const long long nblk = (long long)cb * nb;
+    if (nblk <= 0) continue;
+    if (nblk > 0x7fffffffLL) return hipErrorInvalidValue;
+    k_bem_assemble<TB><<<(int)nblk, 256, 0, st>>>(
+        nb, ldb, cb, d.coords, d.bnode + bo, d.tri_node, d.tnorm, d.inc_rp + bo, d.inc_tri,
+        d.omega + bo, d.h_seg_geo_eps[s], (TB*)d.B + d.h_seg_Boff[s]);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
 }
 
 }  // namespace
 
+void plan_bem_gemv(DeviceOps& d, int num_cus, std::vector<int>* wg_seg,
+                   std::vector<int>* wg_row0) {
+  // two workgroups of four wavefronts per CU; every wavefront gets the same
+  // number of rows (a multiple of kGemvRows), every workgroup one segment
+  long long rows = 0;
+  for (int nb : d.h_seg_nb) rows += nb;
+  const long long waves = 4LL * 2 * (num_cus > 0 ? num_cus : 256);
+  int rpw = (int)((rows + waves - 1) / waves);
+  rpw = std::max(kGemvRows, ((rpw + kGemvRows - 1) / kGemvRows) * kGemvRows);
+  const int rows_per_wg = 4 * rpw;
+  wg_seg->clear();
+  wg_row0->clear();
+  d.max_ldb = 0;
+  for (size_t s = 0; s < d.h_seg_nb.size(); s++) {
+    d.max_ldb = std::max(d.max_ldb, d.h_seg_ldb[s]);
+    for (int r = 0; r < d.h_seg_nb[s]; r += rows_per_wg) {
+      wg_seg->push_back((int)s);
+      wg_row0->push_back(r);
+    }
+  }
+  d.gemv_rpw = rpw;
+  d.gemv_grid = (int)wg_seg->size();
+}
+
 hipError_t launch_bem_assemble(const DeviceOps& d, hipStream_t st) {
-  const int cb = (d.ldb + 255) / 256;
-  const long long nblk = (long long)cb * d.nb;
-  if (nblk <= 0) return hipSuccess;
-  if (nblk > 0x7fffffffLL) return hipErrorInvalidValue;
-  if (d.btype == kBemF64)
-    k_bem_assemble<double><<<(int)nblk, 256, 0, st>>>(
-        d.nb, d.ldb, cb, d.coords, d.bnode, d.tri_node, d.tnorm, d.inc_rp, d.inc_tri,
-        d.omega, d.geo_eps, (double*)d.B);
-  else if (d.btype == kBemF32)
-    k_bem_assemble<float><<<(int)nblk, 256, 0, st>>>(
-        d.nb, d.ldb, cb, d.coords, d.bnode, d.tri_node, d.tnorm, d.inc_rp, d.inc_tri,
-        d.omega, d.geo_eps, (float*)d.B);
-  else
-    k_bem_assemble<uint16_t><<<(int)nblk, 256, 0, st>>>(
-        d.nb, d.ldb, cb, d.coords, d.bnode, d.tri_node, d.tnorm, d.inc_rp, d.inc_tri,
-        d.omega, d.geo_eps, (uint16_t*)d.B);
-  return hipGetLastError();
+  if (d.btype == kBemF64) return assemble_typed<double>(d, st);
+  if (d.btype == kBemF32) return assemble_typed<float>(d, st);
+  return assemble_typed<uint16_t>(d, st);
 }
 
 hipError_t launch_bem_gemv(const DeviceOps& d, const double* phi1b, double* phi2,
                            hipStream_t st, int num_cus) {
-  if (d.nb == 0) return hipSuccess;
-  if (d.btype == kBemF64) return gemv_typed<double>(d, phi1b, phi2, st, num_cus);
-  if (d.btype == kBemF32) return gemv_typed<float>(d, phi1b, phi2, st, num_cus);
-  return gemv_typed<uint16_t>(d, phi1b, phi2, st, num_cus);
+  (void)num_cus;
+  if (d.nb == 0 || d.gemv_grid == 0) return hipSuccess;
+  if (d.btype == kBemF64) return gemv_typed<double>(d, phi1b, phi2, st);
+  if (d.btype == kBemF32) return gemv_typed<float>(d, phi1b, phi2, st);
+  return gemv_typed<uint16_t>(d, phi1b, phi2, st);
 }
 
 }  // namespace nsim

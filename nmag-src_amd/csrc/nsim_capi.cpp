@@ -69,7 +69,10 @@ struct nsim_handle {
   CgWork cgw;
   unsigned* sync_nbc = nullptr;
   unsigned* sync_dbc = nullptr;
-  Coeffs coef{};
+  Coeffs coef{};                 // the first magnet's coefficients
+  std::vector<int> node_off;     // magnet k owns nodes [node_off[k], node_off[k+1])
+  std::vector<int> bnd_off;      // and boundary nodes [bnd_off[k], bnd_off[k+1])
+  long long bem_elems = 0;       // stored BEM entries over all segments
   nsim_options opt{};
   bool mixed = false;
   int device = 0, num_cus = 0;
@@ -141,7 +144,7 @@ int setup_cg(nsim_handle* h, int rpt) {
   std::string err = build_cg_operator(h->host, 256 * rpt, op);
   if (!err.empty()) return fail(err);
   const int maxw = cg_slot_variant(op.max_width, rpt);
-  if (op.parts > 1024 || op.parts > cg_max_workgroups(h->mixed, rpt, maxw, h->device))
+  if (op.parts > 1024 || op.parts > cg_max_workgroups(rpt, maxw, h->device))
     return fail("CG parts do not fit one co-resident grid at " + std::to_string(rpt) +
                 " rows per thread");
   for (void* p : h->cg_allocs) (void)hipFree(p);
@@ -164,10 +167,6 @@ int setup_cg(nsim_handle* h, int rpt) {
             up(&c.p2o, op.p2o) && up(&c.dinv_nbc, op.dinv_nbc) &&
             up(&c.dinv_dbc, op.dinv_dbc) && up(&c.ainv_nbc, op.ainv_nbc) &&
             up(&c.ainv_dbc, op.ainv_dbc);
-  if (ok && h->mixed) {
-    std::vector<float> kf(op.sK.begin(), op.sK.end());
-    ok = up(&c.Kf, kf);
-  }
   if (!ok) return fail("CG operator upload failed");
   h->cg = c;
   return 0;
@@ -193,14 +192,14 @@ int device_rhs(nsim_handle* h, const double* m, double* dmdt) {
   const bool demag = h->opt.do_demag != 0 && h->d.nb > 0;
   {
     Span s(h, "exch_div");
-    HIPCHK(launch_exch_div(h->d, h->mixed, h->coef, m, h->Hexch, h->rho, demag, h->stream));
+    HIPCHK(launch_exch_div(h->d, h->mixed, m, h->Hexch, h->rho, demag, h->stream));
   }
   if (demag) {
     CgWork& w = h->cgw;
     {
       Span s(h, "cg_nbc");
       w.sync = h->sync_nbc;
-      HIPCHK(launch_cg(h->cg, h->mixed, w, false, h->cg_coarse, h->rho, h->phi1,
+      HIPCHK(launch_cg(h->cg, w, false, h->cg_coarse, h->rho, h->phi1,
                        h->opt.nbc_rtol, h->opt.nbc_maxits, nullptr, nullptr, h->phi1b,
                        h->d.blocal, h->stream));
     }
@@ -211,7 +210,7 @@ int device_rhs(nsim_handle* h, const double* m, double* dmdt) {
     {
       Span s(h, "cg_dbc");
       w.sync = h->sync_dbc;
-      HIPCHK(launch_cg(h->cg, h->mixed, w, true, h->cg_coarse, nullptr, h->phi2,
+      HIPCHK(launch_cg(h->cg, w, true, h->cg_coarse, nullptr, h->phi2,
                        h->opt.dbc_rtol, h->opt.dbc_maxits, h->phi1, h->phi, nullptr,
                        h->d.blocal, h->stream));
     }
@@ -220,8 +219,7 @@ int device_rhs(nsim_handle* h, const double* m, double* dmdt) {
     Span s(h, "demag_llg");
     FieldBufs b{h->Hexch, h->Hdemag, h->Hanis, h->Htotal, h->rho,
                 h->phi1,  h->phi2,   h->phi,   h->phi1b};
-    HIPCHK(launch_demag_llg(h->d, h->mixed, h->coef, field_args(h, m), b, dmdt, demag,
-                            h->stream));
+    HIPCHK(launch_demag_llg(h->d, h->mixed, field_args(h, m), b, dmdt, demag, h->stream));
   }
   h->rhs_evals++;
   return 0;
@@ -229,7 +227,7 @@ int device_rhs(nsim_handle* h, const double* m, double* dmdt) {
 
 int device_jtimes(nsim_handle* h, const double* m, const double* v, double* Jv) {
   Span s(h, "jtimes");
-  HIPCHK(launch_jtimes(h->d, h->mixed, h->coef, m, h->Htotal, v, Jv, h->stream));
+  HIPCHK(launch_jtimes(h->d, h->mixed, m, h->Htotal, v, Jv, h->stream));
   return 0;
 }
 
@@ -238,7 +236,8 @@ int handle_n(nsim_handle* h) { return h->d.n; }
 double* handle_m(nsim_handle* h) { return h->m; }
 const double* handle_Htotal(nsim_handle* h) { return h->Htotal; }
 const double* handle_kappa(nsim_handle* h) { return h->kappa; }
-const Coeffs& handle_coeffs(nsim_handle* h) { return h->coef; }
+const Coeffs* handle_coef_table(nsim_handle* h) { return h->d.coefs; }
+const int* handle_inst(nsim_handle* h) { return h->d.inst; }
 int handle_error_flags(nsim_handle* h, unsigned* out) {
   unsigned a[6], b[6];
   HIPCHK(hipMemcpyAsync(a, h->sync_nbc, sizeof(a), hipMemcpyDeviceToHost, h->stream));
@@ -251,22 +250,62 @@ void set_error(const std::string& msg) { g_last_error = msg; }
 
 }  // namespace nsim
 
-extern "C" {
+namespace {
 
-const char* nsim_last_error(void) { return g_last_error.c_str(); }
-
-int nsim_create(const nsim_mesh* mesh, const nsim_material* mat, const nsim_options* opt,
-                nsim_handle** out) {
-  if (!mesh || !mat || !out) return fail("nsim_create: null argument");
+// Builds a handle for `ninst` independent magnets (one for nsim_create).  The
+// magnets' meshes are concatenated, node indices offset, into one operator
+// set: the sparse operators are block-diagonal by construction and the BEM
+// matrix is stored as one dense segment per magnet.
+int create_impl(const char* who, int ninst, const nsim_mesh* meshes,
+                const nsim_material* mats, const nsim_options* opt, nsim_handle** out) {
+  const std::string me(who);
+  if (!meshes || !mats || !out) return fail(me + ": null argument");
   *out = nullptr;
+  if (ninst < 1) return fail(me + ": need at least one magnet");
   nsim_handle* h = new nsim_handle();
   auto cleanup = [&](int code) {
     nsim_destroy(h);
     return code;
   };
-  std::string err =
-      build_host_operators(mesh->n_nodes, mesh->coords, mesh->n_tets, mesh->tets, h->host);
-  if (!err.empty()) return cleanup(fail("nsim_create: " + err));
+  std::vector<int> node_off(ninst + 1, 0);
+  std::vector<double> inst_eps(ninst, 0.0);
+  for (int k = 0; k < ninst; k++) {
+    const nsim_mesh& mk = meshes[k];
+    if (mk.n_nodes <= 0 || mk.n_tets <= 0 || !mk.coords || !mk.tets)
+      return cleanup(fail(me + ": magnet " + std::to_string(k) + " has an empty mesh"));
+    for (long long e = 0; e < 4LL * mk.n_tets; e++)
+      if (mk.tets[e] < 0 || mk.tets[e] >= mk.n_nodes)
+        return cleanup(fail(me + ": magnet " + std::to_string(k) +
+                            ": tetrahedron references a node out of range"));
+    double gmax = 0.0;
+    for (long long e = 0; e < 3LL * mk.n_nodes; e++) gmax = std::max(gmax, std::fabs(mk.coords[e]));
+    inst_eps[k] = gmax * 1e-13;  // the single-magnet geometric tolerance
+    if ((long long)node_off[k] + mk.n_nodes >= (1LL << 21))
+      return cleanup(fail(me + ": more than 2^21 nodes in one handle"));
+    node_off[k + 1] = node_off[k] + mk.n_nodes;
+  }
+  std::string err;
+  if (ninst == 1) {
+    err = build_host_operators(meshes[0].n_nodes, meshes[0].coords, meshes[0].n_tets,
+                               meshes[0].tets, h->host);
+  } else {
+    std::vector<double> xs;
+    std::vector<int> ts;
+    xs.reserve(3 * (size_t)node_off[ninst]);
+    for (int k = 0; k < ninst; k++) {
+      xs.insert(xs.end(), meshes[k].coords, meshes[k].coords + 3 * (size_t)meshes[k].n_nodes);
+      for (long long e = 0; e < 4LL * meshes[k].n_tets; e++)
+        ts.push_back(meshes[k].tets[e] + node_off[k]);
+    }
+    err = build_host_operators(node_off[ninst], xs.data(), (int)(ts.size() / 4), ts.data(),
+                               h->host);
+    h->host.ninst = ninst;
+    h->host.inst.resize(node_off[ninst]);
+    for (int k = 0; k < ninst; k++)
+      std::fill(h->host.inst.begin() + node_off[k], h->host.inst.begin() + node_off[k + 1], k);
+  }
+  if (!err.empty()) return cleanup(fail(me + ": " + err));
+  h->node_off = node_off;
   if (opt) h->opt = *opt;
   else {
     h->opt.precision = NSIM_FP64;
@@ -275,11 +314,18 @@ int nsim_create(const nsim_mesh* mesh, const nsim_material* mat, const nsim_opti
     h->opt.nbc_maxits = h->opt.dbc_maxits = 10000;
   }
   if (h->opt.precision < NSIM_FP64 || h->opt.precision > NSIM_MIXED_BF16)
-    return cleanup(fail("nsim_create: unknown precision"));
+    return cleanup(fail(me + ": unknown precision"));
   if (h->opt.nbc_maxits <= 0) h->opt.nbc_maxits = 10000;
   if (h->opt.dbc_maxits <= 0) h->opt.dbc_maxits = 10000;
   h->mixed = h->opt.precision != NSIM_FP64;
-  h->coef = Coeffs{mat->Ms, mat->negJ, mat->c1, mat->c2, mat->c3, mat->mu0};
+  std::vector<Coeffs> coefs(ninst);
+  for (int k = 0; k < ninst; k++) {
+    const nsim_material& mk = mats[k];
+    if (!(mk.Ms > 0.0) || !(mk.mu0 > 0.0))
+      return cleanup(fail(me + ": magnet " + std::to_string(k) + " needs Ms > 0 and mu0 > 0"));
+    coefs[k] = Coeffs{mk.Ms, mk.negJ, mk.c1, mk.c2, mk.c3, mk.mu0};
+  }
+  h->coef = coefs[0];
 
   const HostOperators& o = h->host;
   DeviceOps& d = h->d;
@@ -288,16 +334,35 @@ int nsim_create(const nsim_mesh* mesh, const nsim_material* mat, const nsim_opti
   d.ntri = o.ntri;
   d.nslices = o.nslices;
   d.sell_total = o.sell_off[o.nslices];
+  d.ninst = ninst;
   for (int sl = 0; sl < o.nslices; sl++)
     d.max_width = std::max(d.max_width, (o.sell_off[sl + 1] - o.sell_off[sl]) / kSlice);
-  d.ldb = ((o.nb + 63) / 64) * 64;
   d.btype = h->opt.precision == NSIM_FP64 ? kBemF64
             : h->opt.precision == NSIM_MIXED_FP32 ? kBemF32 : kBemBF16;
-  d.geo_eps = o.geo_eps;
+  // BEM segments: the boundary nodes are in ascending node order, so each
+  // magnet's boundary nodes form one contiguous run
+  long long boff_elems = 0;
+  for (int k = 0; k < ninst; k++) {
+    const int b0 = (int)(std::lower_bound(o.bnode.begin(), o.bnode.end(), node_off[k]) -
+                         o.bnode.begin());
+    const int b1 = (int)(std::lower_bound(o.bnode.begin(), o.bnode.end(), node_off[k + 1]) -
+                         o.bnode.begin());
+    const int nbk = b1 - b0, ldk = ((nbk + 63) / 64) * 64;
+    d.h_seg_nb.push_back(nbk);
+    d.h_seg_ldb.push_back(ldk);
+    d.h_seg_boff.push_back(b0);
+    d.h_seg_Boff.push_back(boff_elems);
+    d.h_seg_geo_eps.push_back(ninst == 1 ? o.geo_eps : inst_eps[k]);
+    boff_elems += (long long)nbk * ldk;
+    d.ldb = std::max(d.ldb, ldk);
+    h->bnd_off.push_back(b0);
+  }
+  h->bnd_off.push_back(o.nb);
+  h->bem_elems = boff_elems;
   if (h->opt.do_demag && (size_t)d.ldb * (d.btype == kBemBF16 ? 4 : 8) > 64 * 1024)
-    return cleanup(fail("nsim_create: " + std::to_string(o.nb) +
-                        " boundary nodes exceed the BEM GEMV's LDS staging (8192 at FP64/FP32 "
-                        "storage, 16384 at BF16)"));
+    return cleanup(fail(me + ": " + std::to_string(d.ldb) +
+                        " boundary nodes in one magnet exceed the BEM GEMV's LDS staging "
+                        "(8192 at FP64/FP32 storage, 16384 at BF16)"));
 
   hipError_t e;
 #define UP(dst, vec)                                                        \
@@ -324,10 +389,8 @@ int nsim_create(const nsim_mesh* mesh, const nsim_material* mat, const nsim_opti
   UP(d.D, o.sD);
   UP(d.G, o.sG);
   if (h->mixed) {
-    std::vector<float> kf(o.sK.begin(), o.sK.end()), df(o.sD.begin(), o.sD.end()),
-        gf(o.sG.begin(), o.sG.end());
+    std::vector<float> kf(o.sK.begin(), o.sK.end()), gf(o.sG.begin(), o.sG.end());
     UP(d.Kf, kf);
-    UP(d.Df, df);
     UP(d.Gf, gf);
   }
   std::vector<double> vinv(o.n), dn(o.n), dd(o.n);
@@ -341,8 +404,11 @@ int nsim_create(const nsim_mesh* mesh, const nsim_material* mat, const nsim_opti
   UP(d.dinv_dbc, dd);
   UP(d.blocal, o.blocal);
   std::vector<double> kap(o.n);
-  for (int i = 0; i < o.n; i++) kap[i] = h->coef.negJ * o.Kdiag[i] / o.V[i];
+  for (int i = 0; i < o.n; i++)
+    kap[i] = coefs[o.inst.empty() ? 0 : o.inst[i]].negJ * o.Kdiag[i] / o.V[i];
   UP(h->kappa, kap);
+  UP(d.coefs, coefs);
+  if (ninst > 1) UP(d.inst, o.inst);
 
   // BEM matrix, assembled on the device from the boundary geometry
   if (o.nb > 0) {
@@ -355,8 +421,16 @@ int nsim_create(const nsim_mesh* mesh, const nsim_material* mat, const nsim_opti
     UP(d.omega, o.omega);
     UP(d.inc_rp, o.inc_rp);
     UP(d.inc_tri, o.inc_tri);
+    UP(d.seg_nb, d.h_seg_nb);
+    UP(d.seg_ldb, d.h_seg_ldb);
+    UP(d.seg_boff, d.h_seg_boff);
+    UP(d.seg_Boff, d.h_seg_Boff);
+    std::vector<int> wg_seg, wg_row0;
+    plan_bem_gemv(d, h->num_cus, &wg_seg, &wg_row0);
+    UP(d.wg_seg, wg_seg);
+    UP(d.wg_row0, wg_row0);
     size_t esz = d.btype == kBemF64 ? 8 : d.btype == kBemF32 ? 4 : 2;
-    e = hipMalloc(&d.B, (size_t)o.nb * d.ldb * esz);
+    e = hipMalloc(&d.B, (size_t)h->bem_elems * esz);
     if (e != hipSuccess) return cleanup(fail(std::string("BEM alloc: ") + hipGetErrorString(e)));
     h->track(d.B);
     e = launch_bem_assemble(d, h->stream);
@@ -403,6 +477,29 @@ int nsim_create(const nsim_mesh* mesh, const nsim_material* mat, const nsim_opti
 #undef ZEROS
   if (hipDeviceSynchronize() != hipSuccess) return cleanup(fail("device sync failed"));
   *out = h;
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* nsim_last_error(void) { return g_last_error.c_str(); }
+
+int nsim_create(const nsim_mesh* mesh, const nsim_material* mat, const nsim_options* opt,
+                nsim_handle** out) {
+  return create_impl("nsim_create", 1, mesh, mat, opt, out);
+}
+
+int nsim_create_batch(int n_magnets, const nsim_mesh* meshes, const nsim_material* mats,
+                      const nsim_options* opt, nsim_handle** out) {
+  return create_impl("nsim_create_batch", n_magnets, meshes, mats, opt, out);
+}
+
+int nsim_get_batch_layout(nsim_handle* h, int* node_off, int* boundary_off) {
+  if (!h) return fail("nsim_get_batch_layout: null handle");
+  if (node_off) std::copy(h->node_off.begin(), h->node_off.end(), node_off);
+  if (boundary_off) std::copy(h->bnd_off.begin(), h->bnd_off.end(), boundary_off);
   return 0;
 }
 
@@ -507,25 +604,31 @@ int nsim_get_field(nsim_handle* h, const char* name, double* out) {
 
 int nsim_get_bem(nsim_handle* h, double* out) {
   if (!h || !out) return fail("nsim_get_bem: null argument");
-  const size_t nb = h->d.nb, ld = h->d.ldb;
-  if (nb == 0) return 0;
+  if (h->d.nb == 0) return 0;
   const size_t esz = h->d.btype == kBemF64 ? 8 : h->d.btype == kBemF32 ? 4 : 2;
-  std::vector<unsigned char> buf(nb * ld * esz);
+  std::vector<unsigned char> buf((size_t)h->bem_elems * esz);
+  HIPCHK(hipStreamSynchronize(h->stream));
   HIPCHK(hipMemcpy(buf.data(), h->d.B, buf.size(), hipMemcpyDeviceToHost));
-  for (size_t i = 0; i < nb; i++)
-    for (size_t j = 0; j < nb; j++) {
-      const size_t k = i * ld + j;
-      double v;
-      if (h->d.btype == kBemF64) v = reinterpret_cast<double*>(buf.data())[k];
-      else if (h->d.btype == kBemF32) v = reinterpret_cast<float*>(buf.data())[k];
-      else {
-        uint32_t u = (uint32_t)reinterpret_cast<uint16_t*>(buf.data())[k] << 16;
-        float f;
-        std::memcpy(&f, &u, 4);
-        v = f;
+  // segments one after another, each nb_k x nb_k row-major without padding
+  size_t o = 0;
+  for (size_t s = 0; s < h->d.h_seg_nb.size(); s++) {
+    const size_t nb = h->d.h_seg_nb[s], ld = h->d.h_seg_ldb[s], base = h->d.h_seg_Boff[s];
+    for (size_t i = 0; i < nb; i++)
+      for (size_t j = 0; j < nb; j++) {
+        const size_t k = base + i * ld + j;
+        double v;
+        if (h->d.btype == kBemF64) v = reinterpret_cast<double*>(buf.data())[k];
+        else if (h->d.btype == kBemF32) v = reinterpret_cast<float*>(buf.data())[k];
+        else {
+          uint32_t u = (uint32_t)reinterpret_cast<uint16_t*>(buf.data())[k] << 16;
+          float f;
+          std::memcpy(&f, &u, 4);
+          v = f;
+        }
+        out[o + i * nb + j] = v;
       }
-      out[i * nb + j] = v;
-    }
+    o += nb * nb;
+  }
   return 0;
 }
 
@@ -689,7 +792,8 @@ int nsim_psetup(nsim_handle* h, double t, const double* y, const double* fy, int
     yd = h->io_a;
   }
   // H_total of the most recent rhs (CVODE calls psetup after f(t, y))
-  HIPCHK(bjac_setup(h->stream, (int)n, h->coef, gamma, yd, h->Htotal, h->kappa, h->pc_Minv));
+  HIPCHK(bjac_setup(h->stream, (int)n, h->d.coefs, h->d.inst, gamma, yd, h->Htotal, h->kappa,
+                    h->pc_Minv));
   if (jcur) *jcur = 1;
   if (!device_ptrs) HIPCHK(hipStreamSynchronize(h->stream));
   return 0;

@@ -15,6 +15,13 @@
 // is written with agent-coherent (sc1) stores and read with sc1 buffer loads,
 // so no L2 write-back or invalidation and no contended atomic is on the
 // critical path, and all gathers of a row are in flight at once.
+//
+// The operator is stored in FP64 at every precision setting.  Rounding the
+// singular Neumann stiffness to FP32 moves its null eigenvalue off zero (on a
+// 6x4x2 Kuhn mesh to -1.2e-9 of the largest: the matrix becomes indefinite)
+// and the load picks up a component along it, so tight solves stall; the
+// solve is bound by exchange latency, not operator bytes, so FP32 storage
+// would buy no time either.
 #include <hip/hip_runtime.h>
 
 #include "nsim_dev_common.h"
@@ -406,8 +413,8 @@ int cg_slot_variant(int width, int rpt) {
   return 0;
 }
 
-int cg_max_workgroups(bool mixed, int rpt, int maxw, int device) {
-  const void* k = mixed ? select_kernel<float>(rpt, maxw) : select_kernel<double>(rpt, maxw);
+int cg_max_workgroups(int rpt, int maxw, int device) {
+  const void* k = select_kernel<double>(rpt, maxw);
   if (!k) return 0;
   int per_cu = 0, cus = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, 256, 0) != hipSuccess) return 0;
@@ -416,7 +423,7 @@ int cg_max_workgroups(bool mixed, int rpt, int maxw, int device) {
   return per_cu * cus;
 }
 
-hipError_t launch_cg(const CgDevice& c, bool mixed, CgWork& w, bool dirichlet, bool coarse,
+hipError_t launch_cg(const CgDevice& c, CgWork& w, bool dirichlet, bool coarse,
                      const double* b, double* x, double rtol, int maxit, const double* xadd,
                      double* xsum, double* restrict_out, const int* blocal,
                      hipStream_t st) {
@@ -426,19 +433,11 @@ hipError_t launch_cg(const CgDevice& c, bool mixed, CgWork& w, bool dirichlet, b
       2LL * c.npad * 8 >= 0x7fffffffLL || (256 << 0) * c.rpt != (1 << c.cap_shift))
     return hipErrorInvalidConfiguration;
   const unsigned long long epoch = ++w.epoch;
-  const void* k = mixed ? select_kernel<float>(c.rpt, c.maxw)
-                        : select_kernel<double>(c.rpt, c.maxw);
+  const void* k = select_kernel<double>(c.rpt, c.maxw);
   if (!k) return hipErrorInvalidConfiguration;
   const double* dinv = dirichlet ? c.dinv_dbc : c.dinv_nbc;
   const double* ainv = dirichlet ? c.ainv_dbc : c.ainv_nbc;
   const int naggs = coarse ? c.naggs : 0;
-  if (mixed) {
-    CgArgs<float> a{c.npad, w.gmax, maxit, c.cap_shift, c.agg_shift, naggs, c.sell_off,
-                    c.sell_col, c.Kf, c.p2o, dinv, ainv, b, x, w.pub, w.part, w.tag, w.sync,
-                    epoch, rtol * rtol, xadd, xsum, restrict_out, blocal};
-    void* args[] = {&a};
-    return hipLaunchCooperativeKernel(k, dim3(grid), dim3(256), args, 0, st);
-  }
   CgArgs<double> a{c.npad, w.gmax, maxit, c.cap_shift, c.agg_shift, naggs, c.sell_off,
                    c.sell_col, c.K, c.p2o, dinv, ainv, b, x, w.pub, w.part, w.tag, w.sync,
                    epoch, rtol * rtol, xadd, xsum, restrict_out, blocal};

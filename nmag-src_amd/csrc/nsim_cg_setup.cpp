@@ -70,6 +70,90 @@ bool spd_inverse(std::vector<double>& A, int n) {
   return true;
 }
 
+// In-place pseudo-inverse of a symmetric positive semi-definite matrix from its
+// eigen-decomposition by cyclic Jacobi rotations; eigenvalues at or below
+// rel * max(lambda_max, scale) are taken as zero (the null space gets no
+// correction).  `scale` is the operator's own magnitude: a matrix whose every
+// eigenvalue is roundoff (one aggregate) must come out as zero, not 1/roundoff.
+void sym_pinv(std::vector<double>& A, int n, double rel, double scale) {
+  std::vector<double> V((size_t)n * n, 0.0);
+  for (int i = 0; i < n; i++) V[i * n + i] = 1.0;
+  for (int sweep = 0; sweep < 100; sweep++) {
+    double off = 0.0, dg = 0.0;
+    for (int p = 0; p < n; p++) {
+      dg += A[p * n + p] * A[p * n + p];
+      for (int q = p + 1; q < n; q++) off += A[p * n + q] * A[p * n + q];
+    }
+    if (off <= 1e-32 * dg) break;
+    for (int p = 0; p < n; p++)
+      for (int q = p + 1; q < n; q++) {
+        const double apq = A[p * n + q];
+        if (apq == 0.0) continue;
+        const double th = (A[q * n + q] - A[p * n + p]) / (2.0 * apq);
+        const double t = (th >= 0.0 ? 1.0 : -1.0) / (std::fabs(th) + std::sqrt(th * th + 1.0));
+        const double c = 1.0 / std::sqrt(t * t + 1.0), s = t * c;
+        for (int k = 0; k < n; k++) {  // columns p, q
+          const double akp = A[k * n + p], akq = A[k * n + q];
+          A[k * n + p] = c * akp - s * akq;
+          A[k * n + q] = s * akp + c * akq;
+        }
+        for (int k = 0; k < n; k++) {  // rows p, q
+          const double apk = A[p * n + k], aqk = A[q * n + k];
+          A[p * n + k] = c * apk - s * aqk;
+          A[q * n + k] = s * apk + c * aqk;
+        }
+        for (int k = 0; k < n; k++) {
+          const double vkp = V[k * n + p], vkq = V[k * n + q];
+          V[k * n + p] = c * vkp - s * vkq;
+          V[k * n + q] = s * vkp + c * vkq;
+        }
+      }
+  }
+  double lmax = scale;
+  for (int k = 0; k < n; k++) lmax = std::max(lmax, A[k * n + k]);
+  std::vector<double> inv((size_t)n * n, 0.0);
+  for (int k = 0; k < n; k++) {
+    const double lam = A[k * n + k];
+    if (!(lam > rel * lmax)) continue;
+    for (int i = 0; i < n; i++)
+      for (int j = 0; j < n; j++) inv[i * n + j] += V[i * n + k] * V[j * n + k] / lam;
+  }
+  A.swap(inv);
+}
+
+// Coordinates for the bisection: a batch's magnets are shifted to lie side by
+// side along x with gaps, so that bisection separates magnets before it cuts
+// one; a single magnet keeps its own coordinates.
+std::vector<double> bisection_coords(const HostOperators& o) {
+  if (o.ninst <= 1 || o.inst.empty()) return o.x;
+  const int ni = o.ninst;
+  std::vector<double> lo(3 * ni, 1e300), hi(3 * ni, -1e300);
+  for (int i = 0; i < o.n; i++)
+    for (int d = 0; d < 3; d++) {
+      const int k = o.inst[i];
+      lo[3 * k + d] = std::min(lo[3 * k + d], o.x[3 * i + d]);
+      hi[3 * k + d] = std::max(hi[3 * k + d], o.x[3 * i + d]);
+    }
+  double gap = 0.0;
+  for (int k = 0; k < ni; k++)
+    for (int d = 0; d < 3; d++) gap = std::max(gap, hi[3 * k + d] - lo[3 * k + d]);
+  std::vector<double> shift(ni, 0.0);
+  double at = 0.0;
+  for (int k = 0; k < ni; k++) {
+    if (hi[3 * k] < lo[3 * k]) continue;  // magnet without nodes
+    shift[k] = at - lo[3 * k];
+    at += hi[3 * k] - lo[3 * k] + gap + 1.0;
+  }
+  std::vector<double> x(o.x);
+  for (int i = 0; i < o.n; i++) {
+    const int k = o.inst[i];
+    x[3 * i] += shift[k];
+    x[3 * i + 1] -= lo[3 * k + 1];
+    x[3 * i + 2] -= lo[3 * k + 2];
+  }
+  return x;
+}
+
 }  // namespace
 
 std::string build_cg_operator(const HostOperators& o, int cap, CgOperator& cg) {
@@ -80,7 +164,7 @@ std::string build_cg_operator(const HostOperators& o, int cap, CgOperator& cg) {
   std::vector<int> idx(n);
   std::iota(idx.begin(), idx.end(), 0);
   std::vector<std::pair<int, int>> parts;
-  rcb(o.x, idx, 0, n, cg.parts, parts);
+  rcb(bisection_coords(o), idx, 0, n, cg.parts, parts);
   if ((int)parts.size() != cg.parts) return "partitioning failed";
   cg.p2o.assign(cg.npad, -1);
   std::vector<int> o2p(n, -1), part_of(n, -1);
@@ -154,24 +238,20 @@ std::string build_cg_operator(const HostOperators& o, int cap, CgOperator& cg) {
       if (fi && o.blocal[j] < 0) Ad[a * na + b] += o.K[k];
     }
   }
-  // The Neumann coarse matrix is singular along the constants e.  With
-  // tau > 0, (A + (tau/na) e e^T)^-1 = A^+ + e e^T / (tau na), so subtracting
-  // 1/(tau na) from every entry gives the pseudo-inverse: no coarse
-  // correction along the constants, whatever tau.  tau is kept at the scale of
-  // the operator (never the roundoff-sized trace of a single aggregate).
-  double tr = 0.0, kd = 0.0;
-  for (int a = 0; a < na; a++) tr += An[a * na + a];
+  // The Neumann coarse matrix is singular along every vector that is constant
+  // on each magnet (one such direction per magnet whose aggregates it does not
+  // share); its pseudo-inverse gives no coarse correction there.  Roundoff
+  // leaves those eigenvalues near 1e-16 of the operator scale (the mean of
+  // sum K_ii over an aggregate); the smallest genuine one of 64 aggregates is
+  // above 1e-4 of the largest.
+  double kd = 0.0;
   for (int i = 0; i < n; i++) kd += o.Kdiag[i];
-  const double tau = std::max(tr / na, kd / n);
-  for (int a = 0; a < na; a++)
-    for (int b = 0; b < na; b++) An[a * na + b] += tau / na;
+  sym_pinv(An, na, 1e-9, kd / na);
   for (int a = 0; a < na; a++)
     if (!has_free[a]) {
       for (int b = 0; b < na; b++) Ad[a * na + b] = Ad[b * na + a] = 0.0;
       Ad[a * na + a] = 1.0;
     }
-  if (!spd_inverse(An, na)) return "Neumann coarse matrix is not positive definite";
-  for (double& v : An) v -= 1.0 / (tau * na);
   if (!spd_inverse(Ad, na)) return "Dirichlet coarse matrix is not positive definite";
   for (int a = 0; a < na; a++)
     if (!has_free[a]) Ad[a * na + a] = 0.0;  // no coarse correction there

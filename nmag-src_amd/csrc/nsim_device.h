@@ -4,30 +4,52 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <vector>
 
 namespace nsim {
 
 enum BemType { kBemF64 = 0, kBemF32 = 1, kBemBF16 = 2 };
 
+// Material and LLG coefficients of one magnet (simulation units).
+struct Coeffs {
+  double Ms, negJ, c1, c2, c3, mu0;
+};
+
 // Operators resident in HBM (one copy per precision actually used).
 struct DeviceOps {
-  int n = 0, nb = 0, ntri = 0, ldb = 0, nslices = 0, sell_total = 0;
+  int n = 0, nb = 0, ntri = 0, nslices = 0, sell_total = 0;
+  int ldb = 0;        // widest BEM segment's row stride (multiple of 64)
   int max_width = 0;  // widest SELL slice (slots per row)
   int* sell_off = nullptr;
   int* sell_col = nullptr;
   double* K = nullptr;  // FP64 values (always present)
   double* D = nullptr;
   double* G = nullptr;
-  float* Kf = nullptr;  // FP32 copies (mixed precision only)
-  float* Df = nullptr;
-  float* Gf = nullptr;
+  float* Kf = nullptr;  // FP32 copies (mixed precision only); the divergence
+  float* Gf = nullptr;  // D stays FP64, it feeds the singular Neumann problem
   double* Vinv = nullptr;      // 1/V_i
   double* dinv_nbc = nullptr;  // 1/K_ii
   double* dinv_dbc = nullptr;  // 1/K_ii on interior rows, 0 on boundary rows
   int* bnode = nullptr;        // local boundary index -> node
   int* blocal = nullptr;       // node -> local boundary index or -1
-  void* B = nullptr;           // nb x ldb row-major
+  void* B = nullptr;           // block-diagonal dense segments, each nb_s x ldb_s row-major
   int btype = kBemF64;
+  // BEM segments (one per independent magnet): host copies and device tables
+  std::vector<int> h_seg_nb, h_seg_ldb, h_seg_boff;
+  std::vector<long long> h_seg_Boff;
+  std::vector<double> h_seg_geo_eps;
+  int* seg_nb = nullptr;
+  int* seg_ldb = nullptr;
+  int* seg_boff = nullptr;
+  long long* seg_Boff = nullptr;
+  int* wg_seg = nullptr;   // GEMV workgroup -> segment
+  int* wg_row0 = nullptr;  // GEMV workgroup -> first row in the segment
+  int gemv_grid = 0, gemv_rpw = 0, max_ldb = 0;
+  // coefficients: one entry per magnet; inst[i] = magnet of node i (null
+  // when the handle holds a single magnet)
+  Coeffs* coefs = nullptr;
+  int* inst = nullptr;
+  int ninst = 1;
   // BEM geometry (set-up only)
   double* coords = nullptr;
   int* tri_node = nullptr;     // 3*ntri global node indices
@@ -35,7 +57,6 @@ struct DeviceOps {
   double* omega = nullptr;
   int* inc_rp = nullptr;
   int* inc_tri = nullptr;
-  double geo_eps = 0.0;
 };
 
 // Communication buffers of the persistent CG kernel.
@@ -48,10 +69,6 @@ struct CgWork {
                                     // [6] solves stopped at maxit
   int gmax = 0;
   unsigned long long epoch = 0;     // launch counter; makes every phase tag unique
-};
-
-struct Coeffs {
-  double Ms, negJ, c1, c2, c3, mu0;
 };
 
 // Field inputs of one evaluation (device pointers; optional ones may be null).
@@ -70,8 +87,8 @@ struct FieldBufs {
 
 // Launchers (all asynchronous on `st`).  Return hipSuccess or the launch error.
 hipError_t launch_bem_assemble(const DeviceOps& d, hipStream_t st);
-hipError_t launch_exch_div(const DeviceOps& d, bool mixed, const Coeffs& c, const double* m,
-                           double* Hexch, double* rho, bool want_rho, hipStream_t st);
+hipError_t launch_exch_div(const DeviceOps& d, bool mixed, const double* m, double* Hexch,
+                           double* rho, bool want_rho, hipStream_t st);
 // The CG's reordered stiffness operator in HBM (see CgOperator).
 struct CgDevice {
   int parts = 0, npad = 0, rpt = 1, cap_shift = 8, agg_shift = 0, naggs = 0;
@@ -79,29 +96,31 @@ struct CgDevice {
   int* sell_off = nullptr;
   int* sell_col = nullptr;
   double* K = nullptr;
-  float* Kf = nullptr;
   int* p2o = nullptr;
   double *dinv_nbc = nullptr, *dinv_dbc = nullptr;
   double *ainv_nbc = nullptr, *ainv_dbc = nullptr;
 };
 // Persistent pipelined two-level PCG on the stiffness matrix; see the kernel.
 // b, x, xadd, xsum are in the original node order.
-hipError_t launch_cg(const CgDevice& c, bool mixed, CgWork& w, bool dirichlet, bool coarse,
+hipError_t launch_cg(const CgDevice& c, CgWork& w, bool dirichlet, bool coarse,
                      const double* b, double* x, double rtol, int maxit, const double* xadd,
                      double* xsum, double* restrict_out, const int* blocal,
                      hipStream_t st);
 hipError_t launch_bem_gemv(const DeviceOps& d, const double* phi1b, double* phi2,
                            hipStream_t st, int num_cus);
-hipError_t launch_demag_llg(const DeviceOps& d, bool mixed, const Coeffs& c,
-                            const FieldArgs& f, const FieldBufs& b, double* dmdt,
-                            bool do_demag, hipStream_t st);
-hipError_t launch_jtimes(const DeviceOps& d, bool mixed, const Coeffs& c, const double* m,
-                         const double* Htotal, const double* v, double* Jv, hipStream_t st);
+// Host plan of the grouped GEMV: fills d.gemv_grid/gemv_rpw/max_ldb and the
+// workgroup -> (segment, first row) table to upload.
+void plan_bem_gemv(DeviceOps& d, int num_cus, std::vector<int>* wg_seg,
+                   std::vector<int>* wg_row0);
+hipError_t launch_demag_llg(const DeviceOps& d, bool mixed, const FieldArgs& f,
+                            const FieldBufs& b, double* dmdt, bool do_demag, hipStream_t st);
+hipError_t launch_jtimes(const DeviceOps& d, bool mixed, const double* m, const double* Htotal,
+                         const double* v, double* Jv, hipStream_t st);
 
 // Register-slot width of the CG kernel for an operator whose widest slice has
 // `width` slots at `rpt` rows per thread (0: slots streamed from memory).
 int cg_slot_variant(int width, int rpt);
 // Cooperative-launch capacity of the CG kernel (workgroups of 256 threads).
-int cg_max_workgroups(bool mixed, int rpt, int maxw, int device);
+int cg_max_workgroups(int rpt, int maxw, int device);
 
 }  // namespace nsim

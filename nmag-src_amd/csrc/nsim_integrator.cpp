@@ -283,8 +283,8 @@ int Integrator::step_attempt(double h, int q, bool* accepted, double* err_out) {
     // res = -G = a + gamma f - y
     HTRY(vec_lincomb(s_, n3_, 1.0, avec_, gamma_, f_, -1.0, ycur_, 0, nullptr, res_));
     if (opt_.precond)
-      HTRY(bjac_setup(s_, nn_, handle_coeffs(h_), gamma_, ycur_, handle_Htotal(h_),
-                      handle_kappa(h_), Minv_));
+      HTRY(bjac_setup(s_, nn_, handle_coef_table(h_), handle_inst(h_), gamma_, ycur_,
+                      handle_Htotal(h_), handle_kappa(h_), Minv_));
     int its = 0;
     TRY(gmres(res_, del_, &its));
     HTRY(vec_lincomb(s_, n3_, 1.0, ycur_, 1.0, del_, 0, nullptr, 0, nullptr, ycur_));

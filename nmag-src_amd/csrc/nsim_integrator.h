@@ -23,7 +23,8 @@ double* handle_m(nsim_handle* h);
 const double* handle_Htotal(nsim_handle* h);
 const double* handle_kappa(nsim_handle* h);
 struct Coeffs;
-const Coeffs& handle_coeffs(nsim_handle* h);
+const Coeffs* handle_coef_table(nsim_handle* h);  // device, one entry per magnet
+const int* handle_inst(nsim_handle* h);           // device node -> magnet, or null
 int handle_error_flags(nsim_handle* h, unsigned* out);
 void set_error(const std::string& msg);
 

@@ -38,6 +38,9 @@ struct HostOperators {
   // triangles incident to each boundary node: (triangle, vertex slot)
   std::vector<int> inc_rp, inc_tri;
   double geo_eps = 0.0;
+  // batch of independent magnets: node -> magnet (empty for one magnet)
+  int ninst = 1;
+  std::vector<int> inst;
 };
 
 // Builds every host-side operator; returns "" on success, else a message.
@@ -49,8 +52,10 @@ std::string build_host_operators(int n, const double* coords, int T, const int* 
 // most `cap` nodes; part g occupies rows [g*cap, g*cap + size_g) and the rest
 // of its rows are inactive padding.  Consecutive parts are merged in groups of
 // 2^agg_shift into at most 64 coarse aggregates; ainv_* are the inverses of the
-// Galerkin coarse matrices P^T K P (Neumann: regularised on the constants;
-// Dirichlet: free rows and columns only).
+// Galerkin coarse matrices P^T K P (Neumann: the pseudo-inverse, whose null
+// space holds the constants of every magnet; Dirichlet: free rows and columns
+// only).  The magnets of a batch are laid side by side for the bisection, so
+// parts rarely straddle two magnets.
 struct CgOperator {
   int cap = 256, parts = 0, npad = 0, nslices = 0, max_width = 0;
   int agg_shift = 0, naggs = 0;

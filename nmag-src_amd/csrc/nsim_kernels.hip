@@ -9,7 +9,8 @@
 //   5. k_demag_llg  H_demag = G phi, H_total, LLG dm/dt         (this file)
 // Operators use a SELL format with slice height 64 (one wavefront): the rows of
 // a wavefront share one loop bound and every slot load is coalesced.  Values
-// are stored in FP64 or FP32 (mixed precision); accumulation is always FP64.
+// are stored in FP64, or in FP32 at mixed precision for the exchange stiffness
+// and the gradient (the divergence stays FP64); accumulation is always FP64.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -23,15 +24,30 @@ namespace {
 
 using namespace dev;
 
+// Coefficients of the magnet that owns node `row`: the table has one entry per
+// magnet of a batch; `inst` is null for a single magnet.
+__device__ inline const Coeffs& coef_of(const Coeffs* __restrict__ ct,
+                                        const int* __restrict__ inst, int row) {
+  return ct[inst ? inst[row] : 0];
+}
+
 // H_exch = negJ/V_i sum_j K_ij m_j (set_H_exch, nmag_lam.py:734-739) and
 // rho_i = Ms sum_j D_ij . m_j (op_div_m then SCALE -1, nmag_lam.py:742-743).
+// The magnets of a batch share no node, so row and columns share one Ms.
+// The stiffness rows sum to zero (K 1 = 0 for P1 elements), so the exchange
+// sum is taken as sum_j K_ij (m_j - m_i): identical in exact arithmetic, and
+// with FP32-stored K a uniform m still gives exactly zero field (the rounded
+// rows no longer sum to zero, the differences do not care).  D is always FP64:
+// rho is the load of the singular Neumann problem, whose solvability
+// (sum_i rho_i = 0) must hold to FP64 roundoff.
 template <typename TV>
 __global__ __launch_bounds__(256) void k_exch_div(int n, const int* __restrict__ off,
                                                   const int* __restrict__ col,
                                                   const TV* __restrict__ K,
-                                                  const TV* __restrict__ D,
+                                                  const double* __restrict__ D,
                                                   const double* __restrict__ Vinv,
-                                                  double negJ, double Ms,
+                                                  const Coeffs* __restrict__ ct,
+                                                  const int* __restrict__ inst,
                                                   const double* __restrict__ m,
                                                   double* __restrict__ Hexch,
                                                   double* __restrict__ rho, int want_rho) {
@@ -39,43 +55,52 @@ __global__ __launch_bounds__(256) void k_exch_div(int n, const int* __restrict__
   if (row >= n) return;
   const int s = row >> 6, lane = row & 63;
   const int b = off[s], w = (off[s + 1] - b) >> 6;
+  const double mix = m[3 * row], miy = m[3 * row + 1], miz = m[3 * row + 2];
   double ex = 0.0, ey = 0.0, ez = 0.0, r = 0.0;
   for (int k = 0; k < w; k++) {
     const int e = b + (k << 6) + lane;
     const int j = col[e];
     const double kv = (double)K[e];
     const double mx = m[3 * j], my = m[3 * j + 1], mz = m[3 * j + 2];
-    ex += kv * mx;
-    ey += kv * my;
-    ez += kv * mz;
-    if (want_rho)
-      r += (double)D[3 * e] * mx + (double)D[3 * e + 1] * my + (double)D[3 * e + 2] * mz;
+    ex += kv * (mx - mix);
+    ey += kv * (my - miy);
+    ez += kv * (mz - miz);
+    if (want_rho) r += D[3 * e] * mx + D[3 * e + 1] * my + D[3 * e + 2] * mz;
   }
-  const double f = negJ * Vinv[row];
+  const Coeffs& c = coef_of(ct, inst, row);
+  const double f = c.negJ * Vinv[row];
   Hexch[3 * row] = f * ex;
   Hexch[3 * row + 1] = f * ey;
   Hexch[3 * row + 2] = f * ez;
-  if (want_rho) rho[row] = Ms * r;
+  if (want_rho) rho[row] = c.Ms * r;
 }
 
-// H_demag_i = sum_j G_ij phi_j (op_grad_phi + CFBOX, nmag_lam.py:755-759);
+// H_demag_i = sum_j G_ij phi_j (op_grad_phi + CFBOX, nmag_lam.py:755-759),
+// taken as sum_j G_ij (phi_j - phi_i): the rows of G sum to zero (the
+// barycentric gradients of a tetrahedron sum to zero), so this is the same
+// field, and with FP32-stored G it stays blind to the arbitrary constant of
+// the Neumann potential;
 // H_anis from the uniaxial tangential gradient (src/nsim_anisotropy.ml:167-234);
 // H_total (nmag_lam.py:377-411); dm/dt (nmag_lam.py:417-430) times pin.
 template <typename TV>
 __global__ __launch_bounds__(256) void k_demag_llg(int n, const int* __restrict__ off,
                                                    const int* __restrict__ col,
-                                                   const TV* __restrict__ G, Coeffs c,
-                                                   FieldArgs f, FieldBufs bf,
-                                                   double* __restrict__ dmdt, int do_demag) {
+                                                   const TV* __restrict__ G,
+                                                   const Coeffs* __restrict__ ct,
+                                                   const int* __restrict__ inst, FieldArgs f,
+                                                   FieldBufs bf, double* __restrict__ dmdt,
+                                                   int do_demag) {
   const int row = blockIdx.x * 256 + threadIdx.x;
   if (row >= n) return;
+  const Coeffs& c = coef_of(ct, inst, row);
   double hd[3] = {0.0, 0.0, 0.0};
   if (do_demag) {
     const int s = row >> 6, lane = row & 63;
     const int b = off[s], w = (off[s + 1] - b) >> 6;
+    const double phi_i = bf.phi[row];
     for (int k = 0; k < w; k++) {
       const int e = b + (k << 6) + lane;
-      const double ph = bf.phi[col[e]];
+      const double ph = bf.phi[col[e]] - phi_i;
       hd[0] += (double)G[3 * e] * ph;
       hd[1] += (double)G[3 * e + 1] * ph;
       hd[2] += (double)G[3 * e + 2] * ph;
@@ -118,13 +143,16 @@ template <typename TV>
 __global__ __launch_bounds__(256) void k_jtimes(int n, const int* __restrict__ off,
                                                 const int* __restrict__ col,
                                                 const TV* __restrict__ K,
-                                                const double* __restrict__ Vinv, Coeffs c,
+                                                const double* __restrict__ Vinv,
+                                                const Coeffs* __restrict__ ct,
+                                                const int* __restrict__ inst,
                                                 const double* __restrict__ m,
                                                 const double* __restrict__ Ht,
                                                 const double* __restrict__ v,
                                                 double* __restrict__ Jv) {
   const int row = blockIdx.x * 256 + threadIdx.x;
   if (row >= n) return;
+  const Coeffs& c = coef_of(ct, inst, row);
   const int s = row >> 6, lane = row & 63;
   const int b = off[s], w = (off[s + 1] - b) >> 6;
   double hx = 0.0, hy = 0.0, hz = 0.0;
@@ -156,39 +184,42 @@ inline int blocks_for(int n) { return (n + 255) / 256; }
 
 }  // namespace
 
-hipError_t launch_exch_div(const DeviceOps& d, bool mixed, const Coeffs& c, const double* m,
-                           double* Hexch, double* rho, bool want_rho, hipStream_t st) {
+hipError_t launch_exch_div(const DeviceOps& d, bool mixed, const double* m, double* Hexch,
+                           double* rho, bool want_rho, hipStream_t st) {
   if (mixed)
     k_exch_div<float><<<blocks_for(d.n), 256, 0, st>>>(d.n, d.sell_off, d.sell_col, d.Kf,
-                                                         d.Df, d.Vinv, c.negJ, c.Ms, m, Hexch,
-                                                         rho, want_rho ? 1 : 0);
+                                                         d.D, d.Vinv, d.coefs, d.inst, m,
+                                                         Hexch, rho, want_rho ? 1 : 0);
   else
     k_exch_div<double><<<blocks_for(d.n), 256, 0, st>>>(d.n, d.sell_off, d.sell_col, d.K,
-                                                          d.D, d.Vinv, c.negJ, c.Ms, m, Hexch,
-                                                          rho, want_rho ? 1 : 0);
+                                                          d.D, d.Vinv, d.coefs, d.inst, m,
+                                                          Hexch, rho, want_rho ? 1 : 0);
   return hipGetLastError();
 }
 
-hipError_t launch_demag_llg(const DeviceOps& d, bool mixed, const Coeffs& c,
-                            const FieldArgs& f, const FieldBufs& b, double* dmdt,
-                            bool do_demag, hipStream_t st) {
+hipError_t launch_demag_llg(const DeviceOps& d, bool mixed, const FieldArgs& f,
+                            const FieldBufs& b, double* dmdt, bool do_demag, hipStream_t st) {
   if (mixed)
-    k_demag_llg<float><<<blocks_for(d.n), 256, 0, st>>>(d.n, d.sell_off, d.sell_col, d.Gf, c,
-                                                          f, b, dmdt, do_demag ? 1 : 0);
+    k_demag_llg<float><<<blocks_for(d.n), 256, 0, st>>>(d.n, d.sell_off, d.sell_col, d.Gf,
+                                                          d.coefs, d.inst, f, b, dmdt,
+                                                          do_demag ? 1 : 0);
   else
-    k_demag_llg<double><<<blocks_for(d.n), 256, 0, st>>>(d.n, d.sell_off, d.sell_col, d.G, c,
-                                                           f, b, dmdt, do_demag ? 1 : 0);
+    k_demag_llg<double><<<blocks_for(d.n), 256, 0, st>>>(d.n, d.sell_off, d.sell_col, d.G,
+                                                           d.coefs, d.inst, f, b, dmdt,
+                                                           do_demag ? 1 : 0);
   return hipGetLastError();
 }
 
-hipError_t launch_jtimes(const DeviceOps& d, bool mixed, const Coeffs& c, const double* m,
-                         const double* Htotal, const double* v, double* Jv, hipStream_t st) {
+hipError_t launch_jtimes(const DeviceOps& d, bool mixed, const double* m, const double* Htotal,
+                         const double* v, double* Jv, hipStream_t st) {
   if (mixed)
     k_jtimes<float><<<blocks_for(d.n), 256, 0, st>>>(d.n, d.sell_off, d.sell_col, d.Kf,
-                                                       d.Vinv, c, m, Htotal, v, Jv);
+                                                       d.Vinv, d.coefs, d.inst, m, Htotal, v,
+                                                       Jv);
   else
     k_jtimes<double><<<blocks_for(d.n), 256, 0, st>>>(d.n, d.sell_off, d.sell_col, d.K,
-                                                        d.Vinv, c, m, Htotal, v, Jv);
+                                                        d.Vinv, d.coefs, d.inst, m, Htotal, v,
+                                                        Jv);
   return hipGetLastError();
 }
 

@@ -26,8 +26,9 @@ hipError_t vec_axpy_multi(hipStream_t st, int n, int k, const double* V, size_t 
                           const double* c_dev, double* y);
 hipError_t vec_maxdist_partials(hipStream_t st, int nn, const double* a, const double* b,
                                 double* part, int* nparts);
-hipError_t bjac_setup(hipStream_t st, int nn, const Coeffs& c, double gamma, const double* m,
-                      const double* H, const double* kappa, double* Minv);
+// ct: per-magnet coefficient table, inst: node -> magnet (null: one magnet)
+hipError_t bjac_setup(hipStream_t st, int nn, const Coeffs* ct, const int* inst, double gamma,
+                      const double* m, const double* H, const double* kappa, double* Minv);
 hipError_t bjac_solve(hipStream_t st, int nn, const double* Minv, const double* r, double* z);
 
 }  // namespace nsim

@@ -111,11 +111,13 @@ __global__ __launch_bounds__(256) void k_maxdist(int nn, const double* a, const 
 // Per node: M_i = I - gamma J_ii with J_ii the 3x3 diagonal block of the
 // jacobi_ts1 linearisation (H frozen, dH_i/dm_i = kappa_i I, kappa_i =
 // negJ K_ii / V_i); stores M_i^-1 (row-major, 9 doubles).
-__global__ __launch_bounds__(256) void k_bjac_setup(int nn, Coeffs c, double gamma,
+__global__ __launch_bounds__(256) void k_bjac_setup(int nn, const Coeffs* __restrict__ ct,
+                                                    const int* __restrict__ inst, double gamma,
                                                     const double* m, const double* H,
                                                     const double* kappa, double* Minv) {
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= nn) return;
+  const Coeffs& c = ct[inst ? inst[i] : 0];
   const double mi[3] = {m[3 * i], m[3 * i + 1], m[3 * i + 2]};
   const double Hi[3] = {H[3 * i], H[3 * i + 1], H[3 * i + 2]};
   const double kp = kappa[i];
@@ -218,9 +220,9 @@ hipError_t vec_maxdist_partials(hipStream_t st, int nn, const double* a, const d
   return hipGetLastError();
 }
 
-hipError_t bjac_setup(hipStream_t st, int nn, const Coeffs& c, double gamma, const double* m,
-                      const double* H, const double* kappa, double* Minv) {
-  k_bjac_setup<<<(nn + 255) / 256, 256, 0, st>>>(nn, c, gamma, m, H, kappa, Minv);
+hipError_t bjac_setup(hipStream_t st, int nn, const Coeffs* ct, const int* inst, double gamma,
+                      const double* m, const double* H, const double* kappa, double* Minv) {
+  k_bjac_setup<<<(nn + 255) / 256, 256, 0, st>>>(nn, ct, inst, gamma, m, H, kappa, Minv);
   return hipGetLastError();
 }
 

@@ -61,6 +61,10 @@ SIGNATURES = {
                                     ctypes.c_int, ctypes.c_int]),
     "nsim_advance": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_int, ctypes.c_int,
                                     ctypes.POINTER(ctypes.c_double)]),
+    "nsim_create_batch": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(NsimMesh),
+                                         ctypes.POINTER(NsimMaterial),
+                                         ctypes.POINTER(NsimOptions), ctypes.POINTER(_vp)]),
+    "nsim_get_batch_layout": (ctypes.c_int, [_vp, _ip, _ip]),
 }
 
 _lib = None
@@ -105,24 +109,48 @@ class NativeMagnet:
 
     def __init__(self, coords, tets, material, precision="fp64", do_demag=True,
                  nbc_rtol=1e-5, dbc_rtol=1e-5, nbc_maxits=10000, dbc_maxits=10000):
+        self._create([(coords, tets)], [material], precision, do_demag, nbc_rtol, dbc_rtol,
+                     nbc_maxits, dbc_maxits)
+
+    def _create(self, meshes, materials, precision, do_demag, nbc_rtol, dbc_rtol, nbc_maxits,
+                dbc_maxits):
         L = lib()
-        self._coords = np.ascontiguousarray(coords, dtype=np.float64).reshape(-1, 3)
-        self._tets = np.ascontiguousarray(tets, dtype=np.int32).reshape(-1, 4)
-        mesh = NsimMesh(len(self._coords), len(self._tets), _dptr(self._coords),
-                        self._tets.ctypes.data_as(_ip))
-        mat = NsimMaterial(*[float(material[k]) for k in ("Ms", "negJ", "c1", "c2", "c3", "mu0")])
+        self._h = None
+        if len(meshes) != len(materials) or not meshes:
+            raise ValueError("need one material per mesh and at least one mesh")
+        self._keep = []
+        cmeshes = (NsimMesh * len(meshes))()
+        cmats = (NsimMaterial * len(meshes))()
+        for k, ((coords, tets), material) in enumerate(zip(meshes, materials)):
+            c = np.ascontiguousarray(coords, dtype=np.float64).reshape(-1, 3)
+            t = np.ascontiguousarray(tets, dtype=np.int32).reshape(-1, 4)
+            self._keep += [c, t]
+            cmeshes[k] = NsimMesh(len(c), len(t), _dptr(c), t.ctypes.data_as(_ip))
+            cmats[k] = NsimMaterial(*[float(material[key])
+                                      for key in ("Ms", "negJ", "c1", "c2", "c3", "mu0")])
+        self._coords = (self._keep[0] if len(meshes) == 1
+                        else np.concatenate(self._keep[0::2]))
         prec = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
         opt = NsimOptions(prec, 1 if do_demag else 0, nbc_rtol, dbc_rtol, nbc_maxits, dbc_maxits)
         h = _vp()
-        self._h = None
-        _check(L.nsim_create(ctypes.byref(mesh), ctypes.byref(mat), ctypes.byref(opt),
-                             ctypes.byref(h)), "nsim_create")
+        if len(meshes) == 1:
+            _check(L.nsim_create(cmeshes, cmats, ctypes.byref(opt), ctypes.byref(h)),
+                   "nsim_create")
+        else:
+            _check(L.nsim_create_batch(len(meshes), cmeshes, cmats, ctypes.byref(opt),
+                                       ctypes.byref(h)), "nsim_create_batch")
         self._h = h
         s = np.zeros(6, dtype=np.int32)
         _check(L.nsim_get_sizes(h, s.ctypes.data_as(_ip)), "nsim_get_sizes")
         self.n, self.nb, self.ntri, self.ntet, self.slots = [int(v) for v in s[:5]]
         self.nnz = int(s[5])
         self.precision = precision
+        self.n_magnets = len(meshes)
+        self.node_off = np.zeros(self.n_magnets + 1, dtype=np.int32)
+        self.boundary_off = np.zeros(self.n_magnets + 1, dtype=np.int32)
+        _check(L.nsim_get_batch_layout(h, self.node_off.ctypes.data_as(_ip),
+                                       self.boundary_off.ctypes.data_as(_ip)),
+               "nsim_get_batch_layout")
 
     def close(self):
         if self._h is not None and _lib is not None:
@@ -174,9 +202,15 @@ class NativeMagnet:
         return jv
 
     def bem(self):
-        B = np.zeros((self.nb, self.nb))
-        _check(lib().nsim_get_bem(self._h, _dptr(B)), "nsim_get_bem")
-        return B
+        """Dense BEM matrix; for a batch, the list of the magnets' matrices."""
+        sizes = np.diff(self.boundary_off)
+        flat = np.zeros(int(np.sum(sizes.astype(np.int64) ** 2)))
+        _check(lib().nsim_get_bem(self._h, _dptr(flat)), "nsim_get_bem")
+        mats, o = [], 0
+        for nb in sizes:
+            mats.append(flat[o:o + nb * nb].reshape(nb, nb))
+            o += nb * nb
+        return mats[0] if self.n_magnets == 1 else mats
 
     def stat(self, name):
         v = ctypes.c_double(0)
@@ -221,3 +255,24 @@ class NativeMagnet:
                                 ctypes.byref(tr))
         _check(rc, "nsim_advance")
         return tr.value
+
+
+class NativeBatch(NativeMagnet):
+    """Many independent magnets in one handle (nsim_create_batch).
+
+    Field vectors are the concatenation of the magnets' vectors in the order
+    given; `node_off` / `boundary_off` are the instance-offset table, and
+    `split` cuts a concatenated array into per-magnet pieces.
+    """
+
+    def __init__(self, meshes, materials, precision="fp64", do_demag=True, nbc_rtol=1e-5,
+                 dbc_rtol=1e-5, nbc_maxits=10000, dbc_maxits=10000):
+        self._create(list(meshes), list(materials), precision, do_demag, nbc_rtol, dbc_rtol,
+                     nbc_maxits, dbc_maxits)
+
+    def split(self, a):
+        a = np.asarray(a)
+        return [a[self.node_off[k]:self.node_off[k + 1]] for k in range(self.n_magnets)]
+
+    def magnet_of_node(self):
+        return np.repeat(np.arange(self.n_magnets), np.diff(self.node_off))

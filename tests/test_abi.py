@@ -36,3 +36,29 @@ def test_missing_library_fails_loudly(monkeypatch):
         assert "no fallback" in str(e)
     else:
         raise AssertionError("expected ImportError")
+
+
+def test_batch_create_rejects_invalid_meshes_before_touching_the_device():
+    """Input validation of nsim_create_batch runs on the host (no GPU needed)."""
+    import numpy as np
+    from nmag import kuhn_cuboid
+    L = _native.lib()
+    c0, t0 = kuhn_cuboid(2, 1, 1, 2.5)
+    c1, t1 = kuhn_cuboid(1, 1, 1, 2.5)
+    t1 = t1.copy()
+    t1[0, 0] = len(c1)  # a node of the next magnet: must be rejected
+    keep = []
+    meshes = (_native.NsimMesh * 2)()
+    mats = (_native.NsimMaterial * 2)()
+    for k, (c, t) in enumerate([(c0, t0), (c1, t1)]):
+        c = np.ascontiguousarray(c, dtype=np.float64)
+        t = np.ascontiguousarray(t, dtype=np.int32)
+        keep += [c, t]
+        meshes[k] = _native.NsimMesh(len(c), len(t), _native._dptr(c),
+                                     t.ctypes.data_as(_native._ip))
+        mats[k] = _native.NsimMaterial(0.86, -24.0, -0.2, -0.1, 0.1, 1.2566)
+    h = ctypes.c_void_p()
+    rc = L.nsim_create_batch(2, meshes, mats, None, ctypes.byref(h))
+    assert rc < 0 and not h.value
+    assert b"magnet 1" in L.nsim_last_error() and b"out of range" in L.nsim_last_error()
+    assert L.nsim_create_batch(0, meshes, mats, None, ctypes.byref(h)) < 0
