@@ -77,6 +77,9 @@ def run_gpu(prec, coords, tets, mat, Hext, states, warmup, steps, rtol, torch):
     g = NativeMagnet(coords, tets, mat.su_coeffs(), precision=prec, nbc_rtol=rtol,
                      dbc_rtol=rtol)
     g.set_field("H_ext", Hext)
+    g.set_field("m", states[0])
+    tuned = g.autotune()
+    log("%s autotuned: %s" % (prec, tuned))
     n3 = 3 * g.n
     Y = torch.from_numpy(states.reshape(len(states), n3)).to("cuda")
     YD = torch.zeros_like(Y)
@@ -116,6 +119,7 @@ def run_gpu(prec, coords, tets, mat, Hext, states, warmup, steps, rtol, torch):
         "kernel_ms": kern,
         "nb": g.nb, "n": g.n, "nnz": g.nnz,
         "cg_grid": g.stat("cg_grid"),
+        "autotuned": tuned,
     }
     g.close()
     return res, out
@@ -168,6 +172,168 @@ def run_cpu(coords, tets, mat, Hext, states, rtol, budget_s):
     return done / t_used, done, setup, outs
 
 
+def batch_workload(count=64, seed=4, h=2.5):
+    """64 permalloy cuboids: sides U[20,120] x U[20,80] x U[5,15] nm rounded to
+    2.5 nm cubes, Ms U[6e5,1e6] A/m, uniaxial K U[0,5e4] J/m^3 along a random
+    axis, applied field a random unit vector times U[0,50] mT, alpha 0.5; m0
+    normalised i.i.d. Gaussian per node."""
+    from nmag import MagMaterial, SI, kuhn_cuboid, to_su
+    rng = np.random.default_rng(seed)
+    mags = []
+    for k in range(count):
+        L = rng.uniform([20.0, 20.0, 5.0], [120.0, 80.0, 15.0])
+        nx, ny, nz = [max(1, int(round(v / h))) for v in L]
+        Ms = rng.uniform(6e5, 1e6)
+        K1 = rng.uniform(0.0, 5e4)
+        ax = rng.standard_normal(3)
+        ax /= np.linalg.norm(ax)
+        hd = rng.standard_normal(3)
+        hd /= np.linalg.norm(hd)
+        H = hd * rng.uniform(0.0, 50.0) * 1e-3 / (4e-7 * math.pi)  # A/m
+        coords, tets = kuhn_cuboid(nx, ny, nz, h)
+        n = len(coords)
+        mat = MagMaterial("Py%d" % k, Ms=SI(Ms, "A/m"), exchange_coupling=SI(1.3e-11, "J/m"),
+                          llg_gamma_G=SI(2.211e5, "m/A s"), llg_damping=0.5)
+        m0 = rng.standard_normal((n, 3))
+        m0 /= np.linalg.norm(m0, axis=1, keepdims=True)
+        fields = {"m": m0,
+                  "H_ext": np.tile(float(to_su(SI(1.0, "A/m"), "A/m")) * H, (n, 1)),
+                  "K1": np.full(n, float(to_su(SI(K1, "J/m^3"), "J/m^3"))),
+                  "axis": np.tile(ax, (n, 1))}
+        mags.append(dict(dims=(nx, ny, nz), coords=coords, tets=tets, mat=mat, fields=fields))
+    return mags
+
+
+def run_batch(steps, warmup, rtol, window_ps, torch):
+    """Batching form: one handle for all magnets (grouped SpMV/BEM, one
+    persistent CG) against one FP64 handle per magnet run one after another."""
+    from nmag._native import NativeBatch, NativeMagnet
+    mags = batch_workload()
+    meshes = [(g["coords"], g["tets"]) for g in mags]
+    mats = [g["mat"].su_coeffs() for g in mags]
+    sizes = np.array([len(g["coords"]) for g in mags])
+    log("batch: %d magnets, %d nodes total (%d..%d)" % (len(mags), sizes.sum(), sizes.min(),
+                                                         sizes.max()))
+    # a sequence of states: each a small random rotation of the previous one
+    rng = np.random.default_rng(5)
+    m = np.concatenate([g["fields"]["m"] for g in mags])
+    states = []
+    for _ in range(warmup + steps):
+        m = m + 0.002 * rng.standard_normal(m.shape)
+        m /= np.linalg.norm(m, axis=1, keepdims=True)
+        states.append(m.reshape(-1).copy())
+    states = np.array(states)
+    cat = {key: np.concatenate([g["fields"][key] for g in mags]) for key in
+           ("H_ext", "K1", "axis")}
+    out = {"magnets": len(mags), "nodes_total": int(sizes.sum()),
+           "nodes_min": int(sizes.min()), "nodes_max": int(sizes.max())}
+
+    def setup(mag, f):
+        for key in ("H_ext", "axis", "K1"):
+            mag.set_field(key, f[key])
+
+    # batched field evaluations
+    for prec in ("fp64", "mixed"):
+        t0 = time.perf_counter()
+        b = NativeBatch(meshes, mats, precision=prec, nbc_rtol=rtol, dbc_rtol=rtol)
+        t_setup = time.perf_counter() - t0
+        setup(b, cat)
+        b.set_field("m", states[0])
+        tuned = b.autotune()
+        Y = torch.from_numpy(states).to("cuda")
+        YD = torch.zeros_like(Y)
+        torch.cuda.synchronize()
+        for k in range(warmup):
+            b.rhs_device(Y[k].data_ptr(), YD[k].data_ptr())
+        b.synchronize()
+        t0 = time.perf_counter()
+        for k in range(steps):
+            b.rhs_device(Y[warmup + k].data_ptr(), YD[warmup + k].data_ptr())
+        b.synchronize()
+        dt = time.perf_counter() - t0
+        assert b.stat("cg_error") == 0
+        out["batched_" + prec] = {"batch_evals_per_s": round(steps / dt, 2),
+                                  "magnet_evals_per_s": round(len(mags) * steps / dt, 1),
+                                  "ms_per_batch_eval": round(1e3 * dt / steps, 4),
+                                  "setup_s": round(t_setup, 2), "cg_grid": b.stat("cg_grid"),
+                                  "autotuned": tuned}
+        log("batched %s: %.1f batch evals/s (setup %.1f s)" % (prec, steps / dt, t_setup))
+        if prec == "fp64":
+            ref_out = YD[warmup:].cpu().numpy()
+        del Y, YD
+        b.close()
+
+    # 64 sequential FP64 solves: one handle per magnet, one magnet after another
+    t0 = time.perf_counter()
+    singles = [NativeMagnet(c, t, mt, precision="fp64", nbc_rtol=rtol, dbc_rtol=rtol)
+               for (c, t), mt in zip(meshes, mats)]
+    t_setup = time.perf_counter() - t0
+    off = np.concatenate([[0], np.cumsum(sizes)])
+    Ys, YDs = [], []
+    for j, s in enumerate(singles):
+        setup(s, mags[j]["fields"])
+        Ys.append(torch.from_numpy(np.ascontiguousarray(
+            states.reshape(len(states), -1, 3)[:, off[j]:off[j + 1]].reshape(len(states), -1))
+        ).to("cuda"))
+        YDs.append(torch.zeros_like(Ys[-1]))
+    torch.cuda.synchronize()
+    for j, s in enumerate(singles):
+        for k in range(warmup):
+            s.rhs_device(Ys[j][k].data_ptr(), YDs[j][k].data_ptr())
+        s.synchronize()
+    t0 = time.perf_counter()
+    for j, s in enumerate(singles):
+        for k in range(steps):
+            s.rhs_device(Ys[j][warmup + k].data_ptr(), YDs[j][warmup + k].data_ptr())
+        s.synchronize()
+    dt = time.perf_counter() - t0
+    seq = np.concatenate([YDs[j][warmup:].cpu().numpy().reshape(steps, -1, 3)
+                          for j in range(len(singles))], axis=1).reshape(steps, -1)
+    out["sequential_fp64"] = {"batch_evals_per_s": round(steps / dt, 2),
+                              "magnet_evals_per_s": round(len(mags) * steps / dt, 1),
+                              "setup_s": round(t_setup, 2)}
+    out["batched_vs_sequential_fp64"] = round(
+        out["batched_fp64"]["batch_evals_per_s"] / out["sequential_fp64"]["batch_evals_per_s"], 3)
+    out["batched_mixed_vs_sequential_fp64"] = round(
+        out["batched_mixed"]["batch_evals_per_s"] /
+        out["sequential_fp64"]["batch_evals_per_s"], 3)
+    out["batched_vs_sequential_max_rel_diff"] = float(
+        np.max(np.abs(ref_out - seq)) / np.max(np.abs(seq)))
+    log("sequential fp64: %.1f batch evals/s" % (steps / dt))
+
+    # relaxation window at alpha 0.5: one integrator over the batch against one
+    # integrator per magnet, FP64, rtol = atol = 1e-6
+    if window_ps > 0:
+        t0 = time.perf_counter()
+        b = NativeBatch(meshes, mats, precision="fp64", nbc_rtol=rtol, dbc_rtol=rtol)
+        setup(b, cat)
+        b.set_field("m", np.concatenate([g["fields"]["m"] for g in mags]))
+        b.ts_init(0.0, 1e-6, 1e-6)
+        t1 = time.perf_counter()
+        b.advance(window_ps)
+        wall_b = time.perf_counter() - t1
+        steps_b = b.stat("ts_steps")
+        b.close()
+        wall_s, steps_s = 0.0, 0
+        for j, s in enumerate(singles):
+            s.set_field("m", mags[j]["fields"]["m"])
+            s.ts_init(0.0, 1e-6, 1e-6)
+            t1 = time.perf_counter()
+            s.advance(window_ps)
+            wall_s += time.perf_counter() - t1
+            steps_s += int(s.stat("ts_steps"))
+        out["relax_window"] = {"window_ps": window_ps, "batched_wall_s": round(wall_b, 3),
+                               "batched_steps": int(steps_b),
+                               "sequential_wall_s": round(wall_s, 3),
+                               "sequential_steps_total": steps_s,
+                               "speedup": round(wall_s / wall_b, 3)}
+        log("relax window %.1f ps: batched %.2f s, sequential %.2f s" % (window_ps, wall_b,
+                                                                       wall_s))
+    for s in singles:
+        s.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -181,6 +347,11 @@ def main():
     ap.add_argument("--sim-window", type=float, default=20.0,
                     help="ps of LLG dynamics timed for sim-ns/hour (1000 = the full 1 ns)")
     ap.add_argument("--no-dynamics", action="store_true")
+    ap.add_argument("--no-batch", action="store_true",
+                    help="skip the 64-magnet batching leg")
+    ap.add_argument("--batch-steps", type=int, default=20)
+    ap.add_argument("--batch-window", type=float, default=2.0,
+                    help="ps of relaxation timed for the batch (0 skips)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -227,13 +398,13 @@ def main():
                 "traffic": None, "alg_bytes_per_launch": int(alg_bytes)}
 
     # per-iteration algorithmic bytes of one PCG iteration (SpMV over the
-    # stiffness pattern with FP32 values + 4-byte columns, row pointers, and
+    # stiffness pattern with FP64 values + 4-byte columns, row pointers, and
     # about eight FP64 vector passes), times the measured iterations per solve
-    cg_iter_bytes = 8 * nnz + 4 * (n + 1) + 64 * n
+    cg_iter_bytes = 12 * nnz + 4 * (n + 1) + 64 * n
     rooflines = {
-        "cg_nbc": roofline("k_cg<float> Neumann solve",
+        "cg_nbc": roofline("k_cg<double> Neumann solve",
                            cg_iter_bytes * max(mixed["cg_iters_nbc"], 1), kern["cg_nbc"]),
-        "cg_dbc": roofline("k_cg<float> Dirichlet solve",
+        "cg_dbc": roofline("k_cg<double> Dirichlet solve",
                            cg_iter_bytes * max(mixed["cg_iters_dbc"], 1), kern["cg_dbc"]),
         "bem": roofline("k_bem_gemv<float>", gemv_bytes(nb, 4), kern["bem"]),
     }
@@ -253,6 +424,10 @@ def main():
         dyn["window_ps"] = args.sim_window
         dyn["mean_m_max_dev_mixed_vs_fp64"] = float(np.abs(avgs["mixed"] - avgs["fp64"]).max())
         dyn["integrator"] = "BDF order<=2, Newton-GMRES, rtol=atol=1e-6, from the seeded S-state"
+
+    batch = None
+    if not args.no_batch:
+        batch = run_batch(args.batch_steps, 3, args.rtol, args.batch_window, torch)
 
     cpu = None
     if not args.no_cpu:
@@ -286,7 +461,9 @@ def main():
                  "ms_per_step": round(results["fp64"]["ms_per_eval"], 4),
                  "kernel_ms": results["fp64"]["kernel_ms"],
                  "cg_iters": [results["fp64"]["cg_iters_nbc"], results["fp64"]["cg_iters_dbc"]],
-                 "bem_gemv_GBs": round(bem_fp64_gbs, 1)},
+                 "bem_gemv_GBs": round(bem_fp64_gbs, 1),
+                 "autotuned": results["fp64"]["autotuned"]},
+        "mixed_autotuned": mixed["autotuned"],
         "mixed_kernel_ms": kern,
         "mixed_cg_iters": [mixed["cg_iters_nbc"], mixed["cg_iters_dbc"]],
         "mixed_vs_fp64_speedup": round(mixed["evals_per_s"] / results["fp64"]["evals_per_s"], 3),
@@ -296,6 +473,7 @@ def main():
         "roofline_all": rooflines,
         "cpu_baseline": cpu,
         "dynamics": dyn,
+        "batch": batch,
     }
     print(json.dumps(line))
 

@@ -74,6 +74,16 @@ const char* nsim_last_error(void);
  * magnet).  The CG stopping test is on the residual of the whole batch. */
 int nsim_create_batch(int n_magnets, const nsim_mesh* meshes, const nsim_material* mats,
                       const nsim_options* opt, nsim_handle** out);
+/* Autotuner (no counterpart in the reference, whose PETSc kernels have no
+ * tile shapes): times the candidates on the device and keeps the fastest.
+ * what bit 0: the BEM GEMV tile shape (1/2/4 rows streamed together per
+ * wavefront x 1/2/4/8 workgroups per CU); bit 1: the CG layout (1/2/4 rows per
+ * thread, i.e. parts of 256/512/1024 nodes), timed on a cold Neumann solve of
+ * the current m's load (so set m first).  The choice shows in nsim_get_stat
+ * ("bem_rows", "bem_wg_per_cu", "cg_rows_per_thread", "tuned_bem_us",
+ * "tuned_cg_us"); nsim_set_tuning sets the same knobs by hand. */
+int nsim_autotune(nsim_handle* h, int what);
+
 /* Instance-offset table of a handle: magnet k owns nodes
  * [node_off[k], node_off[k+1]) and boundary nodes [boundary_off[k],
  * boundary_off[k+1]); both arrays have n_magnets + 1 entries (either may be

@@ -244,16 +244,26 @@ __global__ __launch_bounds__(256) void k_bem_gemv(const TB* __restrict__ Bbase,
   }
 }
 
-constexpr int kGemvRows = 2;
+template <typename TB, int ROWS>
+void launch_rows(const DeviceOps& d, size_t lds, const double* phi1b, double* phi2,
+                 hipStream_t st) {
+  k_bem_gemv<TB, ROWS><<<d.gemv_grid, 256, lds, st>>>(
+      (const TB*)d.B, d.seg_Boff, d.seg_ldb, d.seg_nb, d.seg_boff, d.wg_seg, d.wg_row0,
+      d.gemv_rpw, phi1b, d.bnode, phi2);
+}
 
 template <typename TB>
 hipError_t gemv_typed(const DeviceOps& d, const double* phi1b, double* phi2, hipStream_t st) {
   using X = typename BemVec<TB>::X;
   const size_t lds = (size_t)d.max_ldb * sizeof(X);
   if (lds > 64 * 1024) return hipErrorInvalidValue;
-  k_bem_gemv<TB, kGemvRows><<<d.gemv_grid, 256, lds, st>>>(
-      (const TB*)d.B, d.seg_Boff, d.seg_ldb, d.seg_nb, d.seg_boff, d.wg_seg, d.wg_row0,
-      d.gemv_rpw, phi1b, d.bnode, phi2);
+  if (d.gemv_rpw % d.gemv_rows != 0) return hipErrorInvalidConfiguration;
+  switch (d.gemv_rows) {
+    case 1: launch_rows<TB, 1>(d, lds, phi1b, phi2, st); break;
+    case 2: launch_rows<TB, 2>(d, lds, phi1b, phi2, st); break;
+    case 4: launch_rows<TB, 4>(d, lds, phi1b, phi2, st); break;
+    default: return hipErrorInvalidConfiguration;
+  }
   return hipGetLastError();
 }
 
@@ -276,15 +286,19 @@ hipError_t assemble_typed(const DeviceOps& d, hipStream_t st) {
 
 }  // namespace
 
-void plan_bem_gemv(DeviceOps& d, int num_cus, std::vector<int>* wg_seg,
-                   std::vector<int>* wg_row0) {
-  // two workgroups of four wavefronts per CU; every wavefront gets the same
-  // number of rows (a multiple of kGemvRows), every workgroup one segment
+bool plan_bem_gemv(DeviceOps& d, int num_cus, int rows_per_pass, int wg_per_cu,
+                   std::vector<int>* wg_seg, std::vector<int>* wg_row0) {
+  if (rows_per_pass != 1 && rows_per_pass != 2 && rows_per_pass != 4) return false;
+  if (wg_per_cu < 1 || wg_per_cu > 8) return false;
+  // wg_per_cu workgroups of four wavefronts per CU; every wavefront gets the
+  // same number of rows (a multiple of rows_per_pass), every workgroup one
+  // segment
   long long rows = 0;
   for (int nb : d.h_seg_nb) rows += nb;
-  const long long waves = 4LL * 2 * (num_cus > 0 ? num_cus : 256);
+  const long long waves = 4LL * wg_per_cu * (num_cus > 0 ? num_cus : 256);
+  const int R = rows_per_pass;
   int rpw = (int)((rows + waves - 1) / waves);
-  rpw = std::max(kGemvRows, ((rpw + kGemvRows - 1) / kGemvRows) * kGemvRows);
+  rpw = std::max(R, ((rpw + R - 1) / R) * R);
   const int rows_per_wg = 4 * rpw;
   wg_seg->clear();
   wg_row0->clear();
@@ -297,7 +311,10 @@ void plan_bem_gemv(DeviceOps& d, int num_cus, std::vector<int>* wg_seg,
     }
   }
   d.gemv_rpw = rpw;
+  d.gemv_rows = R;
+  d.gemv_wg_per_cu = wg_per_cu;
   d.gemv_grid = (int)wg_seg->size();
+  return true;
 }
 
 hipError_t launch_bem_assemble(const DeviceOps& d, hipStream_t st) {

@@ -73,6 +73,7 @@ struct nsim_handle {
   std::vector<int> node_off;     // magnet k owns nodes [node_off[k], node_off[k+1])
   std::vector<int> bnd_off;      // and boundary nodes [bnd_off[k], bnd_off[k+1])
   long long bem_elems = 0;       // stored BEM entries over all segments
+  double tuned_bem_us = 0.0, tuned_cg_us = 0.0;  // autotuner's best timings
   nsim_options opt{};
   bool mixed = false;
   int device = 0, num_cus = 0;
@@ -135,6 +136,31 @@ void flush_timers(nsim_handle* h) {
     }
     kv.second.pending.clear();
   }
+}
+
+// (Re)plans the grouped BEM GEMV for a tile shape and uploads its
+// workgroup table.
+int setup_gemv(nsim_handle* h, int rows_per_pass, int wg_per_cu) {
+  if (h->d.nb == 0) return 0;
+  DeviceOps trial = h->d;
+  std::vector<int> wg_seg, wg_row0;
+  if (!plan_bem_gemv(trial, h->num_cus, rows_per_pass, wg_per_cu, &wg_seg, &wg_row0))
+    return fail("BEM GEMV tile shape out of range (rows 1/2/4, workgroups per CU 1..8)");
+  int *ds = nullptr, *dr = nullptr;
+  hipError_t e = upload(&ds, wg_seg.data(), wg_seg.size());
+  if (e == hipSuccess) e = upload(&dr, wg_row0.data(), wg_row0.size());
+  if (e != hipSuccess) {
+    (void)hipFree(ds);
+    (void)hipFree(dr);
+    return fail(std::string("BEM GEMV plan upload: ") + hipGetErrorString(e));
+  }
+  (void)hipStreamSynchronize(h->stream);
+  (void)hipFree(h->d.wg_seg);
+  (void)hipFree(h->d.wg_row0);
+  trial.wg_seg = ds;
+  trial.wg_row0 = dr;
+  h->d = trial;
+  return 0;
 }
 
 // (Re)builds the CG operator with `rpt` rows per thread (parts of 256*rpt
@@ -425,10 +451,7 @@ int create_impl(const char* who, int ninst, const nsim_mesh* meshes,
     UP(d.seg_ldb, d.h_seg_ldb);
     UP(d.seg_boff, d.h_seg_boff);
     UP(d.seg_Boff, d.h_seg_Boff);
-    std::vector<int> wg_seg, wg_row0;
-    plan_bem_gemv(d, h->num_cus, &wg_seg, &wg_row0);
-    UP(d.wg_seg, wg_seg);
-    UP(d.wg_row0, wg_row0);
+    if (setup_gemv(h, 2, 2) != 0) return cleanup(-1);
     size_t esz = d.btype == kBemF64 ? 8 : d.btype == kBemF32 ? 4 : 2;
     e = hipMalloc(&d.B, (size_t)h->bem_elems * esz);
     if (e != hipSuccess) return cleanup(fail(std::string("BEM alloc: ") + hipGetErrorString(e)));
@@ -514,6 +537,8 @@ void nsim_destroy(nsim_handle* h) {
     }
   for (void* p : h->allocations) (void)hipFree(p);
   for (void* p : h->cg_allocs) (void)hipFree(p);
+  if (h->d.wg_seg) (void)hipFree(h->d.wg_seg);
+  if (h->d.wg_row0) (void)hipFree(h->d.wg_row0);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
 }
@@ -542,7 +567,81 @@ int nsim_set_tuning(nsim_handle* h, const char* key, int value) {
     h->cg_coarse = value != 0;
     return 0;
   }
+  if (k == "bem_rows") return setup_gemv(h, value, h->d.gemv_wg_per_cu);
+  if (k == "bem_wg_per_cu") return setup_gemv(h, h->d.gemv_rows, value);
   return fail("unknown tuning key: " + k);
+}
+
+int nsim_autotune(nsim_handle* h, int what) {
+  if (!h) return fail("nsim_autotune: null handle");
+  hipEvent_t a, b;
+  HIPCHK(hipEventCreate(&a));
+  HIPCHK(hipEventCreate(&b));
+  auto done = [&](int rc) {
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    return rc;
+  };
+  auto elapsed_ms = [&]() {
+    float ms = 0.f;
+    if (hipEventSynchronize(b) != hipSuccess || hipEventElapsedTime(&ms, a, b) != hipSuccess)
+      return 1e30;
+    return (double)ms;
+  };
+  const bool demag = h->opt.do_demag != 0 && h->d.nb > 0;
+  if ((what & 1) && h->d.nb > 0) {
+    // BEM GEMV tile shape: rows streamed together per wavefront x workgroups per CU
+    double best = 1e30;
+    int br = h->d.gemv_rows, bw = h->d.gemv_wg_per_cu;
+    for (int r : {1, 2, 4})
+      for (int w : {1, 2, 4, 8}) {
+        if (setup_gemv(h, r, w) != 0) continue;
+        for (int k = 0; k < 2; k++)
+          if (launch_bem_gemv(h->d, h->phi1b, h->phi2, h->stream, h->num_cus) != hipSuccess)
+            return done(fail("nsim_autotune: GEMV launch failed"));
+        (void)hipEventRecord(a, h->stream);
+        for (int k = 0; k < 10; k++)
+          (void)launch_bem_gemv(h->d, h->phi1b, h->phi2, h->stream, h->num_cus);
+        (void)hipEventRecord(b, h->stream);
+        const double us = elapsed_ms() * 100.0;  // per launch
+        if (us < best) best = us, br = r, bw = w;
+      }
+    if (setup_gemv(h, br, bw) != 0) return done(-1);
+    h->tuned_bem_us = best;
+  }
+  if ((what & 2) && demag) {
+    // CG layout: rows per thread (part size), timed on a cold Neumann solve
+    // of the load of the current m
+    if (launch_exch_div(h->d, h->mixed, h->m, h->Hexch, h->rho, true, h->stream) != hipSuccess)
+      return done(fail("nsim_autotune: load launch failed"));
+    double best = 1e30;
+    int brpt = h->cg.rpt;
+    for (int rpt : {1, 2, 4}) {
+      HIPCHK(hipStreamSynchronize(h->stream));
+      if (setup_cg(h, rpt) != 0) continue;
+      CgWork& w = h->cgw;
+      w.sync = h->sync_nbc;
+      double tot = 0.0;
+      for (int k = 0; k < 3; k++) {
+        HIPCHK(hipMemsetAsync(h->phi1, 0, sizeof(double) * h->d.n, h->stream));
+        (void)hipEventRecord(a, h->stream);
+        HIPCHK(launch_cg(h->cg, w, false, h->cg_coarse, h->rho, h->phi1, h->opt.nbc_rtol,
+                         h->opt.nbc_maxits, nullptr, nullptr, h->phi1b, h->d.blocal,
+                         h->stream));
+        (void)hipEventRecord(b, h->stream);
+        tot += elapsed_ms();
+      }
+      if (tot < best) best = tot, brpt = rpt;
+    }
+    HIPCHK(hipStreamSynchronize(h->stream));
+    if (setup_cg(h, brpt) != 0) return done(-1);
+    HIPCHK(hipMemsetAsync(h->phi1, 0, sizeof(double) * h->d.n, h->stream));
+    HIPCHK(hipMemsetAsync(h->phi2, 0, sizeof(double) * h->d.n, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    h->tuned_cg_us = best * 1e3 / 3.0;
+  }
+  g_last_error.clear();
+  return done(0);
 }
 
 int nsim_set_field(nsim_handle* h, const char* name, const double* data) {
@@ -709,6 +808,16 @@ int nsim_get_stat(nsim_handle* h, const char* name, double* out) {
   }
   if (s == "cg_register_slots") {
     *out = h->cg.maxw;
+    return 0;
+  }
+  if (s == "bem_rows" || s == "bem_wg_per_cu" || s == "bem_grid" || s == "n_magnets" ||
+      s == "tuned_bem_us" || s == "tuned_cg_us") {
+    *out = s == "bem_rows"        ? h->d.gemv_rows
+           : s == "bem_wg_per_cu" ? h->d.gemv_wg_per_cu
+           : s == "bem_grid"      ? h->d.gemv_grid
+           : s == "n_magnets"     ? h->d.ninst
+           : s == "tuned_bem_us"  ? h->tuned_bem_us
+                                  : h->tuned_cg_us;
     return 0;
   }
   unsigned a[8], b[8];

@@ -45,6 +45,7 @@ struct DeviceOps {
   int* wg_seg = nullptr;   // GEMV workgroup -> segment
   int* wg_row0 = nullptr;  // GEMV workgroup -> first row in the segment
   int gemv_grid = 0, gemv_rpw = 0, max_ldb = 0;
+  int gemv_rows = 2, gemv_wg_per_cu = 2;  // tile shape: rows per pass, workgroups per CU
   // coefficients: one entry per magnet; inst[i] = magnet of node i (null
   // when the handle holds a single magnet)
   Coeffs* coefs = nullptr;
@@ -108,10 +109,12 @@ hipError_t launch_cg(const CgDevice& c, CgWork& w, bool dirichlet, bool coarse,
                      hipStream_t st);
 hipError_t launch_bem_gemv(const DeviceOps& d, const double* phi1b, double* phi2,
                            hipStream_t st, int num_cus);
-// Host plan of the grouped GEMV: fills d.gemv_grid/gemv_rpw/max_ldb and the
-// workgroup -> (segment, first row) table to upload.
-void plan_bem_gemv(DeviceOps& d, int num_cus, std::vector<int>* wg_seg,
-                   std::vector<int>* wg_row0);
+// Host plan of the grouped GEMV for a tile shape (rows_per_pass 1, 2 or 4 rows
+// streamed together by a wavefront; wg_per_cu 1..8 workgroups per CU): fills
+// d.gemv_* and max_ldb and the workgroup -> (segment, first row) table to
+// upload.  False for a shape outside those ranges.
+bool plan_bem_gemv(DeviceOps& d, int num_cus, int rows_per_pass, int wg_per_cu,
+                   std::vector<int>* wg_seg, std::vector<int>* wg_row0);
 hipError_t launch_demag_llg(const DeviceOps& d, bool mixed, const FieldArgs& f,
                             const FieldBufs& b, double* dmdt, bool do_demag, hipStream_t st);
 hipError_t launch_jtimes(const DeviceOps& d, bool mixed, const double* m, const double* Htotal,

@@ -65,6 +65,7 @@ SIGNATURES = {
                                          ctypes.POINTER(NsimMaterial),
                                          ctypes.POINTER(NsimOptions), ctypes.POINTER(_vp)]),
     "nsim_get_batch_layout": (ctypes.c_int, [_vp, _ip, _ip]),
+    "nsim_autotune": (ctypes.c_int, [_vp, ctypes.c_int]),
 }
 
 _lib = None
@@ -172,8 +173,11 @@ class NativeMagnet:
         return (self.n,)
 
     def set_field(self, name, data):
-        a = np.ascontiguousarray(np.broadcast_to(np.asarray(data, dtype=np.float64),
-                                                 self._size(name)))
+        shape = self._size(name)
+        a = np.asarray(data, dtype=np.float64)
+        if a.size == int(np.prod(shape)):
+            a = a.reshape(shape)
+        a = np.ascontiguousarray(np.broadcast_to(a, shape))
         _check(lib().nsim_set_field(self._h, name.encode(), _dptr(a)), "nsim_set_field(%s)" % name)
 
     def get_field(self, name):
@@ -227,6 +231,14 @@ class NativeMagnet:
 
     def tune(self, key, value):
         _check(lib().nsim_set_tuning(self._h, key.encode(), int(value)), "nsim_set_tuning")
+
+    def autotune(self, bem=True, cg=True):
+        """Time the tile shapes on the device and keep the fastest; returns the
+        choice.  The CG leg solves the current m's load, so set m first."""
+        _check(lib().nsim_autotune(self._h, (1 if bem else 0) | (2 if cg else 0)),
+               "nsim_autotune")
+        return {k: self.stat(k) for k in ("bem_rows", "bem_wg_per_cu", "cg_rows_per_thread",
+                                          "tuned_bem_us", "tuned_cg_us")}
 
     def synchronize(self):
         _check(lib().nsim_synchronize(self._h), "nsim_synchronize")

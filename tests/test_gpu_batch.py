@@ -158,3 +158,27 @@ def test_batch_integration_matches_single_magnets():
         avg_s = (V[:, None] * ms).sum(0) / V.sum()
         assert np.max(np.abs(avg_b - avg_s)) < 1e-5, k
         assert np.max(np.abs(mb[k] - ms)) < 1e-4, k
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_autotuner_keeps_the_fields(precision):
+    """Every tile shape the autotuner may pick computes the same fields."""
+    meshes, mats, fields = _batch_inputs()
+    b = _batch(meshes, mats, precision, rtol=1e-12)
+    _set_all(b, fields)
+    b.execute("update_dmdt")
+    ref = {k: b.get_field(k) for k in ("H_demag", "dmdt")}
+    choice = b.autotune()
+    assert choice["bem_rows"] in (1, 2, 4) and 1 <= choice["bem_wg_per_cu"] <= 8
+    assert choice["cg_rows_per_thread"] in (1, 2, 4)
+    assert choice["tuned_bem_us"] > 0 and choice["tuned_cg_us"] > 0
+    for r in (1, 4):
+        for w in (1, 8):
+            b.tune("bem_rows", r)
+            b.tune("bem_wg_per_cu", w)
+            b.set_field("phi1", np.zeros(b.n))
+            b.set_field("phi2", np.zeros(b.n))
+            b.execute("update_dmdt")
+            assert rel_err(b.get_field("H_demag"), ref["H_demag"]) < 1e-9, (r, w)
+            assert rel_err(b.get_field("dmdt"), ref["dmdt"]) < 1e-9, (r, w)
+    assert b.stat("cg_error") == 0 and b.stat("cg_unconverged") == 0
