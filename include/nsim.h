@@ -74,6 +74,46 @@ const char* nsim_last_error(void);
  * magnet).  The CG stopping test is on the residual of the whole batch. */
 int nsim_create_batch(int n_magnets, const nsim_mesh* meshes, const nsim_material* mats,
                       const nsim_options* opt, nsim_handle** out);
+/* ---- Differentiation form (no counterpart in the reference, which has no
+ * adjoint).  A handle with a single magnet only. ---- */
+
+/* Vector-Jacobian product of the right-hand side at y: y_bar = J(y)^T lam
+ * with the full Jacobian (exchange, anisotropy, demag through both Poisson
+ * solves and B^T, the LLG torque); optionally K1_bar = (df/dK1)^T lam per
+ * node, *Ms_bar = (df/dMs)^T lam, Hext_bar = (df/dH_ext)^T lam (3N).  Host
+ * pointers.  The handle's fields are left at y. */
+int nsim_rhs_vjp(nsim_handle* h, double t, const double* y, const double* lam, double* y_bar,
+                 double* K1_bar, double* Ms_bar, double* Hext_bar);
+
+/* A fixed-step classical Runge-Kutta trajectory from the handle's m, with the
+ * handle's H_ext plus piecewise-constant pulses: during window w = [w L,
+ * (w+1) L) (L = pulse_len, a multiple of dt) the applied field gains
+ * amps[w] * pulse_dir.  States are kept every checkpoint_every steps for the
+ * adjoint, which recomputes the steps in between. */
+typedef struct {
+  double dt;            /* step, simulation time units (ps)                 */
+  int n_steps;
+  int checkpoint_every;
+  int n_pulses;
+  double pulse_len;
+  double pulse_dir[3];
+} nsim_traj;
+
+/* Runs the trajectory; the handle's m becomes m(n_steps dt). */
+int nsim_traj_forward(nsim_handle* h, const nsim_traj* tr, const double* amps);
+/* Discrete adjoint of the last nsim_traj_forward (same tr and amps): given
+ * m_bar_final = dLoss/dm(T) (3N), returns dLoss/dm(0) (3N), dLoss/dK1 per node
+ * (N), dLoss/dMs and dLoss/damps (n_pulses).  Outputs may be null. */
+int nsim_traj_adjoint(nsim_handle* h, const nsim_traj* tr, const double* amps,
+                      const double* m_bar_final, double* m_bar_initial, double* K1_bar,
+                      double* Ms_bar, double* amps_bar);
+
+/* Changes magnet k's material coefficients in place (the operators do not
+ * depend on them).  Replaces what the reference does when a simulation is
+ * rebuilt with new MagMaterial parameters (interface/nmag/main.py:423-471);
+ * used to vary Ms in gradient checks. */
+int nsim_set_material(nsim_handle* h, int magnet, const nsim_material* mat);
+
 /* Autotuner (no counterpart in the reference, whose PETSc kernels have no
  * tile shapes): times the candidates on the device and keeps the fastest.
  * what bit 0: the BEM GEMV tile shape (1/2/4 rows streamed together per

@@ -15,8 +15,9 @@
 #include <vector>
 
 #include "nsim_device.h"
-#include "nsim_internal.h"
+#include "nsim_handle.h"
 #include "nsim_integrator.h"
+#include "nsim_internal.h"
 #include "nsim_vec.h"
 
 using namespace nsim;
@@ -54,56 +55,6 @@ hipError_t zeros(T** dst, size_t count) {
   if (e != hipSuccess) return e;
   return hipMemset(*dst, 0, count * sizeof(T));
 }
-
-struct Timer {
-  std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
-  double ms = 0.0;
-  long count = 0;
-};
-
-}  // namespace
-
-struct nsim_handle {
-  HostOperators host;
-  DeviceOps d;
-  CgWork cgw;
-  unsigned* sync_nbc = nullptr;
-  unsigned* sync_dbc = nullptr;
-  Coeffs coef{};                 // the first magnet's coefficients
-  std::vector<int> node_off;     // magnet k owns nodes [node_off[k], node_off[k+1])
-  std::vector<int> bnd_off;      // and boundary nodes [bnd_off[k], bnd_off[k+1])
-  long long bem_elems = 0;       // stored BEM entries over all segments
-  double tuned_bem_us = 0.0, tuned_cg_us = 0.0;  // autotuner's best timings
-  nsim_options opt{};
-  bool mixed = false;
-  int device = 0, num_cus = 0;
-  hipStream_t stream = nullptr;
-  CgDevice cg;                  // the CG's reordered operator
-  std::vector<void*> cg_allocs; // its device buffers
-  bool cg_coarse = true;        // two-level preconditioner on
-  // field vectors (site-major 3-vectors or scalars)
-  double *m = nullptr, *Hext = nullptr, *k1 = nullptr, *k2 = nullptr, *axis = nullptr,
-         *pin = nullptr;
-  double *Hexch = nullptr, *Hdemag = nullptr, *Hanis = nullptr, *Htotal = nullptr,
-         *dmdt = nullptr, *rho = nullptr, *phi1 = nullptr, *phi2 = nullptr, *phi = nullptr,
-         *phi1b = nullptr;
-  double *io_a = nullptr, *io_b = nullptr, *io_c = nullptr;  // staging for host-pointer calls
-  bool have_hext = false, have_anis = false, have_k2 = false, have_pin = false;
-  long rhs_evals = 0;
-  bool timers_on = false;
-  std::map<std::string, Timer> timers;
-  Integrator* ts = nullptr;
-  TsOptions ts_opt;
-  double* kappa = nullptr;  // negJ K_ii / V_i: dH_exch_i/dm_i
-  double* pc_Minv = nullptr;  // block-Jacobi preconditioner of nsim_psetup
-  std::vector<void*> allocations;
-
-  void track(void* p) {
-    if (p) allocations.push_back(p);
-  }
-};
-
-namespace {
 
 // Records a named device interval around a launch when timers are enabled.
 struct Span {
@@ -257,6 +208,19 @@ int device_jtimes(nsim_handle* h, const double* m, const double* v, double* Jv) 
   return 0;
 }
 
+int refresh_coefficients(nsim_handle* h) {
+  const HostOperators& o = h->host;
+  std::vector<double> kap(o.n);
+  for (int i = 0; i < o.n; i++)
+    kap[i] = h->coefs[o.inst.empty() ? 0 : o.inst[i]].negJ * o.Kdiag[i] / o.V[i];
+  HIPCHK(hipStreamSynchronize(h->stream));
+  HIPCHK(hipMemcpy(h->kappa, kap.data(), kap.size() * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(h->d.coefs, h->coefs.data(), h->coefs.size() * sizeof(Coeffs),
+                   hipMemcpyHostToDevice));
+  h->coef = h->coefs[0];
+  return 0;
+}
+
 hipStream_t handle_stream(nsim_handle* h) { return h->stream; }
 int handle_n(nsim_handle* h) { return h->d.n; }
 double* handle_m(nsim_handle* h) { return h->m; }
@@ -352,6 +316,7 @@ int create_impl(const char* who, int ninst, const nsim_mesh* meshes,
     coefs[k] = Coeffs{mk.Ms, mk.negJ, mk.c1, mk.c2, mk.c3, mk.mu0};
   }
   h->coef = coefs[0];
+  h->coefs = coefs;
 
   const HostOperators& o = h->host;
   DeviceOps& d = h->d;
@@ -519,6 +484,15 @@ int nsim_create_batch(int n_magnets, const nsim_mesh* meshes, const nsim_materia
   return create_impl("nsim_create_batch", n_magnets, meshes, mats, opt, out);
 }
 
+int nsim_set_material(nsim_handle* h, int magnet, const nsim_material* mat) {
+  if (!h || !mat) return fail("nsim_set_material: null argument");
+  if (magnet < 0 || magnet >= h->d.ninst) return fail("nsim_set_material: no such magnet");
+  if (!(mat->Ms > 0.0) || !(mat->mu0 > 0.0))
+    return fail("nsim_set_material: need Ms > 0 and mu0 > 0");
+  h->coefs[magnet] = Coeffs{mat->Ms, mat->negJ, mat->c1, mat->c2, mat->c3, mat->mu0};
+  return refresh_coefficients(h);
+}
+
 int nsim_get_batch_layout(nsim_handle* h, int* node_off, int* boundary_off) {
   if (!h) return fail("nsim_get_batch_layout: null handle");
   if (node_off) std::copy(h->node_off.begin(), h->node_off.end(), node_off);
@@ -530,6 +504,7 @@ void nsim_destroy(nsim_handle* h) {
   if (!h) return;
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   delete h->ts;
+  destroy_adjoint(h->adj);
   for (auto& kv : h->timers)
     for (auto& ev : kv.second.pending) {
       (void)hipEventDestroy(ev.first);

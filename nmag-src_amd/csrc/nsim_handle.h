@@ -1,0 +1,79 @@
+// The handle behind the C ABI: one magnet (or a batch of independent magnets)
+// with its operators in HBM, its field vectors, one HIP stream and the
+// persistent-CG communication buffers.  Internal to the library.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <string>
+#include <vector>
+
+#include "nsim_device.h"
+#include "nsim_integrator.h"
+#include "nsim_internal.h"
+
+namespace nsim {
+
+struct Timer {
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+  double ms = 0.0;
+  long count = 0;
+};
+
+struct AdjointState;  // buffers of the adjoint (nsim_traj.cpp)
+void destroy_adjoint(AdjointState* a);
+
+}  // namespace nsim
+
+struct nsim_handle {
+  nsim::HostOperators host;
+  nsim::DeviceOps d;
+  nsim::CgWork cgw;
+  unsigned* sync_nbc = nullptr;
+  unsigned* sync_dbc = nullptr;
+  nsim::Coeffs coef{};           // the first magnet's coefficients
+  std::vector<nsim::Coeffs> coefs;  // host copy of the per-magnet table
+  std::vector<int> node_off;     // magnet k owns nodes [node_off[k], node_off[k+1])
+  std::vector<int> bnd_off;      // and boundary nodes [bnd_off[k], bnd_off[k+1])
+  long long bem_elems = 0;       // stored BEM entries over all segments
+  double tuned_bem_us = 0.0, tuned_cg_us = 0.0;  // autotuner's best timings
+  nsim_options opt{};
+  bool mixed = false;
+  int device = 0, num_cus = 0;
+  hipStream_t stream = nullptr;
+  nsim::CgDevice cg;             // the CG's reordered operator
+  std::vector<void*> cg_allocs;  // its device buffers
+  bool cg_coarse = true;         // two-level preconditioner on
+  // field vectors (site-major 3-vectors or scalars)
+  double *m = nullptr, *Hext = nullptr, *k1 = nullptr, *k2 = nullptr, *axis = nullptr,
+         *pin = nullptr;
+  double *Hexch = nullptr, *Hdemag = nullptr, *Hanis = nullptr, *Htotal = nullptr,
+         *dmdt = nullptr, *rho = nullptr, *phi1 = nullptr, *phi2 = nullptr, *phi = nullptr,
+         *phi1b = nullptr;
+  double *io_a = nullptr, *io_b = nullptr, *io_c = nullptr;  // staging for host-pointer calls
+  bool have_hext = false, have_anis = false, have_k2 = false, have_pin = false;
+  long rhs_evals = 0;
+  bool timers_on = false;
+  std::map<std::string, nsim::Timer> timers;
+  nsim::Integrator* ts = nullptr;
+  nsim::TsOptions ts_opt;
+  double* kappa = nullptr;    // negJ K_ii / V_i: dH_exch_i/dm_i
+  double* pc_Minv = nullptr;  // block-Jacobi preconditioner of nsim_psetup
+  nsim::AdjointState* adj = nullptr;
+  std::vector<void*> allocations;
+
+  void track(void* p) {
+    if (p) allocations.push_back(p);
+  }
+};
+
+namespace nsim {
+
+// Shared by the C-ABI translation units.
+int device_rhs(nsim_handle* h, const double* m, double* dmdt);
+// Re-uploads the per-magnet coefficient table and the exchange diagonal kappa
+// from h->coefs.
+int refresh_coefficients(nsim_handle* h);
+
+}  // namespace nsim

@@ -35,6 +35,12 @@ class NsimOptions(ctypes.Structure):
                 ("nbc_maxits", ctypes.c_int), ("dbc_maxits", ctypes.c_int)]
 
 
+class NsimTraj(ctypes.Structure):
+    _fields_ = [("dt", ctypes.c_double), ("n_steps", ctypes.c_int),
+                ("checkpoint_every", ctypes.c_int), ("n_pulses", ctypes.c_int),
+                ("pulse_len", ctypes.c_double), ("pulse_dir", ctypes.c_double * 3)]
+
+
 # name -> (restype, argtypes); every symbol include/nsim.h declares
 SIGNATURES = {
     "nsim_create": (ctypes.c_int, [ctypes.POINTER(NsimMesh), ctypes.POINTER(NsimMaterial),
@@ -66,6 +72,11 @@ SIGNATURES = {
                                          ctypes.POINTER(NsimOptions), ctypes.POINTER(_vp)]),
     "nsim_get_batch_layout": (ctypes.c_int, [_vp, _ip, _ip]),
     "nsim_autotune": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "nsim_set_material": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(NsimMaterial)]),
+    "nsim_rhs_vjp": (ctypes.c_int, [_vp, ctypes.c_double, _dp, _dp, _dp, _dp, _dp, _dp]),
+    "nsim_traj_forward": (ctypes.c_int, [_vp, ctypes.POINTER(NsimTraj), _dp]),
+    "nsim_traj_adjoint": (ctypes.c_int, [_vp, ctypes.POINTER(NsimTraj), _dp, _dp, _dp, _dp,
+                                         _dp, _dp]),
 }
 
 _lib = None
@@ -256,6 +267,54 @@ class NativeMagnet:
         _check(lib().nsim_psolve(self._h, t, None, None, r.ctypes.data, z.ctypes.data, gamma, 0.0,
                                  lr, 0), "nsim_psolve")
         return z
+
+    def set_material(self, material, magnet=0):
+        mat = NsimMaterial(*[float(material[k]) for k in ("Ms", "negJ", "c1", "c2", "c3", "mu0")])
+        _check(lib().nsim_set_material(self._h, int(magnet), ctypes.byref(mat)),
+               "nsim_set_material")
+
+    def rhs_vjp(self, y, lam, t=0.0):
+        """J(y)^T lam and the parameter cotangents (K1 per node, Ms, H_ext)."""
+        y = np.ascontiguousarray(y, dtype=np.float64).reshape(-1)
+        lam = np.ascontiguousarray(lam, dtype=np.float64).reshape(-1)
+        yb = np.zeros_like(y)
+        k1b = np.zeros(self.n)
+        hb = np.zeros_like(y)
+        ms = ctypes.c_double(0.0)
+        _check(lib().nsim_rhs_vjp(self._h, t, _dptr(y), _dptr(lam), _dptr(yb), _dptr(k1b),
+                                  ctypes.byref(ms), _dptr(hb)), "nsim_rhs_vjp")
+        return {"y": yb, "K1": k1b, "Ms": ms.value, "H_ext": hb}
+
+    @staticmethod
+    def _traj(dt, n_steps, checkpoint_every, n_pulses, pulse_len, pulse_dir):
+        d = (ctypes.c_double * 3)(*[float(v) for v in pulse_dir])
+        return NsimTraj(float(dt), int(n_steps), int(checkpoint_every), int(n_pulses),
+                        float(pulse_len), d)
+
+    def traj_forward(self, dt, n_steps, checkpoint_every=50, amps=(), pulse_len=1.0,
+                     pulse_dir=(1.0, 0.0, 0.0)):
+        """Fixed-step RK4 from the current m with pulses amps[w] * pulse_dir in
+        window w; m becomes m(n_steps * dt)."""
+        a = np.ascontiguousarray(amps, dtype=np.float64).reshape(-1)
+        tr = self._traj(dt, n_steps, checkpoint_every, len(a), pulse_len, pulse_dir)
+        _check(lib().nsim_traj_forward(self._h, ctypes.byref(tr),
+                                       _dptr(a) if len(a) else None), "nsim_traj_forward")
+
+    def traj_adjoint(self, m_bar_final, dt, n_steps, checkpoint_every=50, amps=(),
+                     pulse_len=1.0, pulse_dir=(1.0, 0.0, 0.0)):
+        """Discrete adjoint of the last traj_forward: cotangents of m(0), K1 per
+        node, Ms and the pulse amplitudes for dLoss/dm(T) = m_bar_final."""
+        a = np.ascontiguousarray(amps, dtype=np.float64).reshape(-1)
+        tr = self._traj(dt, n_steps, checkpoint_every, len(a), pulse_len, pulse_dir)
+        mb = np.ascontiguousarray(m_bar_final, dtype=np.float64).reshape(-1)
+        m0b = np.zeros_like(mb)
+        k1b = np.zeros(self.n)
+        ms = ctypes.c_double(0.0)
+        ab = np.zeros(max(len(a), 1))
+        _check(lib().nsim_traj_adjoint(self._h, ctypes.byref(tr), _dptr(a) if len(a) else None,
+                                       _dptr(mb), _dptr(m0b), _dptr(k1b), ctypes.byref(ms),
+                                       _dptr(ab)), "nsim_traj_adjoint")
+        return {"m0": m0b.reshape(-1, 3), "K1": k1b, "Ms": ms.value, "amps": ab[:len(a)]}
 
     def ts_init(self, t0=0.0, rtol=1e-6, atol=1e-6, max_order=2, krylov_max=300):
         _check(lib().nsim_ts_init(self._h, t0, rtol, atol, max_order, krylov_max), "nsim_ts_init")

@@ -66,3 +66,20 @@ def test_unsupported_options_fail_loudly():
     with pytest.raises(NmagUserError):
         s.magnet
     assert every("step", 5).value == 5
+
+
+def test_element_to_node_projection_reduces_to_uniform_and_conserves_energy_weights():
+    """Per-element K -> nodal K1 (lumped): a uniform K maps to the same nodal
+    value, and sum_i V_i K1_i = sum_e vol_e K_e (volume integral kept)."""
+    import numpy as np
+    from nmag import kuhn_cuboid
+    from nmag.adjoint import element_to_node
+    coords, tets = kuhn_cuboid(4, 3, 2, 2.5)
+    P = element_to_node(coords, tets)
+    assert np.allclose(P @ np.full(len(tets), 3.7), 3.7, rtol=0, atol=1e-13)
+    x = coords[tets]
+    vol = np.abs(np.einsum("ij,ij->i", x[:, 1] - x[:, 0],
+                           np.cross(x[:, 2] - x[:, 0], x[:, 3] - x[:, 0]))) / 6
+    V = np.bincount(tets.reshape(-1), weights=np.repeat(vol / 4, 4), minlength=len(coords))
+    K = np.random.default_rng(1).uniform(0, 1, len(tets))
+    assert abs(V @ (P @ K) - vol @ K) <= 1e-12 * (vol @ K)
