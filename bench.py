@@ -72,14 +72,20 @@ def gemv_bytes(nb, esz):
     return esz * nb * nb + 16 * nb
 
 
-def run_gpu(prec, coords, tets, mat, Hext, states, warmup, steps, rtol, torch):
+KERNELS = {"dense": ("exch_div", "potential", "demag_llg"),
+           "iterative": ("exch_div", "cg_nbc", "bem", "cg_dbc", "demag_llg")}
+
+
+def run_gpu(prec, route, coords, tets, mat, Hext, states, warmup, steps, rtol, torch):
     from nmag._native import NativeMagnet
+    t0 = time.perf_counter()
     g = NativeMagnet(coords, tets, mat.su_coeffs(), precision=prec, nbc_rtol=rtol,
-                     dbc_rtol=rtol)
+                     dbc_rtol=rtol, potential=route)
+    t_setup = time.perf_counter() - t0
     g.set_field("H_ext", Hext)
     g.set_field("m", states[0])
     tuned = g.autotune()
-    log("%s autotuned: %s" % (prec, tuned))
+    log("%s/%s set-up %.2f s, autotuned: %s" % (prec, route, t_setup, tuned))
     n3 = 3 * g.n
     Y = torch.from_numpy(states.reshape(len(states), n3)).to("cuda")
     YD = torch.zeros_like(Y)
@@ -87,7 +93,8 @@ def run_gpu(prec, coords, tets, mat, Hext, states, warmup, steps, rtol, torch):
     for k in range(warmup):
         g.rhs_device(Y[k].data_ptr(), YD[k].data_ptr())
     g.synchronize()
-    p1, p2 = g.get_field("phi1"), g.get_field("phi2")
+    if route == "iterative":
+        p1, p2 = g.get_field("phi1"), g.get_field("phi2")
     it0 = (g.stat("cg_iters_nbc_total"), g.stat("cg_iters_dbc_total"))
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -98,15 +105,17 @@ def run_gpu(prec, coords, tets, mat, Hext, states, warmup, steps, rtol, torch):
     dt = time.perf_counter() - t0
     it1 = (g.stat("cg_iters_nbc_total"), g.stat("cg_iters_dbc_total"))
     assert g.stat("cg_error") == 0, "CG reported an error"
-    # second pass over the same states with per-kernel HIP-event timers
-    g.set_field("phi1", p1)
-    g.set_field("phi2", p2)
+    # second pass over the same states with per-kernel HIP-event timers (on
+    # the handle's stream, around each launch)
+    if route == "iterative":
+        g.set_field("phi1", p1)
+        g.set_field("phi2", p2)
     g.timers(True)
     for k in range(steps):
         g.rhs_device(Y[warmup + k].data_ptr(), YD[warmup + k].data_ptr())
     g.synchronize()
     kern = {}
-    for name in ("exch_div", "cg_nbc", "bem", "cg_dbc", "demag_llg"):
+    for name in KERNELS[route]:
         ms, cnt = g.timer(name)
         kern[name] = ms / max(cnt, 1)
     g.timers(False)
@@ -120,17 +129,20 @@ def run_gpu(prec, coords, tets, mat, Hext, states, warmup, steps, rtol, torch):
         "nb": g.nb, "n": g.n, "nnz": g.nnz,
         "cg_grid": g.stat("cg_grid"),
         "autotuned": tuned,
+        "setup_s": round(t_setup, 3),
+        "potential_bytes": g.stat("potential_bytes"),
+        "bem_bytes": g.stat("bem_bytes"),
     }
     g.close()
     return res, out
 
 
-def run_dynamics(prec, coords, tets, mat, Hext, m0, window_ps, checkpoint_ps, ksp_rtol):
+def run_dynamics(prec, route, coords, tets, mat, Hext, m0, window_ps, checkpoint_ps, ksp_rtol):
     """advance_time over window_ps from the S-state (BDF, rtol = atol = 1e-6);
     returns wall time, integrator statistics and <m>(t) at the checkpoints."""
     from nmag._native import NativeMagnet
     g = NativeMagnet(coords, tets, mat.su_coeffs(), precision=prec, nbc_rtol=ksp_rtol,
-                     dbc_rtol=ksp_rtol)
+                     dbc_rtol=ksp_rtol, potential=route)
     g.set_field("H_ext", Hext)
     g.set_field("m", m0)
     g.ts_init(0.0, 1e-6, 1e-6)
@@ -334,6 +346,79 @@ def run_batch(steps, warmup, rtol, window_ps, torch):
     return out
 
 
+def run_adjoint(t_ns, rtol, fd_checks=True):
+    """Differentiation form: permalloy slab 100 x 50 x 5 nm, 2.5 nm Kuhn mesh
+    (40 x 20 x 2 cubes, 2583 nodes), alpha 0.02, per-element K ~ N(2e4, 5e3)
+    J/m^3 along z, 16 piecewise-constant x-field pulses over t_ns ~ N(0, 5) mT,
+    loss <m_x>(t_ns); fixed-step RK4 at 0.2 ps, checkpoints every 50 steps."""
+    from nmag import MagMaterial, SI, kuhn_cuboid, to_su
+    from nmag._native import NativeMagnet
+    from nmag.adjoint import TrajectoryProblem
+    coords, tets = kuhn_cuboid(40, 20, 2, 2.5)
+    mat = MagMaterial("Py", Ms=SI(8.6e5, "A/m"), exchange_coupling=SI(1.3e-11, "J/m"),
+                      llg_gamma_G=SI(2.211e5, "m/A s"), llg_damping=0.02)
+    rng = np.random.default_rng(55)
+    n = len(coords)
+    K = rng.normal(2e4, 5e3, len(tets)) * float(to_su(SI(1.0, "J/m^3"), "J/m^3"))
+    mt = float(to_su(SI(1e-3 / (4e-7 * math.pi), "A/m"), "A/m"))
+    amps = rng.normal(0.0, 5.0, 16) * mt
+    m0 = np.tile([1.0, 1.0, 0.0], (n, 1)) / math.sqrt(2.0)
+    m0 += math.radians(1.0) * rng.standard_normal((n, 3))
+    m0 /= np.linalg.norm(m0, axis=1, keepdims=True)
+    dt = 0.2
+    n_steps = int(round(t_ns * 1000.0 / dt))
+    out = {"workload": "Py 100x50x5 nm, 2.5 nm Kuhn mesh, %d nodes, %d tets, %d pulses, "
+                       "RK4 dt=%.1f ps x %d steps, checkpoint every 50" % (
+                           n, len(tets), len(amps), dt, n_steps),
+           "ksp_rtol": rtol}
+    grads = {}
+    for prec in ("mixed", "fp64"):
+        g = NativeMagnet(coords, tets, mat.su_coeffs(), precision=prec, nbc_rtol=rtol,
+                         dbc_rtol=rtol)
+        prob = TrajectoryProblem(g, mat.su_coeffs(), coords, tets, m0,
+                                 np.tile([0.0, 0.0, 1.0], (n, 1)), dt=dt, n_steps=n_steps,
+                                 n_pulses=len(amps), pulse_len=dt * n_steps / len(amps),
+                                 checkpoint_every=50)
+        Ms = mat.su_coeffs()["Ms"]
+        t0 = time.perf_counter()
+        loss = prob.run(K, Ms, amps)
+        t_fwd = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        gK, gMs, gA, gm0 = prob.gradient(amps)
+        t_adj = time.perf_counter() - t0
+        grads[prec] = (gK, gMs, gA)
+        rhs = g.stat("rhs_evals")
+        out[prec] = {"loss": loss, "forward_s": round(t_fwd, 2), "adjoint_s": round(t_adj, 2),
+                     "adjoint_over_forward": round(t_adj / t_fwd, 2), "rhs_evals": int(rhs),
+                     "dLoss_dMs": gMs, "dLoss_damps_first4": [float(v) for v in gA[:4]],
+                     "cg_unconverged": int(g.stat("cg_unconverged"))}
+        log("adjoint %s: loss %.6f forward %.1f s adjoint %.1f s" % (prec, loss, t_fwd, t_adj))
+        if prec == "fp64" and fd_checks:
+            rel = 1e-3
+            h = rel * Ms
+            fd_ms = (prob.run(K, Ms + h, amps) - prob.run(K, Ms - h, amps)) / (2 * h)
+            k = int(np.argmax(np.abs(gA)))
+            ha = rel * np.max(np.abs(amps))
+            ap, am = amps.copy(), amps.copy()
+            ap[k] += ha
+            am[k] -= ha
+            fd_a = (prob.run(K, Ms, ap) - prob.run(K, Ms, am)) / (2 * ha)
+            out["fd_check_fp64"] = {
+                "rel_step": rel,
+                "Ms": {"adjoint": gMs, "fd": fd_ms, "rel_err": abs(gMs - fd_ms) / abs(fd_ms)},
+                "amp_%d" % k: {"adjoint": float(gA[k]), "fd": fd_a,
+                               "rel_err": abs(gA[k] - fd_a) / abs(fd_a)}}
+            log("fd check: %s" % out["fd_check_fp64"])
+        g.close()
+    rel = lambda a, b: float(np.max(np.abs(np.atleast_1d(a) - np.atleast_1d(b))) /
+                             np.max(np.abs(np.atleast_1d(b))))
+    out["mixed_vs_fp64_gradient_rel_err"] = {
+        "K_elem": rel(grads["mixed"][0], grads["fp64"][0]),
+        "Ms": rel(grads["mixed"][1], grads["fp64"][1]),
+        "amps": rel(grads["mixed"][2], grads["fp64"][2])}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -352,6 +437,9 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=20)
     ap.add_argument("--batch-window", type=float, default=2.0,
                     help="ps of relaxation timed for the batch (0 skips)")
+    ap.add_argument("--adjoint-ns", type=float, default=2.0,
+                    help="ns of the differentiation-form trajectory (0 skips)")
+    ap.add_argument("--adjoint-rtol", type=float, default=1e-8)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -375,18 +463,23 @@ def main():
     states = trajectory_states(gen, m0, total, args.dt)
     gen.close()
 
-    results = {}
-    outs = {}
-    for prec in ("fp64", "mixed"):
-        log("timing %s ..." % prec)
-        results[prec], outs[prec] = run_gpu(prec, coords, tets, mat, Hext, states, args.warmup,
-                                            args.steps, args.rtol, torch)
-        log("%s: %.1f evals/s, kernels(ms) %s, cg iters %.1f/%.1f" % (
-            prec, results[prec]["evals_per_s"], results[prec]["kernel_ms"],
-            results[prec]["cg_iters_nbc"], results[prec]["cg_iters_dbc"]))
-    err = float(np.max(np.abs(outs["mixed"] - outs["fp64"])) / np.max(np.abs(outs["fp64"])))
+    results, outs = {}, {}
+    for route in ("dense", "iterative"):
+        for prec in ("fp64", "mixed"):
+            key = prec + "/" + route
+            log("timing %s ..." % key)
+            results[key], outs[key] = run_gpu(prec, route, coords, tets, mat, Hext, states,
+                                              args.warmup, args.steps, args.rtol, torch)
+            log("%s: %.1f evals/s, kernels(ms) %s, cg iters %.1f/%.1f" % (
+                key, results[key]["evals_per_s"], results[key]["kernel_ms"],
+                results[key]["cg_iters_nbc"], results[key]["cg_iters_dbc"]))
 
-    mixed = results["mixed"]
+    def rel(a, b):
+        return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+
+    # headline: the dense potential route
+    mixed, fp64 = results["mixed/dense"], results["fp64/dense"]
+    err = rel(outs["mixed/dense"], outs["fp64/dense"])
     nb, n, nnz = mixed["nb"], mixed["n"], mixed["nnz"]
     kern = mixed["kernel_ms"]
     dominant = max(kern, key=kern.get)
@@ -397,37 +490,53 @@ def main():
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
                 "traffic": None, "alg_bytes_per_launch": int(alg_bytes)}
 
-    # per-iteration algorithmic bytes of one PCG iteration (SpMV over the
-    # stiffness pattern with FP64 values + 4-byte columns, row pointers, and
-    # about eight FP64 vector passes), times the measured iterations per solve
-    cg_iter_bytes = 12 * nnz + 4 * (n + 1) + 64 * n
+    # dense potential GEMV: the N x N operator once, rho in, phi out
+    pot_bytes = lambda esz: esz * n * n + 16 * n
     rooflines = {
-        "cg_nbc": roofline("k_cg<double> Neumann solve",
-                           cg_iter_bytes * max(mixed["cg_iters_nbc"], 1), kern["cg_nbc"]),
-        "cg_dbc": roofline("k_cg<double> Dirichlet solve",
-                           cg_iter_bytes * max(mixed["cg_iters_dbc"], 1), kern["cg_dbc"]),
-        "bem": roofline("k_bem_gemv<float>", gemv_bytes(nb, 4), kern["bem"]),
+        "potential": roofline("k_dense_gemv<float> potential operator (N=%d)" % n,
+                              pot_bytes(4), kern["potential"]),
     }
     roof = rooflines.get(dominant)
-    bem64 = results["fp64"]["kernel_ms"]["bem"]
-    bem_fp64_gbs = gemv_bytes(nb, 8) / (bem64 * 1e-3) / 1e9
+    pot64_gbs = pot_bytes(8) / (fp64["kernel_ms"]["potential"] * 1e-3) / 1e9
+    # iterative route (the reference's structure): per-iteration algorithmic
+    # bytes of one PCG iteration (SpMV over the stiffness pattern with FP64
+    # values + 4-byte columns, row pointers, about eight FP64 vector passes)
+    # times the measured iterations per solve; the BEM GEMV streams B once
+    it_m = results["mixed/iterative"]
+    kit = it_m["kernel_ms"]
+    cg_iter_bytes = 12 * nnz + 4 * (n + 1) + 64 * n
+    rooflines_iterative = {
+        "cg_nbc": roofline("k_cg<double> Neumann solve",
+                           cg_iter_bytes * max(it_m["cg_iters_nbc"], 1), kit["cg_nbc"]),
+        "cg_dbc": roofline("k_cg<double> Dirichlet solve",
+                           cg_iter_bytes * max(it_m["cg_iters_dbc"], 1), kit["cg_dbc"]),
+        "bem": roofline("k_dense_gemv<float> BEM", gemv_bytes(nb, 4), kit["bem"]),
+    }
 
     dyn = None
     if not args.no_dynamics:
         dyn = {}
-        avgs = {}
-        for prec in ("fp64", "mixed"):
-            wall, stats, avgs[prec] = run_dynamics(prec, coords, tets, mat, Hext, m0,
-                                                   args.sim_window, 5.0, args.rtol)
-            dyn[prec] = {"sim_ns_per_hour": round(args.sim_window * 1e-3 / (wall / 3600.0), 3),
-                         "wall_s": round(wall, 3), **{k: int(v) for k, v in stats.items()}}
+        for route in ("dense", "iterative"):
+            dyn[route] = {}
+            avgs = {}
+            for prec in ("fp64", "mixed"):
+                wall, stats, avgs[prec] = run_dynamics(prec, route, coords, tets, mat, Hext, m0,
+                                                       args.sim_window, 5.0, args.rtol)
+                dyn[route][prec] = {
+                    "sim_ns_per_hour": round(args.sim_window * 1e-3 / (wall / 3600.0), 3),
+                    "wall_s": round(wall, 3), **{k: int(v) for k, v in stats.items()}}
+            dyn[route]["mean_m_max_dev_mixed_vs_fp64"] = float(
+                np.abs(avgs["mixed"] - avgs["fp64"]).max())
         dyn["window_ps"] = args.sim_window
-        dyn["mean_m_max_dev_mixed_vs_fp64"] = float(np.abs(avgs["mixed"] - avgs["fp64"]).max())
         dyn["integrator"] = "BDF order<=2, Newton-GMRES, rtol=atol=1e-6, from the seeded S-state"
 
     batch = None
     if not args.no_batch:
         batch = run_batch(args.batch_steps, 3, args.rtol, args.batch_window, torch)
+
+    adjoint = None
+    if args.adjoint_ns > 0:
+        adjoint = run_adjoint(args.adjoint_ns, args.adjoint_rtol)
 
     cpu = None
     if not args.no_cpu:
@@ -452,28 +561,40 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32-mixed (FP32 SpMV/BEM, FP64 dots/state)",
+        "dtype": "fp32-mixed (FP32 potential operator and exchange/gradient SpMV, FP64 "
+                 "accumulation, divergence, dots and state)",
         "data": "synthetic (seeded S-state trajectory, explicit-Euler states 0.1 ps apart)",
         "config": {"workload": "Py slab 160x80x10 nm, 2 nm Kuhn mesh, 19926 nodes, "
-                               "Nb=%d dense BEM, alpha=0.02, 25 mT @170deg" % nb,
-                   "ksp_rtol": args.rtol},
-        "fp64": {"value": round(results["fp64"]["evals_per_s"], 2),
-                 "ms_per_step": round(results["fp64"]["ms_per_eval"], 4),
-                 "kernel_ms": results["fp64"]["kernel_ms"],
-                 "cg_iters": [results["fp64"]["cg_iters_nbc"], results["fp64"]["cg_iters_dbc"]],
-                 "bem_gemv_GBs": round(bem_fp64_gbs, 1),
-                 "autotuned": results["fp64"]["autotuned"]},
+                               "Nb=%d, alpha=0.02, 25 mT @170deg" % nb,
+                   "demag": "dense potential operator M=(I+EBR)K^+ (%.2f GB FP32), built in "
+                            "%.1f s" % (mixed["potential_bytes"] / 1e9, mixed["setup_s"]),
+                   "ksp_rtol_iterative_route": args.rtol},
+        "fp64": {"value": round(fp64["evals_per_s"], 2),
+                 "ms_per_step": round(fp64["ms_per_eval"], 4),
+                 "kernel_ms": fp64["kernel_ms"],
+                 "potential_gemv_GBs": round(pot64_gbs, 1),
+                 "setup_s": fp64["setup_s"],
+                 "autotuned": fp64["autotuned"]},
         "mixed_autotuned": mixed["autotuned"],
         "mixed_kernel_ms": kern,
-        "mixed_cg_iters": [mixed["cg_iters_nbc"], mixed["cg_iters_dbc"]],
-        "mixed_vs_fp64_speedup": round(mixed["evals_per_s"] / results["fp64"]["evals_per_s"], 3),
+        "mixed_vs_fp64_speedup": round(mixed["evals_per_s"] / fp64["evals_per_s"], 3),
         "mixed_dmdt_max_rel_err": err,
+        "dense_vs_iterative_fp64_dmdt_rel_diff": rel(outs["fp64/dense"], outs["fp64/iterative"]),
+        "iterative_route": {
+            key.split("/")[0]: {"value": round(results[key]["evals_per_s"], 2),
+                                "kernel_ms": results[key]["kernel_ms"],
+                                "cg_iters": [results[key]["cg_iters_nbc"],
+                                             results[key]["cg_iters_dbc"]],
+                                "autotuned": results[key]["autotuned"]}
+            for key in ("fp64/iterative", "mixed/iterative")},
         "dominant_kernel": dominant,
         "roofline": roof,
         "roofline_all": rooflines,
+        "roofline_iterative_route": rooflines_iterative,
         "cpu_baseline": cpu,
         "dynamics": dyn,
         "batch": batch,
+        "differentiation": adjoint,
     }
     print(json.dumps(line))
 

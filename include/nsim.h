@@ -55,6 +55,15 @@ typedef struct {
   double dbc_rtol; /* Dirichlet solve relative tolerance                   */
   int nbc_maxits;
   int dbc_maxits;
+  /* Demag route.  -1: the reference's route, two Poisson solves (persistent
+   * CG, warm-started, to nbc_rtol / dbc_rtol) around the BEM GEMV.  1: a
+   * dense potential operator M = (I + E B R) K^+ per magnet, built once at
+   * create time (Cholesky factorisations), so that phi = M rho is one GEMV
+   * streamed from HBM; exact up to rounding, the tolerances are unused.
+   * 0: 1 when every magnet has at most 20480 nodes (40960 with BF16
+   * storage) and all operators fit 32 GiB, else -1; nsim_get_stat
+   * "potential" tells which route a handle took. */
+  int potential;
 } nsim_options;
 
 /* Replaces raw_make_lam (src/nsimexec.ml:3605 -> src/nsim.ml:2828): builds
@@ -149,6 +158,12 @@ int nsim_execute(nsim_handle* h, const char* program);
 /* Dense BEM matrix (N_b x N_b, row-major, FP64 copy) for inspection; for a
  * batch, the magnets' matrices one after another (each N_b,k x N_b,k). */
 int nsim_get_bem(nsim_handle* h, double* host_out);
+
+/* Dense potential operator M = (I + E B R) K^+ of each magnet (N_k x N_k,
+ * row-major, FP64 copy, one after another) for inspection: phi = M rho.  Fails
+ * on a handle that took the iterative route.  No counterpart in the
+ * reference, which only ever applies K^-1 through KSP solves. */
+int nsim_get_potential(nsim_handle* h, double* host_out);
 
 /* CVODE rhs callback (src/sundials_sp_stubs.c:264, src/nsim.ml:2273-2313):
  * ydot = f(t, y).  y and ydot are 3N arrays; device_ptrs != 0 means they are

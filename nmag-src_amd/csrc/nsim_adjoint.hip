@@ -259,15 +259,15 @@ __global__ __launch_bounds__(256) void k_adj_grad(int n, const int* __restrict__
 }
 
 template <typename TB>
-hipError_t gemv_t_typed(const DeviceOps& d, const double* e, double* part, double* chi,
+hipError_t gemv_t_typed(const DenseOp& op, const double* e, double* part, double* chi,
                         hipStream_t st) {
   constexpr int V = TVec<TB>::V;
-  const int ldb = d.h_seg_ldb[0], nb = d.nb;
-  const int chunks = bem_t_chunks(d);
+  const int ldb = op.h_ld[0], nb = op.h_n[0];
+  const int chunks = dense_t_chunks(op);
   const int rpc = (nb + chunks - 1) / chunks;
   const int cols = (ldb / V + 255) / 256;
-  k_bem_gemv_t<TB><<<dim3(cols, chunks), 256, 0, st>>>(nb, ldb, rpc, (const TB*)d.B, e, part);
-  k_bem_t_reduce<<<blocks_for(nb), 256, 0, st>>>(nb, ldb, chunks, part, d.bnode, chi);
+  k_bem_gemv_t<TB><<<dim3(cols, chunks), 256, 0, st>>>(nb, ldb, rpc, (const TB*)op.A, e, part);
+  k_bem_t_reduce<<<blocks_for(nb), 256, 0, st>>>(nb, ldb, chunks, part, op.map, chi);
   return hipGetLastError();
 }
 
@@ -301,22 +301,22 @@ hipError_t launch_adj_boundary(const DeviceOps& d, const double* psi, const doub
   return hipGetLastError();
 }
 
-int bem_t_chunks(const DeviceOps& d) {
-  if (d.nb == 0 || d.h_seg_ldb.empty()) return 1;
+int dense_t_chunks(const DenseOp& op) {
+  if (op.h_n.empty() || op.h_n[0] == 0) return 1;
   // about 2048 workgroups in all, each chunk at least 8 rows
-  const int cols = (d.h_seg_ldb[0] / 2 + 255) / 256;
+  const int cols = (op.h_ld[0] / 2 + 255) / 256;
   int chunks = std::max(1, 2048 / std::max(1, cols));
-  chunks = std::min(chunks, std::max(1, d.nb / 8));
+  chunks = std::min(chunks, std::max(1, op.h_n[0] / 8));
   return chunks;
 }
 
-hipError_t launch_bem_gemv_t(const DeviceOps& d, const double* e, double* part, double* chi,
-                             hipStream_t st) {
-  if (d.nb == 0) return hipSuccess;
-  if (d.h_seg_nb.size() != 1) return hipErrorInvalidConfiguration;
-  if (d.btype == kBemF64) return gemv_t_typed<double>(d, e, part, chi, st);
-  if (d.btype == kBemF32) return gemv_t_typed<float>(d, e, part, chi, st);
-  return gemv_t_typed<uint16_t>(d, e, part, chi, st);
+hipError_t launch_dense_gemv_t(const DenseOp& op, const double* e, double* part, double* chi,
+                               hipStream_t st) {
+  if (!op.A || op.h_n.empty() || op.h_n[0] == 0) return hipSuccess;
+  if (op.h_n.size() != 1) return hipErrorInvalidConfiguration;
+  if (op.type == kBemF64) return gemv_t_typed<double>(op, e, part, chi, st);
+  if (op.type == kBemF32) return gemv_t_typed<float>(op, e, part, chi, st);
+  return gemv_t_typed<uint16_t>(op, e, part, chi, st);
 }
 
 hipError_t launch_mean_project(int n, double* x, hipStream_t st) {

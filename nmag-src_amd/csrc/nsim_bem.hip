@@ -76,28 +76,6 @@ __device__ double lindholm_weight(const double* o, const double* p0, const doubl
   return res;
 }
 
-__device__ inline uint16_t f32_to_bf16(float f) {
-  uint32_t u = __float_as_uint(f);
-  if ((u & 0x7f800000u) == 0x7f800000u && (u & 0x7fffffu)) return 0x7fc0;
-  u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even
-  return (uint16_t)(u >> 16);
-}
-
-template <typename TB>
-__device__ inline void store_b(TB* p, double v);
-template <>
-__device__ inline void store_b<double>(double* p, double v) {
-  *p = v;
-}
-template <>
-__device__ inline void store_b<float>(float* p, double v) {
-  *p = (float)v;
-}
-template <>
-__device__ inline void store_b<uint16_t>(uint16_t* p, double v) {
-  *p = f32_to_bf16((float)v);
-}
-
 // One thread per matrix entry (i, j): B_ij = sum over the triangles incident to
 // boundary node j of j's Lindholm weight seen from node i, plus the diagonal
 // Omega_i/(4 pi) - 1 (src/bem3d.ml:938-1046).  Deterministic; writes coalesced.
@@ -155,17 +133,24 @@ struct BemVec<uint16_t> {
   using A = float;
 };
 
+// 16-byte row chunks of a dense operator.  The operator is streamed once per
+// launch with no reuse, so the loads are non-temporal: on MI355X this lifts a
+// plain streaming read from about 6.4 to 7.1 TB/s, and the GEMV with it.
+typedef double d2v __attribute__((ext_vector_type(2)));
+typedef float f4v __attribute__((ext_vector_type(4)));
+typedef unsigned int u4v __attribute__((ext_vector_type(4)));
+
 template <typename TB, typename X>
 __device__ inline void load_row_chunk(const TB* p, X* out);
 template <>
 __device__ inline void load_row_chunk<double, double>(const double* p, double* out) {
-  const double2 v = *reinterpret_cast<const double2*>(p);
+  const d2v v = __builtin_nontemporal_load(reinterpret_cast<const d2v*>(p));
   out[0] = v.x;
   out[1] = v.y;
 }
 template <>
 __device__ inline void load_row_chunk<float, double>(const float* p, double* out) {
-  const float4 v = *reinterpret_cast<const float4*>(p);
+  const f4v v = __builtin_nontemporal_load(reinterpret_cast<const f4v*>(p));
   out[0] = v.x;
   out[1] = v.y;
   out[2] = v.z;
@@ -173,7 +158,7 @@ __device__ inline void load_row_chunk<float, double>(const float* p, double* out
 }
 template <>
 __device__ inline void load_row_chunk<uint16_t, float>(const uint16_t* p, float* out) {
-  const uint4 v = *reinterpret_cast<const uint4*>(p);
+  const u4v v = __builtin_nontemporal_load(reinterpret_cast<const u4v*>(p));
   const uint32_t w[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
   for (int k = 0; k < 4; k++) {
@@ -182,103 +167,132 @@ __device__ inline void load_row_chunk<uint16_t, float>(const uint16_t* p, float*
   }
 }
 
-// phi2[bnode[i]] = sum_j B_ij phi1b_j over a block-diagonal set of dense
-// matrices (one per independent magnet of a batch; one for a single magnet).
-// A table maps every workgroup to a segment and a run of its rows.  The
-// segment's input vector is staged once per workgroup in LDS; each wavefront
-// streams ROWS rows of B at a time with 16-byte loads per lane and reduces with
-// cross-lane adds.
-template <typename TB, int ROWS>
-__global__ __launch_bounds__(256) void k_bem_gemv(const TB* __restrict__ Bbase,
-                                                  const long long* __restrict__ seg_Boff,
-                                                  const int* __restrict__ seg_ldb,
-                                                  const int* __restrict__ seg_nb,
-                                                  const int* __restrict__ seg_boff,
-                                                  const int* __restrict__ wg_seg,
-                                                  const int* __restrict__ wg_row0,
-                                                  int rows_per_wave,
-                                                  const double* __restrict__ xball,
-                                                  const int* __restrict__ bnodeall,
-                                                  double* __restrict__ y) {
-  using X = typename BemVec<TB>::X;
+// NR consecutive rows rb.. of a row-major segment (stride ldb) against the
+// staged x: y[map[rb + k]] = row . x.  Every lane loads 16 bytes of each row
+// per step; the partial sums are reduced across the wavefront.
+template <typename TB, int NR, typename X>
+__device__ inline void gemv_rows(const TB* __restrict__ B, int ldb, int rb, const X* xs,
+                                 int lane, const int* __restrict__ map, double* __restrict__ y) {
   using A = typename BemVec<TB>::A;
   constexpr int V = BemVec<TB>::V;
+  const TB* rp[NR];
+  A acc[NR];
+#pragma unroll
+  for (int k = 0; k < NR; k++) {
+    rp[k] = B + (size_t)(rb + k) * ldb;
+    acc[k] = 0;
+  }
+#pragma unroll 4
+  for (int c = lane * V; c < ldb; c += 64 * V) {
+    X xv[V];
+#pragma unroll
+    for (int e = 0; e < V; e++) xv[e] = xs[c + e];
+#pragma unroll
+    for (int k = 0; k < NR; k++) {
+      X bv[V];
+      load_row_chunk<TB, X>(rp[k] + c, bv);
+#pragma unroll
+      for (int e = 0; e < V; e++) acc[k] += (A)bv[e] * (A)xv[e];
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < NR; k++) {
+    const double t = wave_sum((double)acc[k]);
+    if (lane == 0) y[map[rb + k]] = t;
+  }
+}
+
+// y[map[off + r]] = sum_j A_rj x[off + j] over a block-diagonal set of dense
+// matrices (one per independent magnet of a batch; one for a single magnet):
+// the BEM GEMV phi2|_b = B phi1|_b and the dense potential phi = M rho.
+// A table maps every workgroup to a segment and a run of its rows.  The
+// segment's input vector is staged once per workgroup in LDS (up to 160 KiB
+// on gfx950); each wavefront streams ROWS rows at a time with 16-byte loads
+// per lane, then its remaining rows one at a time, so a wavefront's row count
+// need not be a multiple of ROWS (no row is read twice).
+template <typename TB, int ROWS, int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void k_dense_gemv(
+    const TB* __restrict__ Abase, const long long* __restrict__ seg_aoff,
+    const int* __restrict__ seg_ld, const int* __restrict__ seg_n,
+    const int* __restrict__ seg_off, const int* __restrict__ wg_seg,
+    const int* __restrict__ wg_row0, int rows_per_wave, const double* __restrict__ xall,
+    const int* __restrict__ mapall, double* __restrict__ y) {
+  using X = typename BemVec<TB>::X;
+  constexpr int T = 64 * WAVES;
   extern __shared__ __align__(16) unsigned char smem[];
   X* xs = reinterpret_cast<X*>(smem);
   const int sg = wg_seg[blockIdx.x];
-  const int ldb = seg_ldb[sg], nb = seg_nb[sg], boff = seg_boff[sg];
-  const TB* B = Bbase + seg_Boff[sg];
-  const double* xb = xball + boff;
-  const int* bnode = bnodeall + boff;
-  for (int j = threadIdx.x; j < ldb; j += 256) xs[j] = j < nb ? (X)xb[j] : (X)0;
+  const int ldb = seg_ld[sg], nb = seg_n[sg], boff = seg_off[sg];
+  const TB* B = Abase + seg_aoff[sg];
+  const double* xb = xall + boff;
+  const int* map = mapall + boff;
+  for (int j = threadIdx.x; j < ldb; j += T) xs[j] = j < nb ? (X)xb[j] : (X)0;
   __syncthreads();
   const int lane = threadIdx.x & 63;
   const int r0 = wg_row0[blockIdx.x] + (threadIdx.x >> 6) * rows_per_wave;
   const int r1 = min(r0 + rows_per_wave, nb);
-  for (int rb = r0; rb < r1; rb += ROWS) {
-    const TB* rp[ROWS];
-    A acc[ROWS];
-#pragma unroll
-    for (int k = 0; k < ROWS; k++) {
-      rp[k] = B + (size_t)min(rb + k, nb - 1) * ldb;
-      acc[k] = 0;
-    }
-#pragma unroll 4
-    for (int c = lane * V; c < ldb; c += 64 * V) {
-      X xv[V];
-#pragma unroll
-      for (int e = 0; e < V; e++) xv[e] = xs[c + e];
-#pragma unroll
-      for (int k = 0; k < ROWS; k++) {
-        X bv[V];
-        load_row_chunk<TB, X>(rp[k] + c, bv);
-#pragma unroll
-        for (int e = 0; e < V; e++) acc[k] += (A)bv[e] * (A)xv[e];
-      }
-    }
-#pragma unroll
-    for (int k = 0; k < ROWS; k++) {
-      const double t = wave_sum((double)acc[k]);
-      if (lane == 0 && rb + k < r1) y[bnode[rb + k]] = t;
-    }
+  int rb = r0;
+  for (; rb + ROWS <= r1; rb += ROWS) gemv_rows<TB, ROWS, X>(B, ldb, rb, xs, lane, map, y);
+  for (; rb < r1; rb++) gemv_rows<TB, 1, X>(B, ldb, rb, xs, lane, map, y);
+}
+
+constexpr size_t kMaxLds = 160 * 1024;  // LDS per workgroup on gfx950
+
+template <typename TB, int ROWS, int WAVES>
+hipError_t launch_shape(const DenseOp& op, size_t lds, const double* x, double* y,
+                        hipStream_t st) {
+  const void* k = reinterpret_cast<const void*>(&k_dense_gemv<TB, ROWS, WAVES>);
+  if (lds > 64 * 1024) {
+    const hipError_t e =
+        hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
   }
+  k_dense_gemv<TB, ROWS, WAVES><<<op.grid, 64 * WAVES, lds, st>>>(
+      (const TB*)op.A, op.aoff, op.ld, op.n, op.off, op.wg_seg, op.wg_row0, op.rpw, x, op.map,
+      y);
+  return hipGetLastError();
 }
 
 template <typename TB, int ROWS>
-void launch_rows(const DeviceOps& d, size_t lds, const double* phi1b, double* phi2,
-                 hipStream_t st) {
-  k_bem_gemv<TB, ROWS><<<d.gemv_grid, 256, lds, st>>>(
-      (const TB*)d.B, d.seg_Boff, d.seg_ldb, d.seg_nb, d.seg_boff, d.wg_seg, d.wg_row0,
-      d.gemv_rpw, phi1b, d.bnode, phi2);
+hipError_t launch_rows(const DenseOp& op, size_t lds, const double* x, double* y,
+                       hipStream_t st) {
+  if (op.waves == 4) return launch_shape<TB, ROWS, 4>(op, lds, x, y, st);
+  if (op.waves == 16) return launch_shape<TB, ROWS, 16>(op, lds, x, y, st);
+  return hipErrorInvalidConfiguration;
 }
 
 template <typename TB>
-hipError_t gemv_typed(const DeviceOps& d, const double* phi1b, double* phi2, hipStream_t st) {
+hipError_t gemv_typed(const DenseOp& op, const double* x, double* y, hipStream_t st) {
   using X = typename BemVec<TB>::X;
-  const size_t lds = (size_t)d.max_ldb * sizeof(X);
-  if (lds > 64 * 1024) return hipErrorInvalidValue;
-  if (d.gemv_rpw % d.gemv_rows != 0) return hipErrorInvalidConfiguration;
-  switch (d.gemv_rows) {
-    case 1: launch_rows<TB, 1>(d, lds, phi1b, phi2, st); break;
-    case 2: launch_rows<TB, 2>(d, lds, phi1b, phi2, st); break;
-    case 4: launch_rows<TB, 4>(d, lds, phi1b, phi2, st); break;
+  const size_t lds = (size_t)op.max_ld * sizeof(X);
+  if (lds > kMaxLds) return hipErrorInvalidValue;
+  switch (op.rows) {
+    case 1: return launch_rows<TB, 1>(op, lds, x, y, st);
+    case 2: return launch_rows<TB, 2>(op, lds, x, y, st);
+    case 4: return launch_rows<TB, 4>(op, lds, x, y, st);
     default: return hipErrorInvalidConfiguration;
   }
+}
+
+template <typename TB>
+hipError_t assemble_segment(const DeviceOps& d, int s, TB* out, int ld, hipStream_t st) {
+  const DenseOp& b = d.bem;
+  const int nb = b.h_n[s], bo = b.h_off[s];
+  const int cb = (ld + 255) / 256;
+  const long long nblk = (long long)cb * nb;
+  if (nblk <= 0) return hipSuccess;
+  if (nblk > 0x7fffffffLL) return hipErrorInvalidValue;
+  k_bem_assemble<TB><<<(int)nblk, 256, 0, st>>>(nb, ld, cb, d.coords, d.bnode + bo, d.tri_node,
+                                                d.tnorm, d.inc_rp + bo, d.inc_tri, d.omega + bo,
+                                                d.h_bem_geo_eps[s], out);
   return hipGetLastError();
 }
 
 template <typename TB>
 hipError_t assemble_typed(const DeviceOps& d, hipStream_t st) {
-  for (size_t s = 0; s < d.h_seg_nb.size(); s++) {
-    const int nb = d.h_seg_nb[s], ldb = d.h_seg_ldb[s], bo = d.h_seg_boff[s];
-    const int cb = (ldb + 255) / 256;
-    const long long nblk = (long long)cb * nb;
-    if (nblk <= 0) continue;
-    if (nblk > 0x7fffffffLL) return hipErrorInvalidValue;
-    k_bem_assemble<TB><<<(int)nblk, 256, 0, st>>>(
-        nb, ldb, cb, d.coords, d.bnode + bo, d.tri_node, d.tnorm, d.inc_rp + bo, d.inc_tri,
-        d.omega + bo, d.h_seg_geo_eps[s], (TB*)d.B + d.h_seg_Boff[s]);
-    const hipError_t e = hipGetLastError();
+  for (size_t s = 0; s < d.bem.h_n.size(); s++) {
+    const hipError_t e = assemble_segment<TB>(d, (int)s, (TB*)d.bem.A + d.bem.h_aoff[s],
+                                              d.bem.h_ld[s], st);
     if (e != hipSuccess) return e;
   }
   return hipSuccess;
@@ -286,50 +300,57 @@ hipError_t assemble_typed(const DeviceOps& d, hipStream_t st) {
 
 }  // namespace
 
-bool plan_bem_gemv(DeviceOps& d, int num_cus, int rows_per_pass, int wg_per_cu,
-                   std::vector<int>* wg_seg, std::vector<int>* wg_row0) {
+int dense_gemv_max_order(int type) {
+  const size_t x = type == kBemBF16 ? sizeof(float) : sizeof(double);
+  return (int)(kMaxLds / x);
+}
+
+bool plan_dense_gemv(DenseOp& op, int num_cus, int rows_per_pass, int wg_per_cu, int waves,
+                     std::vector<int>* wg_seg, std::vector<int>* wg_row0) {
   if (rows_per_pass != 1 && rows_per_pass != 2 && rows_per_pass != 4) return false;
   if (wg_per_cu < 1 || wg_per_cu > 8) return false;
-  // wg_per_cu workgroups of four wavefronts per CU; every wavefront gets the
-  // same number of rows (a multiple of rows_per_pass), every workgroup one
-  // segment
+  if (waves != 4 && waves != 16) return false;
+  // wg_per_cu workgroups of `waves` wavefronts per CU; every wavefront gets
+  // the same number of rows, every workgroup one segment
   long long rows = 0;
-  for (int nb : d.h_seg_nb) rows += nb;
-  const long long waves = 4LL * wg_per_cu * (num_cus > 0 ? num_cus : 256);
+  for (int nb : op.h_n) rows += nb;
+  const long long nw = (long long)waves * wg_per_cu * (num_cus > 0 ? num_cus : 256);
   const int R = rows_per_pass;
-  int rpw = (int)((rows + waves - 1) / waves);
-  rpw = std::max(R, ((rpw + R - 1) / R) * R);
-  const int rows_per_wg = 4 * rpw;
+  const int rpw = std::max(1, (int)((rows + nw - 1) / nw));
+  const int rows_per_wg = waves * rpw;
   wg_seg->clear();
   wg_row0->clear();
-  d.max_ldb = 0;
-  for (size_t s = 0; s < d.h_seg_nb.size(); s++) {
-    d.max_ldb = std::max(d.max_ldb, d.h_seg_ldb[s]);
-    for (int r = 0; r < d.h_seg_nb[s]; r += rows_per_wg) {
+  op.max_ld = 0;
+  for (size_t s = 0; s < op.h_n.size(); s++) {
+    op.max_ld = std::max(op.max_ld, op.h_ld[s]);
+    for (int r = 0; r < op.h_n[s]; r += rows_per_wg) {
       wg_seg->push_back((int)s);
       wg_row0->push_back(r);
     }
   }
-  d.gemv_rpw = rpw;
-  d.gemv_rows = R;
-  d.gemv_wg_per_cu = wg_per_cu;
-  d.gemv_grid = (int)wg_seg->size();
+  op.rpw = rpw;
+  op.rows = R;
+  op.wg_per_cu = wg_per_cu;
+  op.waves = waves;
+  op.grid = (int)wg_seg->size();
   return true;
 }
 
 hipError_t launch_bem_assemble(const DeviceOps& d, hipStream_t st) {
-  if (d.btype == kBemF64) return assemble_typed<double>(d, st);
-  if (d.btype == kBemF32) return assemble_typed<float>(d, st);
+  if (d.bem.type == kBemF64) return assemble_typed<double>(d, st);
+  if (d.bem.type == kBemF32) return assemble_typed<float>(d, st);
   return assemble_typed<uint16_t>(d, st);
 }
 
-hipError_t launch_bem_gemv(const DeviceOps& d, const double* phi1b, double* phi2,
-                           hipStream_t st, int num_cus) {
-  (void)num_cus;
-  if (d.nb == 0 || d.gemv_grid == 0) return hipSuccess;
-  if (d.btype == kBemF64) return gemv_typed<double>(d, phi1b, phi2, st);
-  if (d.btype == kBemF32) return gemv_typed<float>(d, phi1b, phi2, st);
-  return gemv_typed<uint16_t>(d, phi1b, phi2, st);
+hipError_t launch_bem_assemble_f64(const DeviceOps& d, int s, double* out, hipStream_t st) {
+  return assemble_segment<double>(d, s, out, d.bem.h_ld[s], st);
+}
+
+hipError_t launch_dense_gemv(const DenseOp& op, const double* x, double* y, hipStream_t st) {
+  if (!op.A || op.grid == 0) return hipSuccess;
+  if (op.type == kBemF64) return gemv_typed<double>(op, x, y, st);
+  if (op.type == kBemF32) return gemv_typed<float>(op, x, y, st);
+  return gemv_typed<uint16_t>(op, x, y, st);
 }
 
 }  // namespace nsim

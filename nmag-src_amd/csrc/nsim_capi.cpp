@@ -89,28 +89,29 @@ void flush_timers(nsim_handle* h) {
   }
 }
 
-// (Re)plans the grouped BEM GEMV for a tile shape and uploads its
-// workgroup table.
-int setup_gemv(nsim_handle* h, int rows_per_pass, int wg_per_cu) {
-  if (h->d.nb == 0) return 0;
-  DeviceOps trial = h->d;
+// (Re)plans a grouped dense GEMV (the BEM or the potential operator) for a
+// tile shape and uploads its workgroup table.
+int setup_gemv(nsim_handle* h, DenseOp& op, int rows_per_pass, int wg_per_cu, int waves) {
+  if (op.h_n.empty()) return 0;
+  DenseOp trial = op;
   std::vector<int> wg_seg, wg_row0;
-  if (!plan_bem_gemv(trial, h->num_cus, rows_per_pass, wg_per_cu, &wg_seg, &wg_row0))
-    return fail("BEM GEMV tile shape out of range (rows 1/2/4, workgroups per CU 1..8)");
+  if (!plan_dense_gemv(trial, h->num_cus, rows_per_pass, wg_per_cu, waves, &wg_seg, &wg_row0))
+    return fail("GEMV tile shape out of range (rows 1/2/4, workgroups per CU 1..8, "
+                "wavefronts per workgroup 4/16)");
   int *ds = nullptr, *dr = nullptr;
   hipError_t e = upload(&ds, wg_seg.data(), wg_seg.size());
   if (e == hipSuccess) e = upload(&dr, wg_row0.data(), wg_row0.size());
   if (e != hipSuccess) {
     (void)hipFree(ds);
     (void)hipFree(dr);
-    return fail(std::string("BEM GEMV plan upload: ") + hipGetErrorString(e));
+    return fail(std::string("GEMV plan upload: ") + hipGetErrorString(e));
   }
   (void)hipStreamSynchronize(h->stream);
-  (void)hipFree(h->d.wg_seg);
-  (void)hipFree(h->d.wg_row0);
+  (void)hipFree(op.wg_seg);
+  (void)hipFree(op.wg_row0);
   trial.wg_seg = ds;
   trial.wg_row0 = dr;
-  h->d = trial;
+  op = trial;
   return 0;
 }
 
@@ -171,7 +172,11 @@ int device_rhs(nsim_handle* h, const double* m, double* dmdt) {
     Span s(h, "exch_div");
     HIPCHK(launch_exch_div(h->d, h->mixed, m, h->Hexch, h->rho, demag, h->stream));
   }
-  if (demag) {
+  if (demag && h->d.pot.A) {
+    // phi = M rho with the precomputed potential operator M = (I + E B R) K^+
+    Span s(h, "potential");
+    HIPCHK(launch_dense_gemv(h->d.pot, h->rho, h->phi, h->stream));
+  } else if (demag) {
     CgWork& w = h->cgw;
     {
       Span s(h, "cg_nbc");
@@ -182,7 +187,7 @@ int device_rhs(nsim_handle* h, const double* m, double* dmdt) {
     }
     {
       Span s(h, "bem");
-      HIPCHK(launch_bem_gemv(h->d, h->phi1b, h->phi2, h->stream, h->num_cus));
+      HIPCHK(launch_dense_gemv(h->d.bem, h->phi1b, h->phi2, h->stream));
     }
     {
       Span s(h, "cg_dbc");
@@ -302,6 +307,7 @@ int create_impl(const char* who, int ninst, const nsim_mesh* meshes,
     h->opt.do_demag = 1;
     h->opt.nbc_rtol = h->opt.dbc_rtol = 1e-5;
     h->opt.nbc_maxits = h->opt.dbc_maxits = 10000;
+    h->opt.potential = 0;
   }
   if (h->opt.precision < NSIM_FP64 || h->opt.precision > NSIM_MIXED_BF16)
     return cleanup(fail(me + ": unknown precision"));
@@ -328,32 +334,54 @@ int create_impl(const char* who, int ninst, const nsim_mesh* meshes,
   d.ninst = ninst;
   for (int sl = 0; sl < o.nslices; sl++)
     d.max_width = std::max(d.max_width, (o.sell_off[sl + 1] - o.sell_off[sl]) / kSlice);
-  d.btype = h->opt.precision == NSIM_FP64 ? kBemF64
-            : h->opt.precision == NSIM_MIXED_FP32 ? kBemF32 : kBemBF16;
+  const int stype = h->opt.precision == NSIM_FP64         ? kBemF64
+                    : h->opt.precision == NSIM_MIXED_FP32 ? kBemF32
+                                                          : kBemBF16;
+  d.bem.type = stype;
   // BEM segments: the boundary nodes are in ascending node order, so each
   // magnet's boundary nodes form one contiguous run
   long long boff_elems = 0;
+  int max_ldb = 0;
   for (int k = 0; k < ninst; k++) {
     const int b0 = (int)(std::lower_bound(o.bnode.begin(), o.bnode.end(), node_off[k]) -
                          o.bnode.begin());
     const int b1 = (int)(std::lower_bound(o.bnode.begin(), o.bnode.end(), node_off[k + 1]) -
                          o.bnode.begin());
     const int nbk = b1 - b0, ldk = ((nbk + 63) / 64) * 64;
-    d.h_seg_nb.push_back(nbk);
-    d.h_seg_ldb.push_back(ldk);
-    d.h_seg_boff.push_back(b0);
-    d.h_seg_Boff.push_back(boff_elems);
-    d.h_seg_geo_eps.push_back(ninst == 1 ? o.geo_eps : inst_eps[k]);
+    d.bem.h_n.push_back(nbk);
+    d.bem.h_ld.push_back(ldk);
+    d.bem.h_off.push_back(b0);
+    d.bem.h_aoff.push_back(boff_elems);
+    d.h_bem_geo_eps.push_back(ninst == 1 ? o.geo_eps : inst_eps[k]);
     boff_elems += (long long)nbk * ldk;
-    d.ldb = std::max(d.ldb, ldk);
+    max_ldb = std::max(max_ldb, ldk);
     h->bnd_off.push_back(b0);
   }
   h->bnd_off.push_back(o.nb);
-  h->bem_elems = boff_elems;
-  if (h->opt.do_demag && (size_t)d.ldb * (d.btype == kBemBF16 ? 4 : 8) > 64 * 1024)
-    return cleanup(fail(me + ": " + std::to_string(d.ldb) +
-                        " boundary nodes in one magnet exceed the BEM GEMV's LDS staging "
-                        "(8192 at FP64/FP32 storage, 16384 at BF16)"));
+  d.bem.elems = boff_elems;
+  if (h->opt.do_demag && max_ldb > dense_gemv_max_order(stype))
+    return cleanup(fail(me + ": " + std::to_string(max_ldb) +
+                        " boundary nodes in one magnet exceed the BEM GEMV's LDS staging (" +
+                        std::to_string(dense_gemv_max_order(stype)) + ")"));
+  // dense potential operator: per magnet N_k x N_k, when every magnet fits
+  // the GEMV's staging and all of them fit the budget
+  {
+    long long pot_elems = 0;
+    int max_n = 0;
+    for (int k = 0; k < ninst; k++) {
+      const int nk = node_off[k + 1] - node_off[k];
+      pot_elems += (long long)nk * (((nk + 63) / 64) * 64);
+      max_n = std::max(max_n, nk);
+    }
+    const size_t esz = stype == kBemF64 ? 8 : stype == kBemF32 ? 4 : 2;
+    const bool fits = max_n <= dense_gemv_max_order(stype) &&
+                      (double)pot_elems * esz <= kPotentialBudgetBytes;
+    if (h->opt.potential > 0 && !fits)
+      return cleanup(fail(me + ": the dense potential operator needs every magnet to have at "
+                               "most " + std::to_string(dense_gemv_max_order(stype)) +
+                          " nodes and at most 32 GiB in all"));
+    h->use_potential = h->opt.do_demag && o.nb > 0 && h->opt.potential >= 0 && fits;
+  }
 
   hipError_t e;
 #define UP(dst, vec)                                                        \
@@ -412,19 +440,27 @@ int create_impl(const char* who, int ninst, const nsim_mesh* meshes,
     UP(d.omega, o.omega);
     UP(d.inc_rp, o.inc_rp);
     UP(d.inc_tri, o.inc_tri);
-    UP(d.seg_nb, d.h_seg_nb);
-    UP(d.seg_ldb, d.h_seg_ldb);
-    UP(d.seg_boff, d.h_seg_boff);
-    UP(d.seg_Boff, d.h_seg_Boff);
-    if (setup_gemv(h, 2, 2) != 0) return cleanup(-1);
-    size_t esz = d.btype == kBemF64 ? 8 : d.btype == kBemF32 ? 4 : 2;
-    e = hipMalloc(&d.B, (size_t)h->bem_elems * esz);
+    UP(d.bem.n, d.bem.h_n);
+    UP(d.bem.ld, d.bem.h_ld);
+    UP(d.bem.off, d.bem.h_off);
+    UP(d.bem.aoff, d.bem.h_aoff);
+    d.bem.map = d.bnode;
+    if (setup_gemv(h, d.bem, 2, 2, 4) != 0) return cleanup(-1);
+    const size_t esz = stype == kBemF64 ? 8 : stype == kBemF32 ? 4 : 2;
+    e = hipMalloc(&d.bem.A, (size_t)d.bem.elems * esz);
     if (e != hipSuccess) return cleanup(fail(std::string("BEM alloc: ") + hipGetErrorString(e)));
-    h->track(d.B);
+    h->track(d.bem.A);
     e = launch_bem_assemble(d, h->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
     if (e != hipSuccess)
       return cleanup(fail(std::string("BEM assembly: ") + hipGetErrorString(e)));
+    if (h->use_potential) {
+      std::vector<int> iota(o.n);
+      for (int i = 0; i < o.n; i++) iota[i] = i;
+      UP(d.iota, iota);
+      if (build_potential(h) != 0) return cleanup(-1);
+      if (setup_gemv(h, d.pot, 2, 1, 16) != 0) return cleanup(-1);
+    }
   }
 
   // field vectors
@@ -512,8 +548,10 @@ void nsim_destroy(nsim_handle* h) {
     }
   for (void* p : h->allocations) (void)hipFree(p);
   for (void* p : h->cg_allocs) (void)hipFree(p);
-  if (h->d.wg_seg) (void)hipFree(h->d.wg_seg);
-  if (h->d.wg_row0) (void)hipFree(h->d.wg_row0);
+  for (DenseOp* op : {&h->d.bem, &h->d.pot}) {
+    if (op->wg_seg) (void)hipFree(op->wg_seg);
+    if (op->wg_row0) (void)hipFree(op->wg_row0);
+  }
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
 }
@@ -542,10 +580,52 @@ int nsim_set_tuning(nsim_handle* h, const char* key, int value) {
     h->cg_coarse = value != 0;
     return 0;
   }
-  if (k == "bem_rows") return setup_gemv(h, value, h->d.gemv_wg_per_cu);
-  if (k == "bem_wg_per_cu") return setup_gemv(h, h->d.gemv_rows, value);
+  DenseOp& b = h->d.bem;
+  DenseOp& p = h->d.pot;
+  if (k == "bem_rows") return setup_gemv(h, b, value, b.wg_per_cu, b.waves);
+  if (k == "bem_wg_per_cu") return setup_gemv(h, b, b.rows, value, b.waves);
+  if (k == "bem_waves") return setup_gemv(h, b, b.rows, b.wg_per_cu, value);
+  if (k == "pot_rows") return setup_gemv(h, p, value, p.wg_per_cu, p.waves);
+  if (k == "pot_wg_per_cu") return setup_gemv(h, p, p.rows, value, p.waves);
+  if (k == "pot_waves") return setup_gemv(h, p, p.rows, p.wg_per_cu, value);
   return fail("unknown tuning key: " + k);
 }
+
+}  // extern "C"
+
+namespace {
+
+// Times every tile shape of a grouped dense GEMV and keeps the fastest;
+// returns its time per launch in microseconds (0 if the operator is empty).
+double tune_dense(nsim_handle* h, DenseOp& op, const double* x, double* y, hipEvent_t a,
+                  hipEvent_t b) {
+  if (!op.A) return 0.0;
+  double best = 1e30;
+  int br = op.rows, bw = op.wg_per_cu, bv = op.waves;
+  for (int v : {4, 16})
+    for (int r : {1, 2, 4})
+      for (int w : {1, 2, 4, 8}) {
+        if (setup_gemv(h, op, r, w, v) != 0) continue;
+        bool ok = true;
+        for (int k = 0; k < 2 && ok; k++)
+          ok = launch_dense_gemv(op, x, y, h->stream) == hipSuccess;
+        if (!ok) continue;
+        (void)hipEventRecord(a, h->stream);
+        for (int k = 0; k < 10; k++) (void)launch_dense_gemv(op, x, y, h->stream);
+        (void)hipEventRecord(b, h->stream);
+        float ms = 0.f;
+        if (hipEventSynchronize(b) != hipSuccess || hipEventElapsedTime(&ms, a, b) != hipSuccess)
+          continue;
+        const double us = ms * 100.0;  // per launch
+        if (us < best) best = us, br = r, bw = w, bv = v;
+      }
+  if (setup_gemv(h, op, br, bw, bv) != 0) return -1.0;
+  return best;
+}
+
+}  // namespace
+
+extern "C" {
 
 int nsim_autotune(nsim_handle* h, int what) {
   if (!h) return fail("nsim_autotune: null handle");
@@ -565,26 +645,15 @@ int nsim_autotune(nsim_handle* h, int what) {
   };
   const bool demag = h->opt.do_demag != 0 && h->d.nb > 0;
   if ((what & 1) && h->d.nb > 0) {
-    // BEM GEMV tile shape: rows streamed together per wavefront x workgroups per CU
-    double best = 1e30;
-    int br = h->d.gemv_rows, bw = h->d.gemv_wg_per_cu;
-    for (int r : {1, 2, 4})
-      for (int w : {1, 2, 4, 8}) {
-        if (setup_gemv(h, r, w) != 0) continue;
-        for (int k = 0; k < 2; k++)
-          if (launch_bem_gemv(h->d, h->phi1b, h->phi2, h->stream, h->num_cus) != hipSuccess)
-            return done(fail("nsim_autotune: GEMV launch failed"));
-        (void)hipEventRecord(a, h->stream);
-        for (int k = 0; k < 10; k++)
-          (void)launch_bem_gemv(h->d, h->phi1b, h->phi2, h->stream, h->num_cus);
-        (void)hipEventRecord(b, h->stream);
-        const double us = elapsed_ms() * 100.0;  // per launch
-        if (us < best) best = us, br = r, bw = w;
-      }
-    if (setup_gemv(h, br, bw) != 0) return done(-1);
-    h->tuned_bem_us = best;
+    // BEM GEMV tile shape (scratch output: the GEMV result is not kept)
+    h->tuned_bem_us = tune_dense(h, h->d.bem, h->phi1b, h->phi2, a, b);
+    if (h->tuned_bem_us < 0.0) return done(-1);
   }
-  if ((what & 2) && demag) {
+  if ((what & 4) && h->d.pot.A) {
+    h->tuned_pot_us = tune_dense(h, h->d.pot, h->rho, h->phi2, a, b);
+    if (h->tuned_pot_us < 0.0) return done(-1);
+  }
+  if ((what & 2) && demag && !h->d.pot.A) {
     // CG layout: rows per thread (part size), timed on a cold Neumann solve
     // of the load of the current m
     if (launch_exch_div(h->d, h->mixed, h->m, h->Hexch, h->rho, true, h->stream) != hipSuccess)
@@ -657,6 +726,9 @@ int nsim_get_field(nsim_handle* h, const char* name, double* out) {
     for (int i = 0; i < h->d.nb; i++) out[i] = h->host.bnode[i];
     return 0;
   }
+  if ((f == "phi1" || f == "phi2") && h->d.pot.A)
+    return fail("nsim_get_field: '" + f + "' is not formed on the dense potential route "
+                "(phi = phi1 + phi2 is)");
   if (f == "m") src = h->m, cnt = 3 * n;
   else if (f == "H_ext") src = h->Hext, cnt = 3 * n;
   else if (f == "H_exch") src = h->Hexch, cnt = 3 * n;
@@ -676,23 +748,27 @@ int nsim_get_field(nsim_handle* h, const char* name, double* out) {
   return 0;
 }
 
-int nsim_get_bem(nsim_handle* h, double* out) {
-  if (!h || !out) return fail("nsim_get_bem: null argument");
-  if (h->d.nb == 0) return 0;
-  const size_t esz = h->d.btype == kBemF64 ? 8 : h->d.btype == kBemF32 ? 4 : 2;
-  std::vector<unsigned char> buf((size_t)h->bem_elems * esz);
+}  // extern "C"
+
+namespace {
+
+// Copies a DenseOp's segments to the host as FP64, one after another, each
+// n_k x n_k row-major without padding.
+int copy_dense(nsim_handle* h, const DenseOp& op, double* out) {
+  if (!op.A) return 0;
+  const size_t esz = op.type == kBemF64 ? 8 : op.type == kBemF32 ? 4 : 2;
+  std::vector<unsigned char> buf((size_t)op.elems * esz);
   HIPCHK(hipStreamSynchronize(h->stream));
-  HIPCHK(hipMemcpy(buf.data(), h->d.B, buf.size(), hipMemcpyDeviceToHost));
-  // segments one after another, each nb_k x nb_k row-major without padding
+  HIPCHK(hipMemcpy(buf.data(), op.A, buf.size(), hipMemcpyDeviceToHost));
   size_t o = 0;
-  for (size_t s = 0; s < h->d.h_seg_nb.size(); s++) {
-    const size_t nb = h->d.h_seg_nb[s], ld = h->d.h_seg_ldb[s], base = h->d.h_seg_Boff[s];
+  for (size_t s = 0; s < op.h_n.size(); s++) {
+    const size_t nb = op.h_n[s], ld = op.h_ld[s], base = op.h_aoff[s];
     for (size_t i = 0; i < nb; i++)
       for (size_t j = 0; j < nb; j++) {
         const size_t k = base + i * ld + j;
         double v;
-        if (h->d.btype == kBemF64) v = reinterpret_cast<double*>(buf.data())[k];
-        else if (h->d.btype == kBemF32) v = reinterpret_cast<float*>(buf.data())[k];
+        if (op.type == kBemF64) v = reinterpret_cast<double*>(buf.data())[k];
+        else if (op.type == kBemF32) v = reinterpret_cast<float*>(buf.data())[k];
         else {
           uint32_t u = (uint32_t)reinterpret_cast<uint16_t*>(buf.data())[k] << 16;
           float f;
@@ -704,6 +780,21 @@ int nsim_get_bem(nsim_handle* h, double* out) {
     o += nb * nb;
   }
   return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int nsim_get_bem(nsim_handle* h, double* out) {
+  if (!h || !out) return fail("nsim_get_bem: null argument");
+  return copy_dense(h, h->d.bem, out);
+}
+
+int nsim_get_potential(nsim_handle* h, double* out) {
+  if (!h || !out) return fail("nsim_get_potential: null argument");
+  if (!h->d.pot.A) return fail("nsim_get_potential: this handle uses the iterative route");
+  return copy_dense(h, h->d.pot, out);
 }
 
 int nsim_execute(nsim_handle* h, const char* program) {
@@ -785,15 +876,22 @@ int nsim_get_stat(nsim_handle* h, const char* name, double* out) {
     *out = h->cg.maxw;
     return 0;
   }
-  if (s == "bem_rows" || s == "bem_wg_per_cu" || s == "bem_grid" || s == "n_magnets" ||
-      s == "tuned_bem_us" || s == "tuned_cg_us") {
-    *out = s == "bem_rows"        ? h->d.gemv_rows
-           : s == "bem_wg_per_cu" ? h->d.gemv_wg_per_cu
-           : s == "bem_grid"      ? h->d.gemv_grid
-           : s == "n_magnets"     ? h->d.ninst
-           : s == "tuned_bem_us"  ? h->tuned_bem_us
-                                  : h->tuned_cg_us;
-    return 0;
+  {
+    const DenseOp& b = h->d.bem;
+    const DenseOp& p = h->d.pot;
+    const std::map<std::string, double> simple = {
+        {"bem_rows", b.rows}, {"bem_wg_per_cu", b.wg_per_cu}, {"bem_waves", b.waves},
+        {"bem_grid", b.grid}, {"pot_rows", p.rows}, {"pot_wg_per_cu", p.wg_per_cu},
+        {"pot_waves", p.waves}, {"pot_grid", p.grid}, {"n_magnets", h->d.ninst},
+        {"potential", p.A ? 1.0 : 0.0}, {"tuned_bem_us", h->tuned_bem_us},
+        {"tuned_cg_us", h->tuned_cg_us}, {"tuned_pot_us", h->tuned_pot_us},
+        {"potential_bytes", (double)p.elems * (p.type == kBemF64 ? 8 : p.type == kBemF32 ? 4 : 2)},
+        {"bem_bytes", (double)b.elems * (b.type == kBemF64 ? 8 : b.type == kBemF32 ? 4 : 2)}};
+    const auto it = simple.find(s);
+    if (it != simple.end()) {
+      *out = it->second;
+      return 0;
+    }
   }
   unsigned a[8], b[8];
   HIPCHK(hipMemcpyAsync(a, h->sync_nbc, sizeof(a), hipMemcpyDeviceToHost, h->stream));

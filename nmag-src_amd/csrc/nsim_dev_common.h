@@ -64,6 +64,31 @@ __device__ inline unsigned ld_coh(const unsigned* p) {
   return __hip_atomic_load(const_cast<unsigned*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// FP32 -> BF16 with round-to-nearest-even (NaN kept a quiet NaN).
+__device__ inline uint16_t f32_to_bf16(float f) {
+  uint32_t u = __float_as_uint(f);
+  if ((u & 0x7f800000u) == 0x7f800000u && (u & 0x7fffffu)) return 0x7fc0;
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+
+// Stores an FP64 value in a dense operator of storage type TB (FP64, FP32
+// or BF16 as uint16_t).
+template <typename TB>
+__device__ inline void store_b(TB* p, double v);
+template <>
+__device__ inline void store_b<double>(double* p, double v) {
+  *p = v;
+}
+template <>
+__device__ inline void store_b<float>(float* p, double v) {
+  *p = (float)v;
+}
+template <>
+__device__ inline void store_b<uint16_t>(uint16_t* p, double v) {
+  *p = f32_to_bf16((float)v);
+}
+
 // Wait until every memory operation this wave issued has completed.  A
 // workgroup barrier alone does not wait for stores on gfx950.
 __device__ inline void wait_all_memory() { __builtin_amdgcn_s_waitcnt(0); }

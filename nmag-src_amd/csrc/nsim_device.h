@@ -15,10 +15,32 @@ struct Coeffs {
   double Ms, negJ, c1, c2, c3, mu0;
 };
 
+// A block-diagonal set of dense row-major matrices (one segment per magnet)
+// applied by one grouped GEMV launch as
+//     y[map[off_s + r]] = sum_j A_s[r][j] x[off_s + j].
+// Used for the BEM matrix (x and y on the boundary nodes) and for the dense
+// demag potential operator (x and y on all nodes).
+struct DenseOp {
+  void* A = nullptr;
+  int type = kBemF64;                 // storage: FP64, FP32 or BF16
+  std::vector<int> h_n, h_ld, h_off;  // per segment: order, row stride (x64), x/map offset
+  std::vector<long long> h_aoff;      // per segment: first element in A
+  long long elems = 0;
+  int* n = nullptr;
+  int* ld = nullptr;
+  int* off = nullptr;
+  long long* aoff = nullptr;
+  const int* map = nullptr;  // output index map (device)
+  int* wg_seg = nullptr;     // plan: workgroup -> segment
+  int* wg_row0 = nullptr;    //       workgroup -> first row in the segment
+  int grid = 0, rpw = 0, max_ld = 0;
+  int rows = 2, wg_per_cu = 2, waves = 4;  // tile shape: rows per pass, workgroups per
+                                           // CU, wavefronts per workgroup
+};
+
 // Operators resident in HBM (one copy per precision actually used).
 struct DeviceOps {
   int n = 0, nb = 0, ntri = 0, nslices = 0, sell_total = 0;
-  int ldb = 0;        // widest BEM segment's row stride (multiple of 64)
   int max_width = 0;  // widest SELL slice (slots per row)
   int* sell_off = nullptr;
   int* sell_col = nullptr;
@@ -32,20 +54,10 @@ struct DeviceOps {
   double* dinv_dbc = nullptr;  // 1/K_ii on interior rows, 0 on boundary rows
   int* bnode = nullptr;        // local boundary index -> node
   int* blocal = nullptr;       // node -> local boundary index or -1
-  void* B = nullptr;           // block-diagonal dense segments, each nb_s x ldb_s row-major
-  int btype = kBemF64;
-  // BEM segments (one per independent magnet): host copies and device tables
-  std::vector<int> h_seg_nb, h_seg_ldb, h_seg_boff;
-  std::vector<long long> h_seg_Boff;
-  std::vector<double> h_seg_geo_eps;
-  int* seg_nb = nullptr;
-  int* seg_ldb = nullptr;
-  int* seg_boff = nullptr;
-  long long* seg_Boff = nullptr;
-  int* wg_seg = nullptr;   // GEMV workgroup -> segment
-  int* wg_row0 = nullptr;  // GEMV workgroup -> first row in the segment
-  int gemv_grid = 0, gemv_rpw = 0, max_ldb = 0;
-  int gemv_rows = 2, gemv_wg_per_cu = 2;  // tile shape: rows per pass, workgroups per CU
+  DenseOp bem;                      // BEM matrix, one segment per magnet
+  std::vector<double> h_bem_geo_eps;  // per segment geometric tolerance
+  DenseOp pot;                      // dense demag potential operator (A null: iterative)
+  int* iota = nullptr;              // identity map of the nodes
   // coefficients: one entry per magnet; inst[i] = magnet of node i (null
   // when the handle holds a single magnet)
   Coeffs* coefs = nullptr;
@@ -107,14 +119,20 @@ hipError_t launch_cg(const CgDevice& c, CgWork& w, bool dirichlet, bool coarse,
                      const double* b, double* x, double rtol, int maxit, const double* xadd,
                      double* xsum, double* restrict_out, const int* blocal,
                      hipStream_t st);
-hipError_t launch_bem_gemv(const DeviceOps& d, const double* phi1b, double* phi2,
-                           hipStream_t st, int num_cus);
+// Grouped GEMV of a DenseOp (see there).
+hipError_t launch_dense_gemv(const DenseOp& op, const double* x, double* y, hipStream_t st);
 // Host plan of the grouped GEMV for a tile shape (rows_per_pass 1, 2 or 4 rows
-// streamed together by a wavefront; wg_per_cu 1..8 workgroups per CU): fills
-// d.gemv_* and max_ldb and the workgroup -> (segment, first row) table to
-// upload.  False for a shape outside those ranges.
-bool plan_bem_gemv(DeviceOps& d, int num_cus, int rows_per_pass, int wg_per_cu,
-                   std::vector<int>* wg_seg, std::vector<int>* wg_row0);
+// streamed together by a wavefront; wg_per_cu 1..8 workgroups per CU; waves 4
+// or 16 wavefronts per workgroup): fills op.grid/rpw/max_ld/rows/wg_per_cu/
+// waves and the workgroup -> (segment, first row) table to upload.  False for
+// a shape outside those ranges.
+bool plan_dense_gemv(DenseOp& op, int num_cus, int rows_per_pass, int wg_per_cu, int waves,
+                     std::vector<int>* wg_seg, std::vector<int>* wg_row0);
+// Largest segment order the grouped GEMV can stage (its x in LDS).
+int dense_gemv_max_order(int type);
+// Assembles BEM segment s in FP64 into `out` (row stride op.h_ld[s]), whatever
+// the storage type of d.bem.
+hipError_t launch_bem_assemble_f64(const DeviceOps& d, int s, double* out, hipStream_t st);
 hipError_t launch_demag_llg(const DeviceOps& d, bool mixed, const FieldArgs& f,
                             const FieldBufs& b, double* dmdt, bool do_demag, hipStream_t st);
 hipError_t launch_jtimes(const DeviceOps& d, bool mixed, const double* m, const double* Htotal,

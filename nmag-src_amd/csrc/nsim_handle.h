@@ -36,8 +36,8 @@ struct nsim_handle {
   std::vector<nsim::Coeffs> coefs;  // host copy of the per-magnet table
   std::vector<int> node_off;     // magnet k owns nodes [node_off[k], node_off[k+1])
   std::vector<int> bnd_off;      // and boundary nodes [bnd_off[k], bnd_off[k+1])
-  long long bem_elems = 0;       // stored BEM entries over all segments
-  double tuned_bem_us = 0.0, tuned_cg_us = 0.0;  // autotuner's best timings
+  bool use_potential = false;    // demag through the dense potential operator
+  double tuned_bem_us = 0.0, tuned_cg_us = 0.0, tuned_pot_us = 0.0;  // autotuner's best
   nsim_options opt{};
   bool mixed = false;
   int device = 0, num_cus = 0;
@@ -70,8 +70,13 @@ struct nsim_handle {
 
 namespace nsim {
 
+// Memory the dense potential operators of a handle may take.
+constexpr double kPotentialBudgetBytes = 32.0 * 1024 * 1024 * 1024;
+
 // Shared by the C-ABI translation units.
 int device_rhs(nsim_handle* h, const double* m, double* dmdt);
+// Builds the dense demag potential operator of every magnet (nsim_dense.cpp).
+int build_potential(nsim_handle* h);
 // Re-uploads the per-magnet coefficient table and the exchange diagonal kappa
 // from h->coefs.
 int refresh_coefficients(nsim_handle* h);

@@ -31,6 +31,28 @@ __device__ inline const Coeffs& coef_of(const Coeffs* __restrict__ ct,
   return ct[inst ? inst[row] : 0];
 }
 
+// Work split of the sparse kernels below: a workgroup of four wavefronts owns
+// one SELL slice (64 rows, one per lane); wavefront t takes the slots
+// k = t, t + 4, ... of every row, so even a 20k-node mesh keeps 1200+
+// wavefronts in flight, and partial sums meet in LDS in a fixed order.
+constexpr int kSplit = 4;
+
+// Sums the kSplit partials of Q quantities into wavefront 0's registers.
+template <int Q>
+__device__ inline void gather_partials(double (&v)[Q], double (*sh)[64], int t, int lane) {
+#pragma unroll
+  for (int q = 0; q < Q; q++) sh[t * Q + q][lane] = v[q];
+  __syncthreads();
+  if (t == 0) {
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+      double s = sh[q][lane];
+      for (int u = 1; u < kSplit; u++) s += sh[u * Q + q][lane];
+      v[q] = s;
+    }
+  }
+}
+
 // H_exch = negJ/V_i sum_j K_ij m_j (set_H_exch, nmag_lam.py:734-739) and
 // rho_i = Ms sum_j D_ij . m_j (op_div_m then SCALE -1, nmag_lam.py:742-743).
 // The magnets of a batch share no node, so row and columns share one Ms.
@@ -51,28 +73,33 @@ __global__ __launch_bounds__(256) void k_exch_div(int n, const int* __restrict__
                                                   const double* __restrict__ m,
                                                   double* __restrict__ Hexch,
                                                   double* __restrict__ rho, int want_rho) {
-  const int row = blockIdx.x * 256 + threadIdx.x;
-  if (row >= n) return;
-  const int s = row >> 6, lane = row & 63;
+  __shared__ double sh[4 * kSplit][64];
+  const int s = blockIdx.x, t = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int row = s * 64 + lane;
+  const bool live = row < n;
   const int b = off[s], w = (off[s + 1] - b) >> 6;
-  const double mix = m[3 * row], miy = m[3 * row + 1], miz = m[3 * row + 2];
-  double ex = 0.0, ey = 0.0, ez = 0.0, r = 0.0;
-  for (int k = 0; k < w; k++) {
-    const int e = b + (k << 6) + lane;
-    const int j = col[e];
-    const double kv = (double)K[e];
-    const double mx = m[3 * j], my = m[3 * j + 1], mz = m[3 * j + 2];
-    ex += kv * (mx - mix);
-    ey += kv * (my - miy);
-    ez += kv * (mz - miz);
-    if (want_rho) r += D[3 * e] * mx + D[3 * e + 1] * my + D[3 * e + 2] * mz;
+  double v[4] = {0.0, 0.0, 0.0, 0.0};
+  if (live) {
+    const double mix = m[3 * row], miy = m[3 * row + 1], miz = m[3 * row + 2];
+    for (int k = t; k < w; k += kSplit) {
+      const int e = b + (k << 6) + lane;
+      const int j = col[e];
+      const double kv = (double)K[e];
+      const double mx = m[3 * j], my = m[3 * j + 1], mz = m[3 * j + 2];
+      v[0] += kv * (mx - mix);
+      v[1] += kv * (my - miy);
+      v[2] += kv * (mz - miz);
+      if (want_rho) v[3] += D[3 * e] * mx + D[3 * e + 1] * my + D[3 * e + 2] * mz;
+    }
   }
+  gather_partials<4>(v, sh, t, lane);
+  if (t != 0 || !live) return;
   const Coeffs& c = coef_of(ct, inst, row);
   const double f = c.negJ * Vinv[row];
-  Hexch[3 * row] = f * ex;
-  Hexch[3 * row + 1] = f * ey;
-  Hexch[3 * row + 2] = f * ez;
-  if (want_rho) rho[row] = c.Ms * r;
+  Hexch[3 * row] = f * v[0];
+  Hexch[3 * row + 1] = f * v[1];
+  Hexch[3 * row + 2] = f * v[2];
+  if (want_rho) rho[row] = c.Ms * v[3];
 }
 
 // H_demag_i = sum_j G_ij phi_j (op_grad_phi + CFBOX, nmag_lam.py:755-759),
@@ -90,21 +117,28 @@ __global__ __launch_bounds__(256) void k_demag_llg(int n, const int* __restrict_
                                                    const int* __restrict__ inst, FieldArgs f,
                                                    FieldBufs bf, double* __restrict__ dmdt,
                                                    int do_demag) {
-  const int row = blockIdx.x * 256 + threadIdx.x;
-  if (row >= n) return;
-  const Coeffs& c = coef_of(ct, inst, row);
+  __shared__ double sh[3 * kSplit][64];
+  const int s = blockIdx.x, t = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int row = s * 64 + lane;
+  const bool live = row < n;
   double hd[3] = {0.0, 0.0, 0.0};
   if (do_demag) {
-    const int s = row >> 6, lane = row & 63;
     const int b = off[s], w = (off[s + 1] - b) >> 6;
-    const double phi_i = bf.phi[row];
-    for (int k = 0; k < w; k++) {
-      const int e = b + (k << 6) + lane;
-      const double ph = bf.phi[col[e]] - phi_i;
-      hd[0] += (double)G[3 * e] * ph;
-      hd[1] += (double)G[3 * e + 1] * ph;
-      hd[2] += (double)G[3 * e + 2] * ph;
+    if (live) {
+      const double phi_i = bf.phi[row];
+      for (int k = t; k < w; k += kSplit) {
+        const int e = b + (k << 6) + lane;
+        const double ph = bf.phi[col[e]] - phi_i;
+        hd[0] += (double)G[3 * e] * ph;
+        hd[1] += (double)G[3 * e + 1] * ph;
+        hd[2] += (double)G[3 * e + 2] * ph;
+      }
     }
+    gather_partials<3>(hd, sh, t, lane);
+  }
+  if (t != 0 || !live) return;
+  const Coeffs& c = coef_of(ct, inst, row);
+  if (do_demag) {
     bf.Hdemag[3 * row] = hd[0];
     bf.Hdemag[3 * row + 1] = hd[1];
     bf.Hdemag[3 * row + 2] = hd[2];
@@ -150,22 +184,26 @@ __global__ __launch_bounds__(256) void k_jtimes(int n, const int* __restrict__ o
                                                 const double* __restrict__ Ht,
                                                 const double* __restrict__ v,
                                                 double* __restrict__ Jv) {
-  const int row = blockIdx.x * 256 + threadIdx.x;
-  if (row >= n) return;
-  const Coeffs& c = coef_of(ct, inst, row);
-  const int s = row >> 6, lane = row & 63;
+  __shared__ double sh[3 * kSplit][64];
+  const int s = blockIdx.x, t = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int row = s * 64 + lane;
+  const bool live = row < n;
   const int b = off[s], w = (off[s + 1] - b) >> 6;
-  double hx = 0.0, hy = 0.0, hz = 0.0;
-  for (int k = 0; k < w; k++) {
-    const int e = b + (k << 6) + lane;
-    const int j = col[e];
-    const double kv = (double)K[e];
-    hx += kv * v[3 * j];
-    hy += kv * v[3 * j + 1];
-    hz += kv * v[3 * j + 2];
-  }
+  double hv[3] = {0.0, 0.0, 0.0};
+  if (live)
+    for (int k = t; k < w; k += kSplit) {
+      const int e = b + (k << 6) + lane;
+      const int j = col[e];
+      const double kv = (double)K[e];
+      hv[0] += kv * v[3 * j];
+      hv[1] += kv * v[3 * j + 1];
+      hv[2] += kv * v[3 * j + 2];
+    }
+  gather_partials<3>(hv, sh, t, lane);
+  if (t != 0 || !live) return;
+  const Coeffs& c = coef_of(ct, inst, row);
   const double fct = c.negJ * Vinv[row];
-  const double h[3] = {fct * hx, fct * hy, fct * hz};
+  const double h[3] = {fct * hv[0], fct * hv[1], fct * hv[2]};
   const double mi[3] = {m[3 * row], m[3 * row + 1], m[3 * row + 2]};
   const double vi[3] = {v[3 * row], v[3 * row + 1], v[3 * row + 2]};
   const double H[3] = {Ht[3 * row], Ht[3 * row + 1], Ht[3 * row + 2]};
@@ -180,18 +218,16 @@ __global__ __launch_bounds__(256) void k_jtimes(int n, const int* __restrict__ o
     Jv[3 * row + d] = c.c1 * (vxH[d] + mxh[d]) + c.c2 * (vxmxH[d] + mxvxH[d] + mxmxh[d]);
 }
 
-inline int blocks_for(int n) { return (n + 255) / 256; }
-
 }  // namespace
 
 hipError_t launch_exch_div(const DeviceOps& d, bool mixed, const double* m, double* Hexch,
                            double* rho, bool want_rho, hipStream_t st) {
   if (mixed)
-    k_exch_div<float><<<blocks_for(d.n), 256, 0, st>>>(d.n, d.sell_off, d.sell_col, d.Kf,
+    k_exch_div<float><<<d.nslices, 256, 0, st>>>(d.n, d.sell_off, d.sell_col, d.Kf,
                                                          d.D, d.Vinv, d.coefs, d.inst, m,
                                                          Hexch, rho, want_rho ? 1 : 0);
   else
-    k_exch_div<double><<<blocks_for(d.n), 256, 0, st>>>(d.n, d.sell_off, d.sell_col, d.K,
+    k_exch_div<double><<<d.nslices, 256, 0, st>>>(d.n, d.sell_off, d.sell_col, d.K,
                                                           d.D, d.Vinv, d.coefs, d.inst, m,
                                                           Hexch, rho, want_rho ? 1 : 0);
   return hipGetLastError();
@@ -200,11 +236,11 @@ hipError_t launch_exch_div(const DeviceOps& d, bool mixed, const double* m, doub
 hipError_t launch_demag_llg(const DeviceOps& d, bool mixed, const FieldArgs& f,
                             const FieldBufs& b, double* dmdt, bool do_demag, hipStream_t st) {
   if (mixed)
-    k_demag_llg<float><<<blocks_for(d.n), 256, 0, st>>>(d.n, d.sell_off, d.sell_col, d.Gf,
+    k_demag_llg<float><<<d.nslices, 256, 0, st>>>(d.n, d.sell_off, d.sell_col, d.Gf,
                                                           d.coefs, d.inst, f, b, dmdt,
                                                           do_demag ? 1 : 0);
   else
-    k_demag_llg<double><<<blocks_for(d.n), 256, 0, st>>>(d.n, d.sell_off, d.sell_col, d.G,
+    k_demag_llg<double><<<d.nslices, 256, 0, st>>>(d.n, d.sell_off, d.sell_col, d.G,
                                                            d.coefs, d.inst, f, b, dmdt,
                                                            do_demag ? 1 : 0);
   return hipGetLastError();
@@ -213,11 +249,11 @@ hipError_t launch_demag_llg(const DeviceOps& d, bool mixed, const FieldArgs& f,
 hipError_t launch_jtimes(const DeviceOps& d, bool mixed, const double* m, const double* Htotal,
                          const double* v, double* Jv, hipStream_t st) {
   if (mixed)
-    k_jtimes<float><<<blocks_for(d.n), 256, 0, st>>>(d.n, d.sell_off, d.sell_col, d.Kf,
+    k_jtimes<float><<<d.nslices, 256, 0, st>>>(d.n, d.sell_off, d.sell_col, d.Kf,
                                                        d.Vinv, d.coefs, d.inst, m, Htotal, v,
                                                        Jv);
   else
-    k_jtimes<double><<<blocks_for(d.n), 256, 0, st>>>(d.n, d.sell_off, d.sell_col, d.K,
+    k_jtimes<double><<<d.nslices, 256, 0, st>>>(d.n, d.sell_off, d.sell_col, d.K,
                                                         d.Vinv, d.coefs, d.inst, m, Htotal, v,
                                                         Jv);
   return hipGetLastError();

@@ -90,8 +90,9 @@ int ensure_vjp(nsim_handle* h) {
     for (double** p : {&A->g, &A->x, &A->a, &A->lam, &A->out}) TRY(dalloc(A, p, n3));
     for (double** p : {&A->psi, &A->z, &A->phia, &A->k1bar, &A->msbar}) TRY(dalloc(A, p, n));
     TRY(dalloc(A, &A->e, std::max(1, h->d.nb)));
-    const size_t ldb = h->d.h_seg_ldb.empty() ? 1 : h->d.h_seg_ldb[0];
-    TRY(dalloc(A, &A->part, (size_t)bem_t_chunks(h->d) * ldb));
+    const DenseOp& op = h->d.pot.A ? h->d.pot : h->d.bem;
+    const size_t ld = op.h_ld.empty() ? 1 : op.h_ld[0];
+    TRY(dalloc(A, &A->part, (size_t)dense_t_chunks(op) * ld));
   }
   return 0;
 }
@@ -119,7 +120,12 @@ int vjp(nsim_handle* h, const double* y, const double* lam, double* out, const A
               h->phi1,  h->phi2, h->phi, h->phi1b};
   HIPCHK(launch_adj_local(d, field_args_of(h, y), b, lam, A->g, A->x, out, acc, st));
   HIPCHK(launch_adj_exch_div(d, h->mixed, A->x, out, A->psi, st));
-  if (demag) {
+  if (demag && d.pot.A) {
+    // dense route phi = M rho: the adjoint potential is M^T psi
+    HIPCHK(hipMemsetAsync(A->phia, 0, (size_t)d.n * sizeof(double), st));
+    HIPCHK(launch_dense_gemv_t(d.pot, A->psi, A->part, A->phia, st));
+    HIPCHK(launch_adj_grad(d, h->mixed, A->phia, out, st));
+  } else if (demag) {
     CgWork& w = h->cgw;
     // E^T psi: interior solve K_II z = psi_I (z = 0 on the boundary), then
     // e = psi_B - K_BI z_I
@@ -129,7 +135,7 @@ int vjp(nsim_handle* h, const double* y, const double* lam, double* out, const A
     HIPCHK(launch_adj_boundary(d, A->psi, A->z, A->e, st));
     // chi = psi + R^T B^T e, then K^+ chi (the pseudo-inverse projects out the
     // constants first)
-    HIPCHK(launch_bem_gemv_t(d, A->e, A->part, A->psi, st));
+    HIPCHK(launch_dense_gemv_t(d.bem, A->e, A->part, A->psi, st));
     HIPCHK(launch_mean_project(d.n, A->psi, st));
     w.sync = h->sync_nbc;
     HIPCHK(launch_cg(h->cg, w, false, h->cg_coarse, A->psi, A->phia, h->opt.nbc_rtol,

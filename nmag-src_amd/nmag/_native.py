@@ -32,7 +32,11 @@ class NsimMaterial(ctypes.Structure):
 class NsimOptions(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int), ("do_demag", ctypes.c_int),
                 ("nbc_rtol", ctypes.c_double), ("dbc_rtol", ctypes.c_double),
-                ("nbc_maxits", ctypes.c_int), ("dbc_maxits", ctypes.c_int)]
+                ("nbc_maxits", ctypes.c_int), ("dbc_maxits", ctypes.c_int),
+                ("potential", ctypes.c_int)]
+
+
+POTENTIAL = {"auto": 0, "dense": 1, "iterative": -1}
 
 
 class NsimTraj(ctypes.Structure):
@@ -72,6 +76,7 @@ SIGNATURES = {
                                          ctypes.POINTER(NsimOptions), ctypes.POINTER(_vp)]),
     "nsim_get_batch_layout": (ctypes.c_int, [_vp, _ip, _ip]),
     "nsim_autotune": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "nsim_get_potential": (ctypes.c_int, [_vp, _dp]),
     "nsim_set_material": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(NsimMaterial)]),
     "nsim_rhs_vjp": (ctypes.c_int, [_vp, ctypes.c_double, _dp, _dp, _dp, _dp, _dp, _dp]),
     "nsim_traj_forward": (ctypes.c_int, [_vp, ctypes.POINTER(NsimTraj), _dp]),
@@ -120,12 +125,16 @@ class NativeMagnet:
     SCALAR = ("K1", "K2", "pin", "rho", "phi", "phi1", "phi2", "volumes")
 
     def __init__(self, coords, tets, material, precision="fp64", do_demag=True,
-                 nbc_rtol=1e-5, dbc_rtol=1e-5, nbc_maxits=10000, dbc_maxits=10000):
+                 nbc_rtol=1e-5, dbc_rtol=1e-5, nbc_maxits=10000, dbc_maxits=10000,
+                 potential="auto"):
+        """potential: "iterative" (the two Poisson solves around the BEM, as
+        the reference), "dense" (the precomputed potential operator) or "auto"
+        (dense when it fits)."""
         self._create([(coords, tets)], [material], precision, do_demag, nbc_rtol, dbc_rtol,
-                     nbc_maxits, dbc_maxits)
+                     nbc_maxits, dbc_maxits, potential)
 
     def _create(self, meshes, materials, precision, do_demag, nbc_rtol, dbc_rtol, nbc_maxits,
-                dbc_maxits):
+                dbc_maxits, potential="auto"):
         L = lib()
         self._h = None
         if len(meshes) != len(materials) or not meshes:
@@ -143,7 +152,9 @@ class NativeMagnet:
         self._coords = (self._keep[0] if len(meshes) == 1
                         else np.concatenate(self._keep[0::2]))
         prec = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
-        opt = NsimOptions(prec, 1 if do_demag else 0, nbc_rtol, dbc_rtol, nbc_maxits, dbc_maxits)
+        pot = POTENTIAL[potential] if isinstance(potential, str) else int(potential)
+        opt = NsimOptions(prec, 1 if do_demag else 0, nbc_rtol, dbc_rtol, nbc_maxits, dbc_maxits,
+                          pot)
         h = _vp()
         if len(meshes) == 1:
             _check(L.nsim_create(cmeshes, cmats, ctypes.byref(opt), ctypes.byref(h)),
@@ -227,6 +238,21 @@ class NativeMagnet:
             o += nb * nb
         return mats[0] if self.n_magnets == 1 else mats
 
+    def potential(self):
+        """Dense potential operator M (phi = M rho); a list for a batch."""
+        sizes = np.diff(self.node_off)
+        flat = np.zeros(int(np.sum(sizes.astype(np.int64) ** 2)))
+        _check(lib().nsim_get_potential(self._h, _dptr(flat)), "nsim_get_potential")
+        mats, o = [], 0
+        for n in sizes:
+            mats.append(flat[o:o + n * n].reshape(n, n))
+            o += n * n
+        return mats[0] if self.n_magnets == 1 else mats
+
+    @property
+    def dense_potential(self):
+        return self.stat("potential") == 1.0
+
     def stat(self, name):
         v = ctypes.c_double(0)
         _check(lib().nsim_get_stat(self._h, name.encode(), ctypes.byref(v)), "nsim_get_stat")
@@ -243,13 +269,15 @@ class NativeMagnet:
     def tune(self, key, value):
         _check(lib().nsim_set_tuning(self._h, key.encode(), int(value)), "nsim_set_tuning")
 
-    def autotune(self, bem=True, cg=True):
+    def autotune(self, bem=True, cg=True, potential=True):
         """Time the tile shapes on the device and keep the fastest; returns the
         choice.  The CG leg solves the current m's load, so set m first."""
-        _check(lib().nsim_autotune(self._h, (1 if bem else 0) | (2 if cg else 0)),
-               "nsim_autotune")
-        return {k: self.stat(k) for k in ("bem_rows", "bem_wg_per_cu", "cg_rows_per_thread",
-                                          "tuned_bem_us", "tuned_cg_us")}
+        what = (1 if bem else 0) | (2 if cg else 0) | (4 if potential else 0)
+        _check(lib().nsim_autotune(self._h, what), "nsim_autotune")
+        return {k: self.stat(k) for k in ("bem_rows", "bem_wg_per_cu", "bem_waves",
+                                          "cg_rows_per_thread", "pot_rows", "pot_wg_per_cu",
+                                          "pot_waves", "tuned_bem_us", "tuned_cg_us",
+                                          "tuned_pot_us")}
 
     def synchronize(self):
         _check(lib().nsim_synchronize(self._h), "nsim_synchronize")
@@ -337,9 +365,9 @@ class NativeBatch(NativeMagnet):
     """
 
     def __init__(self, meshes, materials, precision="fp64", do_demag=True, nbc_rtol=1e-5,
-                 dbc_rtol=1e-5, nbc_maxits=10000, dbc_maxits=10000):
+                 dbc_rtol=1e-5, nbc_maxits=10000, dbc_maxits=10000, potential="auto"):
         self._create(list(meshes), list(materials), precision, do_demag, nbc_rtol, dbc_rtol,
-                     nbc_maxits, dbc_maxits)
+                     nbc_maxits, dbc_maxits, potential)
 
     def split(self, a):
         a = np.asarray(a)

@@ -34,10 +34,11 @@ def rel_err(a, b):
     return float(np.max(np.abs(a - b)) / (scale if scale > 0 else 1.0))
 
 
-def make_pair(nx, ny, nz, h, mat=None, precision="fp64", rtol=1e-13, build_bem=True):
+def make_pair(nx, ny, nz, h, mat=None, precision="fp64", rtol=1e-13, build_bem=True,
+              potential="auto"):
     mat = mat or material()
     coords, tets = kuhn_cuboid(nx, ny, nz, h)
     orc = oracle.OracleMesh(coords, tets, build_bem=build_bem)
     gpu = NativeMagnet(coords, tets, mat.su_coeffs(), precision=precision, nbc_rtol=rtol,
-                       dbc_rtol=rtol, nbc_maxits=20000, dbc_maxits=20000)
+                       dbc_rtol=rtol, nbc_maxits=20000, dbc_maxits=20000, potential=potential)
     return coords, tets, orc, gpu, mat

@@ -23,11 +23,11 @@ K_SU = float(to_su(SI(1.0, "J/m^3"), "J/m^3"))
 MT_SU = float(to_su(SI(1e-3 / (4e-7 * np.pi), "A/m"), "A/m"))  # 1 mT / mu0 in su
 
 
-def _magnet(dims=(6, 4, 2, 2.5), precision="fp64", alpha=0.5, rtol=1e-13):
+def _magnet(dims=(6, 4, 2, 2.5), precision="fp64", alpha=0.5, rtol=1e-13, potential="auto"):
     coords, tets = kuhn_cuboid(*dims)
     mat = material(alpha=alpha)
     g = NativeMagnet(coords, tets, mat.su_coeffs(), precision=precision, nbc_rtol=rtol,
-                     dbc_rtol=rtol, nbc_maxits=20000, dbc_maxits=20000)
+                     dbc_rtol=rtol, nbc_maxits=20000, dbc_maxits=20000, potential=potential)
     return coords, tets, mat, g
 
 
@@ -51,8 +51,10 @@ def _f(g, m):
     return g.rhs(m.reshape(-1))
 
 
-def test_rhs_vjp_dot_products_against_central_differences():
-    coords, tets, mat, g = _magnet()
+@pytest.mark.parametrize("route", ["iterative", "dense"])
+def test_rhs_vjp_dot_products_against_central_differences(route):
+    coords, tets, mat, g = _magnet(potential=route)
+    assert g.dense_potential == (route == "dense")
     m, H, k1, ax = _inputs(g.n, 1)
     _setup(g, H, k1, ax)
     rng = np.random.default_rng(2)
@@ -129,11 +131,11 @@ def test_rhs_vjp_mixed_precision_close_to_fp64():
 
 
 def _problem(precision="fp64", checkpoint_every=10, n_steps=100, n_pulses=4, seed=7,
-             rtol=1e-12, dims=(8, 4, 1, 2.5)):
+             rtol=1e-12, dims=(8, 4, 1, 2.5), potential="auto"):
     coords, tets = kuhn_cuboid(*dims)
     mat = material(alpha=0.02)
     g = NativeMagnet(coords, tets, mat.su_coeffs(), precision=precision, nbc_rtol=rtol,
-                     dbc_rtol=rtol, nbc_maxits=20000, dbc_maxits=20000)
+                     dbc_rtol=rtol, nbc_maxits=20000, dbc_maxits=20000, potential=potential)
     rng = np.random.default_rng(seed)
     n = len(coords)
     m0 = np.tile([1.0, 0.3, 0.1], (n, 1)) + 0.05 * rng.standard_normal((n, 3))
@@ -148,8 +150,9 @@ def _problem(precision="fp64", checkpoint_every=10, n_steps=100, n_pulses=4, see
     return prob, K, mat.su_coeffs()["Ms"], amps
 
 
-def test_trajectory_gradient_matches_central_differences():
-    prob, K, Ms, amps = _problem()
+@pytest.mark.parametrize("route", ["iterative", "dense"])
+def test_trajectory_gradient_matches_central_differences(route):
+    prob, K, Ms, amps = _problem(potential=route)
     prob.run(K, Ms, amps)
     gK, gMs, gA, gm0 = prob.gradient(amps)
     rel = 1e-3

@@ -160,25 +160,37 @@ def test_batch_integration_matches_single_magnets():
         assert np.max(np.abs(mb[k] - ms)) < 1e-4, k
 
 
+@pytest.mark.parametrize("route", ["iterative", "dense"])
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_autotuner_keeps_the_fields(precision):
+def test_autotuner_keeps_the_fields(precision, route):
     """Every tile shape the autotuner may pick computes the same fields."""
     meshes, mats, fields = _batch_inputs()
-    b = _batch(meshes, mats, precision, rtol=1e-12)
+    b = NativeBatch(meshes, [m.su_coeffs() for m in mats], precision=precision,
+                    nbc_rtol=1e-12, dbc_rtol=1e-12, nbc_maxits=20000, dbc_maxits=20000,
+                    potential=route)
+    assert b.dense_potential == (route == "dense")
     _set_all(b, fields)
     b.execute("update_dmdt")
     ref = {k: b.get_field(k) for k in ("H_demag", "dmdt")}
     choice = b.autotune()
     assert choice["bem_rows"] in (1, 2, 4) and 1 <= choice["bem_wg_per_cu"] <= 8
     assert choice["cg_rows_per_thread"] in (1, 2, 4)
-    assert choice["tuned_bem_us"] > 0 and choice["tuned_cg_us"] > 0
+    assert choice["tuned_bem_us"] > 0
+    if route == "dense":
+        assert choice["tuned_pot_us"] > 0 and choice["pot_waves"] in (4, 16)
+    else:
+        assert choice["tuned_cg_us"] > 0
+    op = "pot" if route == "dense" else "bem"
     for r in (1, 4):
         for w in (1, 8):
-            b.tune("bem_rows", r)
-            b.tune("bem_wg_per_cu", w)
-            b.set_field("phi1", np.zeros(b.n))
-            b.set_field("phi2", np.zeros(b.n))
-            b.execute("update_dmdt")
-            assert rel_err(b.get_field("H_demag"), ref["H_demag"]) < 1e-9, (r, w)
-            assert rel_err(b.get_field("dmdt"), ref["dmdt"]) < 1e-9, (r, w)
+            for v in (4, 16):
+                b.tune(op + "_rows", r)
+                b.tune(op + "_wg_per_cu", w)
+                b.tune(op + "_waves", v)
+                if route == "iterative":
+                    b.set_field("phi1", np.zeros(b.n))
+                    b.set_field("phi2", np.zeros(b.n))
+                b.execute("update_dmdt")
+                assert rel_err(b.get_field("H_demag"), ref["H_demag"]) < 1e-9, (r, w, v)
+                assert rel_err(b.get_field("dmdt"), ref["dmdt"]) < 1e-9, (r, w, v)
     assert b.stat("cg_error") == 0 and b.stat("cg_unconverged") == 0

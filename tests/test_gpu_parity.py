@@ -37,11 +37,13 @@ def test_bem_matrix_matches_oracle(dims):
     assert np.allclose(Bg.sum(axis=1), -1.0, atol=1e-10)
 
 
+@pytest.mark.parametrize("route", ["iterative", "dense"])
 @pytest.mark.parametrize("dims,seed", [((12, 12, 4, 2.5), 1), ((5, 3, 2, 2.0), 7),
                                        ((2, 2, 1, 2.5), 3), ((1, 1, 1, 2.5), 4)])
-def test_fields_fp64_match_oracle(dims, seed):
+def test_fields_fp64_match_oracle(dims, seed, route):
     nx, ny, nz, h = dims
-    _, _, orc, gpu, mat = make_pair(nx, ny, nz, h)
+    _, _, orc, gpu, mat = make_pair(nx, ny, nz, h, potential=route)
+    assert gpu.dense_potential == (route == "dense")
     m = random_m(orc.n, seed)
     P = orc_params(mat)
     ref = orc.fields(P, m, rtol=1e-13)
@@ -84,12 +86,14 @@ def test_rhs_host_pointers_and_jtimes():
     assert rel_err(jv, jref) < 1e-10
 
 
+@pytest.mark.parametrize("route", ["iterative", "dense"])
 @pytest.mark.parametrize("precision,tol_exch,tol_demag", [("fp32", 1e-7, 1e-6),
                                                            ("bf16", 1e-7, 2e-2)])
-def test_mixed_precision_field_error(precision, tol_exch, tol_demag):
+def test_mixed_precision_field_error(precision, tol_exch, tol_demag, route):
     """FP32 (BF16) operator storage: the per-field error against the FP64 oracle
     is that of rounding the stored operators; solves run to 1e-10."""
-    _, _, orc, gpu, mat = make_pair(12, 12, 4, 2.5, precision=precision, rtol=1e-10)
+    _, _, orc, gpu, mat = make_pair(12, 12, 4, 2.5, precision=precision, rtol=1e-10,
+                                    potential=route)
     m = random_m(orc.n, 1)
     ref = orc.fields(orc_params(mat), m, rtol=1e-13)
     got = _fields_gpu(gpu, m)
@@ -98,7 +102,7 @@ def test_mixed_precision_field_error(precision, tol_exch, tol_demag):
 
 
 def test_warm_start_reduces_iterations():
-    _, _, orc, gpu, mat = make_pair(12, 12, 4, 2.5, rtol=1e-8)
+    _, _, orc, gpu, mat = make_pair(12, 12, 4, 2.5, rtol=1e-8, potential="iterative")
     m = random_m(orc.n, 5)
     gpu.set_field("m", m)
     gpu.execute()
@@ -114,7 +118,7 @@ def test_two_level_preconditioner_matches_jacobi_solution(dims):
     """The coarse correction changes the iteration, not the solution; on one
     part (a single aggregate) it must vanish for the Neumann problem."""
     nx, ny, nz, h = dims
-    _, _, orc, gpu, mat = make_pair(nx, ny, nz, h, rtol=1e-12)
+    _, _, orc, gpu, mat = make_pair(nx, ny, nz, h, rtol=1e-12, potential="iterative")
     m = random_m(orc.n, 13)
     a = _fields_gpu(gpu, m)
     its2 = gpu.stat("cg_iters_nbc")
@@ -142,3 +146,26 @@ def test_demag_linear_and_uniform_cuboid_factors():
     hz = _fields_gpu(gpu, mz)["H_demag"]
     Nz = -(hz[:, 2] * V).sum() / V.sum() / mat.su_Ms
     assert abs(Nz - aharoni_nz(30.0, 30.0, 10.0)) < 0.02
+
+
+def test_dense_potential_operator_properties():
+    """M = (I + E B R) K^+: M 1 = 0 (K^+ 1 = 0), M rho matches the two solves
+    of the iterative route, and on the boundary M = K^+ + B K^+ restricted."""
+    coords, tets, orc, gpu, mat = make_pair(6, 4, 3, 2.5, potential="dense")
+    M = gpu.potential()
+    assert M.shape == (gpu.n, gpu.n)
+    assert np.max(np.abs(M.sum(axis=1))) <= 1e-10 * np.max(np.abs(M))
+    it = make_pair(6, 4, 3, 2.5, potential="iterative")[3]
+    m = random_m(gpu.n, 31)
+    for g in (gpu, it):
+        g.set_field("m", m)
+        g.execute()
+    rho = it.get_field("rho")
+    phi_it = it.get_field("phi")
+    phi = M @ rho
+    # the Neumann gauge constant of phi1 cancels in phi1 + phi2 (B 1 = -1), so
+    # the two routes give the same potential, not just the same field
+    assert np.max(np.abs(phi - phi_it)) <= 1e-10 * np.max(np.abs(phi_it))
+    assert np.max(np.abs(gpu.get_field("phi") - phi_it)) <= 1e-10 * np.max(np.abs(phi_it))
+    with pytest.raises(Exception):
+        gpu.get_field("phi1")
