@@ -2,11 +2,16 @@
 
 Workload (the metric's configuration): permalloy slab 160 x 80 x 10 nm, 2 nm
 Kuhn-split cubes (80 x 40 x 5 cubes, 96,000 tets, 19,926 nodes, 7,602 boundary
-nodes, dense 7,602^2 BEM), alpha = 0.02, 25 mT applied at 170 deg in-plane,
-seeded S-state.  One step = one full right-hand-side evaluation
-y -> H_exch + H_demag (2 PCG solves + BEM GEMV) + H_ext -> dm/dt, on a
-sequence of magnetisation states along a trajectory (warm-started solves, as
-the reference's KSPs are).  Inputs are resident in HBM before timing starts.
+nodes), alpha = 0.02, 25 mT applied at 170 deg in-plane, seeded S-state.  One
+step = one full right-hand-side evaluation y -> H_exch + H_demag + H_ext ->
+dm/dt on a sequence of magnetisation states along a trajectory.  The demag is
+timed on both routes: the dense potential operator (phi = M rho, one GEMV; the
+headline) and the reference's structure (two warm-started PCG solves around
+the BEM GEMV).  Inputs are resident in HBM before timing starts.
+
+Further legs: sim-ns/hour of the BDF integrator, the 64-magnet batch against
+64 sequential handles, the differentiation form (adjoint of a 2 ns RK4
+trajectory of the 100 x 50 x 5 nm slab), and the CPU oracle as the baseline.
 
 Prints ONE JSON line.  Progress goes to stderr.
 """
@@ -72,6 +77,14 @@ def gemv_bytes(nb, esz):
     return esz * nb * nb + 16 * nb
 
 
+AUTOTUNE = True
+# HBM bytes per launch of the potential GEMV at the 20k-node slab, measured
+# with rocprofv3 (FETCH_SIZE x 2.000, the factor a 1 GiB device copy gives,
+# plus WRITE_SIZE; profiles/potential_gemv_c2_slab_hbm_counters.md).  Counters
+# cannot be read inside a timed run, so the figure is reported for this
+# workload only.
+MEASURED_TRAFFIC = {(4, 19926): 777846 * 2048 + 179 * 1024,
+                    (8, 19926): 1554955 * 2048 + 185 * 1024}
 KERNELS = {"dense": ("exch_div", "potential", "demag_llg"),
            "iterative": ("exch_div", "cg_nbc", "bem", "cg_dbc", "demag_llg")}
 
@@ -84,7 +97,7 @@ def run_gpu(prec, route, coords, tets, mat, Hext, states, warmup, steps, rtol, t
     t_setup = time.perf_counter() - t0
     g.set_field("H_ext", Hext)
     g.set_field("m", states[0])
-    tuned = g.autotune()
+    tuned = g.autotune() if AUTOTUNE else {}
     log("%s/%s set-up %.2f s, autotuned: %s" % (prec, route, t_setup, tuned))
     n3 = 3 * g.n
     Y = torch.from_numpy(states.reshape(len(states), n3)).to("cuda")
@@ -394,21 +407,32 @@ def run_adjoint(t_ns, rtol, fd_checks=True):
                      "cg_unconverged": int(g.stat("cg_unconverged"))}
         log("adjoint %s: loss %.6f forward %.1f s adjoint %.1f s" % (prec, loss, t_fwd, t_adj))
         if prec == "fp64" and fd_checks:
-            rel = 1e-3
-            h = rel * Ms
-            fd_ms = (prob.run(K, Ms + h, amps) - prob.run(K, Ms - h, amps)) / (2 * h)
+            # central differences at relative step 1e-3, and Richardson's
+            # extrapolation from steps 1e-3 and 5e-4 (error O(eps^4)): over
+            # 2 ns the precession phase makes the loss strongly nonlinear in Ms,
+            # so the plain 1e-3 difference carries its own ~1e-3 error there
             k = int(np.argmax(np.abs(gA)))
-            ha = rel * np.max(np.abs(amps))
-            ap, am = amps.copy(), amps.copy()
-            ap[k] += ha
-            am[k] -= ha
-            fd_a = (prob.run(K, Ms, ap) - prob.run(K, Ms, am)) / (2 * ha)
-            out["fd_check_fp64"] = {
-                "rel_step": rel,
-                "Ms": {"adjoint": gMs, "fd": fd_ms, "rel_err": abs(gMs - fd_ms) / abs(fd_ms)},
-                "amp_%d" % k: {"adjoint": float(gA[k]), "fd": fd_a,
-                               "rel_err": abs(gA[k] - fd_a) / abs(fd_a)}}
-            log("fd check: %s" % out["fd_check_fp64"])
+
+            def fd(fun, x0, h):
+                return (fun(x0 + h) - fun(x0 - h)) / (2 * h)
+
+            def amp_fun(v):
+                a = amps.copy()
+                a[k] = v
+                return prob.run(K, Ms, a)
+
+            checks = {}
+            for name, fun, x0, scale, adj in (
+                    ("Ms", lambda v: prob.run(K, v, amps), Ms, Ms, gMs),
+                    ("amp_%d" % k, amp_fun, amps[k], np.max(np.abs(amps)), float(gA[k]))):
+                d1 = fd(fun, x0, 1e-3 * scale)
+                d2 = fd(fun, x0, 5e-4 * scale)
+                rich = (4.0 * d2 - d1) / 3.0
+                checks[name] = {"adjoint": adj, "fd_1e-3": float(d1), "fd_richardson": float(rich),
+                                "rel_err_fd_1e-3": float(abs(adj - d1) / abs(d1)),
+                                "rel_err_richardson": float(abs(adj - rich) / abs(rich))}
+            out["fd_check_fp64"] = checks
+            log("fd check: %s" % checks)
         g.close()
     rel = lambda a, b: float(np.max(np.abs(np.atleast_1d(a) - np.atleast_1d(b))) /
                              np.max(np.abs(np.atleast_1d(b))))
@@ -432,6 +456,12 @@ def main():
     ap.add_argument("--sim-window", type=float, default=20.0,
                     help="ps of LLG dynamics timed for sim-ns/hour (1000 = the full 1 ns)")
     ap.add_argument("--no-dynamics", action="store_true")
+    ap.add_argument("--dynamics-routes", default="dense,iterative",
+                    help="demag routes timed in the dynamics leg")
+    ap.add_argument("--routes", default="dense,iterative",
+                    help="demag routes timed in the field-evaluation leg (dense first)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default tile shapes (short profiler runs)")
     ap.add_argument("--no-batch", action="store_true",
                     help="skip the 64-magnet batching leg")
     ap.add_argument("--batch-steps", type=int, default=20)
@@ -440,7 +470,15 @@ def main():
     ap.add_argument("--adjoint-ns", type=float, default=2.0,
                     help="ns of the differentiation-form trajectory (0 skips)")
     ap.add_argument("--adjoint-rtol", type=float, default=1e-8)
+    ap.add_argument("--copy-mb", type=int, default=0,
+                    help="also run a plain device copy of this many MiB (HBM counter "
+                         "calibration under rocprofv3 --pmc)")
     args = ap.parse_args()
+    global AUTOTUNE
+    AUTOTUNE = not args.no_autotune
+    if not args.routes.startswith("dense"):
+        print("--routes must start with dense (the headline route)", file=sys.stderr)
+        sys.exit(2)
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(each magnet runs on a single MI355X); use --gpus 1", file=sys.stderr)
@@ -465,6 +503,8 @@ def main():
 
     results, outs = {}, {}
     for route in ("dense", "iterative"):
+        if route not in args.routes.split(","):
+            continue
         for prec in ("fp64", "mixed"):
             key = prec + "/" + route
             log("timing %s ..." % key)
@@ -484,17 +524,20 @@ def main():
     kern = mixed["kernel_ms"]
     dominant = max(kern, key=kern.get)
 
-    def roofline(kernel, alg_bytes, ms):
+    def roofline(kernel, alg_bytes, ms, traffic=None):
         ach = alg_bytes / (ms * 1e-3) / 1e9
         return {"bound": "hbm", "kernel": kernel, "achieved": round(ach, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
-                "traffic": None, "alg_bytes_per_launch": int(alg_bytes)}
+                "traffic": traffic, "alg_bytes_per_launch": int(alg_bytes)}
 
     # dense potential GEMV: the N x N operator once, rho in, phi out
     pot_bytes = lambda esz: esz * n * n + 16 * n
     rooflines = {
         "potential": roofline("k_dense_gemv<float> potential operator (N=%d)" % n,
-                              pot_bytes(4), kern["potential"]),
+                              pot_bytes(4), kern["potential"], MEASURED_TRAFFIC.get((4, n))),
+        "potential_fp64": roofline("k_dense_gemv<double> potential operator (N=%d)" % n,
+                                   pot_bytes(8), fp64["kernel_ms"]["potential"],
+                                   MEASURED_TRAFFIC.get((8, n))),
     }
     roof = rooflines.get(dominant)
     pot64_gbs = pot_bytes(8) / (fp64["kernel_ms"]["potential"] * 1e-3) / 1e9
@@ -502,21 +545,23 @@ def main():
     # bytes of one PCG iteration (SpMV over the stiffness pattern with FP64
     # values + 4-byte columns, row pointers, about eight FP64 vector passes)
     # times the measured iterations per solve; the BEM GEMV streams B once
-    it_m = results["mixed/iterative"]
-    kit = it_m["kernel_ms"]
-    cg_iter_bytes = 12 * nnz + 4 * (n + 1) + 64 * n
-    rooflines_iterative = {
-        "cg_nbc": roofline("k_cg<double> Neumann solve",
-                           cg_iter_bytes * max(it_m["cg_iters_nbc"], 1), kit["cg_nbc"]),
-        "cg_dbc": roofline("k_cg<double> Dirichlet solve",
-                           cg_iter_bytes * max(it_m["cg_iters_dbc"], 1), kit["cg_dbc"]),
-        "bem": roofline("k_dense_gemv<float> BEM", gemv_bytes(nb, 4), kit["bem"]),
-    }
+    rooflines_iterative = None
+    if "mixed/iterative" in results:
+        it_m = results["mixed/iterative"]
+        kit = it_m["kernel_ms"]
+        cg_iter_bytes = 12 * nnz + 4 * (n + 1) + 64 * n
+        rooflines_iterative = {
+            "cg_nbc": roofline("k_cg<double> Neumann solve",
+                               cg_iter_bytes * max(it_m["cg_iters_nbc"], 1), kit["cg_nbc"]),
+            "cg_dbc": roofline("k_cg<double> Dirichlet solve",
+                               cg_iter_bytes * max(it_m["cg_iters_dbc"], 1), kit["cg_dbc"]),
+            "bem": roofline("k_dense_gemv<float> BEM", gemv_bytes(nb, 4), kit["bem"]),
+        }
 
     dyn = None
     if not args.no_dynamics:
         dyn = {}
-        for route in ("dense", "iterative"):
+        for route in args.dynamics_routes.split(","):
             dyn[route] = {}
             avgs = {}
             for prec in ("fp64", "mixed"):
@@ -529,6 +574,16 @@ def main():
                 np.abs(avgs["mixed"] - avgs["fp64"]).max())
         dyn["window_ps"] = args.sim_window
         dyn["integrator"] = "BDF order<=2, Newton-GMRES, rtol=atol=1e-6, from the seeded S-state"
+
+    if args.copy_mb > 0:
+        # a copy of known size: FETCH_SIZE / WRITE_SIZE of its kernel calibrate
+        # the counters read for the potential GEMV
+        src = torch.ones(args.copy_mb * (1 << 20) // 4, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log("copy of %d MiB done (3 launches)" % args.copy_mb)
 
     batch = None
     if not args.no_batch:
@@ -579,14 +634,16 @@ def main():
         "mixed_kernel_ms": kern,
         "mixed_vs_fp64_speedup": round(mixed["evals_per_s"] / fp64["evals_per_s"], 3),
         "mixed_dmdt_max_rel_err": err,
-        "dense_vs_iterative_fp64_dmdt_rel_diff": rel(outs["fp64/dense"], outs["fp64/iterative"]),
+        "dense_vs_iterative_fp64_dmdt_rel_diff": (
+            rel(outs["fp64/dense"], outs["fp64/iterative"]) if "fp64/iterative" in outs
+            else None),
         "iterative_route": {
             key.split("/")[0]: {"value": round(results[key]["evals_per_s"], 2),
                                 "kernel_ms": results[key]["kernel_ms"],
                                 "cg_iters": [results[key]["cg_iters_nbc"],
                                              results[key]["cg_iters_dbc"]],
                                 "autotuned": results[key]["autotuned"]}
-            for key in ("fp64/iterative", "mixed/iterative")},
+            for key in ("fp64/iterative", "mixed/iterative") if key in results},
         "dominant_kernel": dominant,
         "roofline": roof,
         "roofline_all": rooflines,

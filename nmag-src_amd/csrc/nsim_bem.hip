@@ -209,8 +209,10 @@ __device__ inline void gemv_rows(const TB* __restrict__ B, int ldb, int rb, cons
 // segment's input vector is staged once per workgroup in LDS (up to 160 KiB
 // on gfx950); each wavefront streams ROWS rows at a time with 16-byte loads
 // per lane, then its remaining rows one at a time, so a wavefront's row count
-// need not be a multiple of ROWS (no row is read twice).
-template <typename TB, int ROWS, int WAVES>
+// need not be a multiple of ROWS (no row is read twice).  KIND only names the
+// instantiation after its operator (0: BEM, 1: potential) so that profiles
+// report the two GEMVs apart.
+template <typename TB, int ROWS, int WAVES, int KIND>
 __global__ __launch_bounds__(64 * WAVES) void k_dense_gemv(
     const TB* __restrict__ Abase, const long long* __restrict__ seg_aoff,
     const int* __restrict__ seg_ld, const int* __restrict__ seg_n,
@@ -238,27 +240,34 @@ __global__ __launch_bounds__(64 * WAVES) void k_dense_gemv(
 
 constexpr size_t kMaxLds = 160 * 1024;  // LDS per workgroup on gfx950
 
-template <typename TB, int ROWS, int WAVES>
+template <typename TB, int ROWS, int WAVES, int KIND>
 hipError_t launch_shape(const DenseOp& op, size_t lds, const double* x, double* y,
                         hipStream_t st) {
-  const void* k = reinterpret_cast<const void*>(&k_dense_gemv<TB, ROWS, WAVES>);
+  const void* k = reinterpret_cast<const void*>(&k_dense_gemv<TB, ROWS, WAVES, KIND>);
   if (lds > 64 * 1024) {
     const hipError_t e =
         hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
   }
-  k_dense_gemv<TB, ROWS, WAVES><<<op.grid, 64 * WAVES, lds, st>>>(
+  k_dense_gemv<TB, ROWS, WAVES, KIND><<<op.grid, 64 * WAVES, lds, st>>>(
       (const TB*)op.A, op.aoff, op.ld, op.n, op.off, op.wg_seg, op.wg_row0, op.rpw, x, op.map,
       y);
   return hipGetLastError();
 }
 
+template <typename TB, int ROWS, int KIND>
+hipError_t launch_waves(const DenseOp& op, size_t lds, const double* x, double* y,
+                        hipStream_t st) {
+  if (op.waves == 4) return launch_shape<TB, ROWS, 4, KIND>(op, lds, x, y, st);
+  if (op.waves == 16) return launch_shape<TB, ROWS, 16, KIND>(op, lds, x, y, st);
+  return hipErrorInvalidConfiguration;
+}
+
 template <typename TB, int ROWS>
 hipError_t launch_rows(const DenseOp& op, size_t lds, const double* x, double* y,
                        hipStream_t st) {
-  if (op.waves == 4) return launch_shape<TB, ROWS, 4>(op, lds, x, y, st);
-  if (op.waves == 16) return launch_shape<TB, ROWS, 16>(op, lds, x, y, st);
-  return hipErrorInvalidConfiguration;
+  return op.kind == 1 ? launch_waves<TB, ROWS, 1>(op, lds, x, y, st)
+                      : launch_waves<TB, ROWS, 0>(op, lds, x, y, st);
 }
 
 template <typename TB>

@@ -23,6 +23,7 @@ struct Coeffs {
 struct DenseOp {
   void* A = nullptr;
   int type = kBemF64;                 // storage: FP64, FP32 or BF16
+  int kind = 0;                       // 0: BEM matrix, 1: potential operator
   std::vector<int> h_n, h_ld, h_off;  // per segment: order, row stride (x64), x/map offset
   std::vector<long long> h_aoff;      // per segment: first element in A
   long long elems = 0;

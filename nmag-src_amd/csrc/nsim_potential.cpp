@@ -102,6 +102,7 @@ int build_potential(nsim_handle* h) {
   hipStream_t st = h->stream;
   DenseOp& p = d.pot;
   p.type = d.bem.type;
+  p.kind = 1;
   const size_t esz = p.type == kBemF64 ? 8 : p.type == kBemF32 ? 4 : 2;
   const int ninst = d.ninst;
   p.h_n.clear();
