@@ -79,18 +79,25 @@ Integrator::Integrator(nsim_handle* h, const TsOptions& o) : h_(h), opt_(o) {
     return hipMemset(*p, 0, count * sizeof(double)) == hipSuccess;
   };
   const size_t n = n3_;
+  np_ = vec_reduce_grid(n3_);
   ok_ = alloc(&yn_, n) && alloc(&ynm1_, n) && alloc(&ynm2_, n) && alloc(&fn_, n) &&
         alloc(&yp_, n) && alloc(&ycur_, n) && alloc(&avec_, n) && alloc(&f_, n) &&
         alloc(&res_, n) && alloc(&del_, n) && alloc(&tmp_, n) && alloc(&tmp2_, n) &&
         alloc(&ewt_, n) && alloc(&ystart_, n) &&
         alloc(&V_, n * (size_t)(opt_.krylov_max + 1)) && alloc(&Minv_, 9 * (size_t)nn_) &&
-        alloc(&coef_, 64) && alloc(&part_, 8 * (size_t)kReduceBlocks);
+        alloc(&coef_, (size_t)opt_.krylov_max + 8) &&
+        alloc(&hdev_, 2 * (size_t)opt_.krylov_max + 8) &&
+        alloc(&part_, ((size_t)opt_.krylov_max + 8) * kReduceBlocks);
   host_part_.resize(8 * kReduceBlocks);
+  if (ok_ && hipHostMalloc(reinterpret_cast<void**>(&hhost_),
+                           (2 * (size_t)opt_.krylov_max + 8) * sizeof(double)) != hipSuccess)
+    ok_ = false;
 }
 
 Integrator::~Integrator() {
   (void)hipStreamSynchronize(s_);
   for (void* p : allocs_) (void)hipFree(p);
+  if (hhost_) (void)hipHostFree(hhost_);
 }
 
 int Integrator::reduce(int nparts, int k, std::vector<double>* out) {
@@ -167,26 +174,28 @@ int Integrator::gmres(const double* r, double* x, int* iters) {
     else HTRY(hipMemcpyAsync(del_, tmp_, sizeof(double) * n3_, hipMemcpyDeviceToDevice, s_));
     TRY(apply_A(del_, tmp_));
     HTRY(vec_mul(s_, n3_, tmp_, ewt_, false, Vn));
-    for (int pass = 0; pass < 2; pass++) {  // classical Gram-Schmidt, twice
-      std::vector<double> hcol(j + 1);
+    // classical Gram-Schmidt, twice; the projections are summed and applied on
+    // the device, and the host reads the two coefficient columns and the new
+    // norm with one synchronisation per Arnoldi step
+    for (int pass = 0; pass < 2; pass++) {
       for (int c0 = 0; c0 <= j; c0 += 8) {
         const int kc = std::min(8, j + 1 - c0);
-        HTRY(vec_multidot_partials(s_, n3_, kc, V_ + c0 * ld, ld, Vn, part_, &np));
-        TRY(reduce(np, kc, &red));
-        for (int i = 0; i < kc; i++) hcol[c0 + i] = red[i];
+        int npc = 0;  // always vec_reduce_grid(n3_)
+        HTRY(vec_multidot_partials(s_, n3_, kc, V_ + c0 * ld, ld, Vn, part_ + c0 * np_, &npc));
       }
+      HTRY(vec_reduce_parts(s_, np_, j + 1, part_, hdev_ + pass * (j + 1), coef_));
       for (int c0 = 0; c0 <= j; c0 += 64) {
         const int kc = std::min(64, j + 1 - c0);
-        std::vector<double> neg(kc);
-        for (int i = 0; i < kc; i++) neg[i] = -hcol[c0 + i];
-        HTRY(hipMemcpyAsync(coef_, neg.data(), sizeof(double) * kc, hipMemcpyHostToDevice, s_));
-        HTRY(vec_axpy_multi(s_, n3_, kc, V_ + c0 * ld, ld, coef_, Vn));
+        HTRY(vec_axpy_multi(s_, n3_, kc, V_ + c0 * ld, ld, coef_ + c0, Vn));
       }
-      for (int i = 0; i <= j; i++) H[i * maxl + j] += hcol[i];
     }
     HTRY(vec_wsq_partials(s_, n3_, Vn, nullptr, part_, &np));
-    TRY(reduce(np, 1, &red));
-    const double hn = std::sqrt(red[0]);
+    HTRY(vec_reduce_parts(s_, np, 1, part_, hdev_ + 2 * (j + 1), nullptr));
+    HTRY(hipMemcpyAsync(hhost_, hdev_, sizeof(double) * (2 * (j + 1) + 1),
+                        hipMemcpyDeviceToHost, s_));
+    HTRY(hipStreamSynchronize(s_));
+    for (int i = 0; i <= j; i++) H[i * maxl + j] += hhost_[i] + hhost_[j + 1 + i];
+    const double hn = std::sqrt(hhost_[2 * (j + 1)]);
     H[(j + 1) * maxl + j] = hn;
     if (hn > 0.0)
       HTRY(vec_lincomb(s_, n3_, 1.0 / hn, Vn, 0, nullptr, 0, nullptr, 0, nullptr, Vn));

@@ -82,6 +82,9 @@ class Integrator {
   double* Minv_ = nullptr;  // 9 per node
   double* coef_ = nullptr;  // device coefficients for multi-axpy
   double* part_ = nullptr;  // reduction partials
+  double* hdev_ = nullptr;  // device-reduced Gram-Schmidt coefficients and norm
+  double* hhost_ = nullptr; // their pinned host copy
+  int np_ = 0;              // partials per reduction at this vector length
   std::vector<double> host_part_;
   double gamma_ = 0.0;      // current Newton matrix coefficient
   double crate_ = 1.0;      // Newton convergence-rate estimate

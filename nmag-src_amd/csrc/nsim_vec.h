@@ -24,6 +24,12 @@ hipError_t vec_multidot_partials(hipStream_t st, int n, int k, const double* V, 
                                  const double* y, double* part, int* nparts);
 hipError_t vec_axpy_multi(hipStream_t st, int n, int k, const double* V, size_t ld,
                           const double* c_dev, double* y);
+// Number of partials the *_partials reductions produce at length n.
+int vec_reduce_grid(int n);
+// Device-side sum of k reductions' partials (part[j * np + b]) into out[j],
+// and -out[j] into neg (may be null).
+hipError_t vec_reduce_parts(hipStream_t st, int np, int k, const double* part, double* out,
+                            double* neg);
 hipError_t vec_maxdist_partials(hipStream_t st, int nn, const double* a, const double* b,
                                 double* part, int* nparts);
 // ct: per-magnet coefficient table, inst: node -> magnet (null: one magnet)

@@ -170,7 +170,29 @@ inline int grid_for(int n) {
   return g < kReduceBlocks ? g : kReduceBlocks;
 }
 
+// out[j] = sum_b part[j * np + b] in a fixed order, neg[j] = -out[j] (neg may
+// be null): the device-side end of a reduction whose result feeds further
+// device work without a host round trip.
+__global__ __launch_bounds__(256) void k_reduce_parts(int np, int k, const double* part,
+                                                      double* out, double* neg) {
+  for (int j = threadIdx.x; j < k; j += 256) {
+    double s = 0.0;
+    for (int b = 0; b < np; b++) s += part[(size_t)j * np + b];
+    out[j] = s;
+    if (neg) neg[j] = -s;
+  }
+}
+
 }  // namespace
+
+int vec_reduce_grid(int n) { return grid_for(n); }
+
+hipError_t vec_reduce_parts(hipStream_t st, int np, int k, const double* part, double* out,
+                            double* neg) {
+  if (k <= 0) return hipSuccess;
+  k_reduce_parts<<<1, 256, 0, st>>>(np, k, part, out, neg);
+  return hipGetLastError();
+}
 
 hipError_t vec_lincomb(hipStream_t st, int n, double c0, const double* x0, double c1,
                        const double* x1, double c2, const double* x2, double c3,
