@@ -106,6 +106,7 @@ int setup_gemv(nsim_handle* h, DenseOp& op, int rows_per_pass, int wg_per_cu, in
     (void)hipFree(dr);
     return fail(std::string("GEMV plan upload: ") + hipGetErrorString(e));
   }
+  drop_rhs_graphs(h);  // they hold the old plan's tables and launch shape
   (void)hipStreamSynchronize(h->stream);
   (void)hipFree(op.wg_seg);
   (void)hipFree(op.wg_row0);
@@ -166,8 +167,99 @@ FieldArgs field_args(nsim_handle* h, const double* m) {
 namespace nsim {
 
 // Full field evaluation for magnetisation m (device) into dmdt (device).
+namespace {
+
+// The dense route's three launches, as recorded into a graph.
+hipError_t dense_rhs_launches(nsim_handle* h, const double* m, double* dmdt, bool demag) {
+  hipError_t e = launch_exch_div(h->d, h->mixed, m, h->Hexch, h->rho, demag, h->stream);
+  if (e == hipSuccess && demag) e = launch_dense_gemv(h->d.pot, h->rho, h->phi, h->stream);
+  if (e != hipSuccess) return e;
+  FieldBufs b{h->Hexch, h->Hdemag, h->Hanis, h->Htotal, h->rho,
+              h->phi1,  h->phi2,   h->phi,   h->phi1b};
+  return launch_demag_llg(h->d, h->mixed, field_args(h, m), b, dmdt, demag, h->stream);
+}
+
+bool same_args(const FieldArgs& a, const FieldArgs& b) {
+  return a.m == b.m && a.Hext == b.Hext && a.k1 == b.k1 && a.k2 == b.k2 && a.axis == b.axis &&
+         a.pin == b.pin;
+}
+
+// Replays the captured evaluation for (m, dmdt) or captures it first.  Graph
+// launches carry the same kernels and arguments as the eager path, so results
+// are bit-identical; what they save is the per-launch host work.
+int graph_rhs(nsim_handle* h, const double* m, double* dmdt, bool demag) {
+  const FieldArgs f = field_args(h, m);
+  auto same = [&](const RhsGraph& g) {
+    return g.m == m && g.dmdt == dmdt && g.demag == demag && same_args(g.f, f);
+  };
+  for (RhsGraph& g : h->graphs)
+    if (same(g)) {
+      HIPCHK(hipGraphLaunch(g.exec, h->stream));
+      h->graph_launches++;
+      return 0;
+    }
+  const auto it = std::find_if(h->seen.begin(), h->seen.end(), same);
+  if (it == h->seen.end()) {
+    // first sight of this key: evaluate eagerly and remember it
+    if (h->seen.size() >= kMaxRhsSeen) h->seen.erase(h->seen.begin());
+    RhsGraph k;
+    k.m = m;
+    k.dmdt = dmdt;
+    k.f = f;
+    k.demag = demag;
+    h->seen.push_back(k);
+    HIPCHK(dense_rhs_launches(h, m, dmdt, demag));
+    return 0;
+  }
+  h->seen.erase(it);
+  if (h->graphs.size() >= kMaxRhsGraphs) {
+    HIPCHK(hipStreamSynchronize(h->stream));
+    (void)hipGraphExecDestroy(h->graphs.front().exec);
+    h->graphs.erase(h->graphs.begin());
+  }
+  // This evaluation runs eagerly (capture only records); the eager launches
+  // also set the GEMV's LDS attribute, so nothing but kernels is recorded.
+  HIPCHK(dense_rhs_launches(h, m, dmdt, demag));
+  HIPCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+  const hipError_t le = dense_rhs_launches(h, m, dmdt, demag);
+  hipGraph_t graph = nullptr;
+  const hipError_t ce = hipStreamEndCapture(h->stream, &graph);
+  if (le != hipSuccess || ce != hipSuccess) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return fail(std::string("graph capture of the field evaluation failed: ") +
+                hipGetErrorString(le != hipSuccess ? le : ce));
+  }
+  RhsGraph g;
+  g.m = m;
+  g.dmdt = dmdt;
+  g.f = f;
+  g.demag = demag;
+  const hipError_t ie = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  HIPCHK(ie);
+  h->graphs.push_back(g);
+  h->graph_captures++;
+  return 0;
+}
+
+}  // namespace
+
+void drop_rhs_graphs(nsim_handle* h) {
+  h->seen.clear();
+  if (h->graphs.empty()) return;
+  (void)hipStreamSynchronize(h->stream);
+  for (RhsGraph& g : h->graphs) (void)hipGraphExecDestroy(g.exec);
+  h->graphs.clear();
+}
+
 int device_rhs(nsim_handle* h, const double* m, double* dmdt) {
   const bool demag = h->opt.do_demag != 0 && h->d.nb > 0;
+  if (h->rhs_graph && !h->timers_on && (!demag || h->d.pot.A)) {
+    // the eager first evaluation inside graph_rhs counts as the same one
+    const int rc = graph_rhs(h, m, dmdt, demag);
+    if (rc == 0) h->rhs_evals++;
+    return rc;
+  }
   {
     Span s(h, "exch_div");
     HIPCHK(launch_exch_div(h->d, h->mixed, m, h->Hexch, h->rho, demag, h->stream));
@@ -541,6 +633,7 @@ void nsim_destroy(nsim_handle* h) {
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   delete h->ts;
   destroy_adjoint(h->adj);
+  drop_rhs_graphs(h);
   for (auto& kv : h->timers)
     for (auto& ev : kv.second.pending) {
       (void)hipEventDestroy(ev.first);
@@ -578,6 +671,11 @@ int nsim_set_tuning(nsim_handle* h, const char* key, int value) {
   }
   if (k == "cg_coarse") {
     h->cg_coarse = value != 0;
+    return 0;
+  }
+  if (k == "rhs_graph") {
+    h->rhs_graph = value != 0;
+    if (!h->rhs_graph) drop_rhs_graphs(h);
     return 0;
   }
   DenseOp& b = h->d.bem;
@@ -842,6 +940,10 @@ int nsim_get_stat(nsim_handle* h, const char* name, double* out) {
   std::string s(name);
   if (s == "rhs_evals") {
     *out = (double)h->rhs_evals;
+    return 0;
+  }
+  if (s == "graph_launches" || s == "graph_captures") {
+    *out = (double)(s == "graph_launches" ? h->graph_launches : h->graph_captures);
     return 0;
   }
   if (s == "cg_grid") {

@@ -21,6 +21,17 @@ struct Timer {
   long count = 0;
 };
 
+// One dense-route field evaluation captured as a HIP graph (exchange and
+// divergence, potential GEMV, demag + LLG), keyed by everything its kernel
+// arguments depend on.
+struct RhsGraph {
+  const double* m = nullptr;
+  double* dmdt = nullptr;
+  FieldArgs f{};
+  bool demag = false;
+  hipGraphExec_t exec = nullptr;
+};
+
 struct AdjointState;  // buffers of the adjoint (nsim_traj.cpp)
 void destroy_adjoint(AdjointState* a);
 
@@ -54,6 +65,10 @@ struct nsim_handle {
   double *io_a = nullptr, *io_b = nullptr, *io_c = nullptr;  // staging for host-pointer calls
   bool have_hext = false, have_anis = false, have_k2 = false, have_pin = false;
   long rhs_evals = 0;
+  bool rhs_graph = true;               // replay captured dense-route evaluations
+  std::vector<nsim::RhsGraph> graphs;  // at most kMaxRhsGraphs, oldest first
+  std::vector<nsim::RhsGraph> seen;    // keys evaluated once, not captured yet
+  long graph_launches = 0, graph_captures = 0;
   bool timers_on = false;
   std::map<std::string, nsim::Timer> timers;
   nsim::Integrator* ts = nullptr;
@@ -73,8 +88,16 @@ namespace nsim {
 // Memory the dense potential operators of a handle may take.
 constexpr double kPotentialBudgetBytes = 32.0 * 1024 * 1024 * 1024;
 
+// Captured evaluations kept per handle (one per (m, dmdt) buffer pair in use).
+// A pair is captured on its second evaluation, so callers that stream through
+// many buffers once each (a sweep over stored states) stay on eager launches.
+constexpr size_t kMaxRhsGraphs = 8;
+constexpr size_t kMaxRhsSeen = 16;
+
 // Shared by the C-ABI translation units.
 int device_rhs(nsim_handle* h, const double* m, double* dmdt);
+// Destroys the captured evaluations; called whenever a launch shape changes.
+void drop_rhs_graphs(nsim_handle* h);
 // Builds the dense demag potential operator of every magnet (nsim_dense.cpp).
 int build_potential(nsim_handle* h);
 // Re-uploads the per-magnet coefficient table and the exchange diagonal kappa
