@@ -325,6 +325,9 @@ const double* handle_Htotal(nsim_handle* h) { return h->Htotal; }
 const double* handle_kappa(nsim_handle* h) { return h->kappa; }
 const Coeffs* handle_coef_table(nsim_handle* h) { return h->d.coefs; }
 const int* handle_inst(nsim_handle* h) { return h->d.inst; }
+const DeviceOps& handle_ops(nsim_handle* h) { return h->d; }
+bool handle_mixed(nsim_handle* h) { return h->mixed; }
+int handle_device(nsim_handle* h) { return h->device; }
 int handle_error_flags(nsim_handle* h, unsigned* out) {
   unsigned a[6], b[6];
   HIPCHK(hipMemcpyAsync(a, h->sync_nbc, sizeof(a), hipMemcpyDeviceToHost, h->stream));
@@ -671,6 +674,11 @@ int nsim_set_tuning(nsim_handle* h, const char* key, int value) {
   }
   if (k == "cg_coarse") {
     h->cg_coarse = value != 0;
+    return 0;
+  }
+  if (k == "gmres_device") {
+    h->ts_opt.device_gmres = value != 0;
+    if (h->ts) h->ts->options().device_gmres = h->ts_opt.device_gmres;
     return 0;
   }
   if (k == "rhs_graph") {

@@ -64,6 +64,22 @@ __device__ inline unsigned ld_coh(const unsigned* p) {
   return __hip_atomic_load(const_cast<unsigned*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// One row of the jacobi_ts1 Jacobian-times-vector (nmag_lam.py:432-436,
+// src/nsim.ml:1004-1042): with H_total frozen except for its exchange part h,
+//   Jv = c1 (v x H + m x h) + c2 (v x (m x H) + m x (v x H) + m x (m x h)).
+__device__ inline void llg_jv_row(double c1, double c2, const double* h, const double* m,
+                                  const double* v, const double* H, double* out) {
+  double vxH[3], mxh[3], mxH[3], vxmxH[3], mxvxH[3], mxmxh[3];
+  cross(v, H, vxH);
+  cross(m, h, mxh);
+  cross(m, H, mxH);
+  cross(v, mxH, vxmxH);
+  cross(m, vxH, mxvxH);
+  cross(m, mxh, mxmxh);
+  for (int d = 0; d < 3; d++)
+    out[d] = c1 * (vxH[d] + mxh[d]) + c2 * (vxmxH[d] + mxvxH[d] + mxmxh[d]);
+}
+
 // FP32 -> BF16 with round-to-nearest-even (NaN kept a quiet NaN).
 __device__ inline uint16_t f32_to_bf16(float f) {
   uint32_t u = __float_as_uint(f);

@@ -92,6 +92,34 @@ Integrator::Integrator(nsim_handle* h, const TsOptions& o) : h_(h), opt_(o) {
   if (ok_ && hipHostMalloc(reinterpret_cast<void**>(&hhost_),
                            (2 * (size_t)opt_.krylov_max + 8) * sizeof(double)) != hipSuccess)
     ok_ = false;
+  // persistent GMRES: one co-resident grid over the SELL slices
+  const DeviceOps& d = handle_ops(h);
+  int grid = 0, spw = 0;
+  if (ok_ && gmres_grid(d.nslices, opt_.krylov_max, handle_device(h), 1 << 20, &grid, &spw)) {
+    gw_.V = V_;
+    gw_.gmax = grid;
+    gw_.pstride = (opt_.krylov_max + 8) & ~7;
+    const size_t maxl = opt_.krylov_max;
+    bool g = alloc(&gw_.zp, n) && alloc(&gw_.part, 2 * (size_t)grid * gw_.pstride) &&
+             alloc(&gw_.R, maxl * maxl) && alloc(&gw_.y, maxl) && alloc(&gw_.status, 8);
+    if (g && hipMalloc(reinterpret_cast<void**>(&gw_.tag),
+                       2 * (size_t)grid * sizeof(unsigned long long)) == hipSuccess) {
+      allocs_.push_back(gw_.tag);
+      g = hipMemset(gw_.tag, 0, 2 * (size_t)grid * sizeof(unsigned long long)) == hipSuccess;
+    } else {
+      g = false;
+    }
+    if (g && hipMalloc(reinterpret_cast<void**>(&gw_.err), sizeof(unsigned)) == hipSuccess) {
+      allocs_.push_back(gw_.err);
+      g = hipMemset(gw_.err, 0, sizeof(unsigned)) == hipSuccess;
+    } else {
+      g = false;
+    }
+    if (g) {
+      ggrid_ = grid;
+      gspw_ = spw;
+    }
+  }
 }
 
 Integrator::~Integrator() {
@@ -146,7 +174,9 @@ int Integrator::apply_A(const double* z, double* w) {
 
 // Right-preconditioned GMRES in the space scaled by the error weights, so the
 // Euclidean residual is sqrt(n) times its WRMS norm.  x = (I - gamma J)^-1 r.
-int Integrator::gmres(const double* r, double* x, int* iters) {
+int Integrator::gmres(const double* r, double* x, int* iters, double* xw2) {
+  if (opt_.device_gmres && ggrid_ > 0) return gmres_device(r, x, iters, xw2);
+  *xw2 = -1.0;  // not formed here
   const int maxl = opt_.krylov_max;
   const size_t ld = n3_;
   double* V0 = V_;
@@ -236,6 +266,27 @@ int Integrator::gmres(const double* r, double* x, int* iters) {
   return 0;
 }
 
+// The same solve as one persistent launch (nsim_gmres.hip); the host reads
+// the iteration count and the correction's weighted norm with one
+// synchronisation.
+int Integrator::gmres_device(const double* r, double* x, int* iters, double* xw2) {
+  const double tol = kLinCoef * kNewtonCoef * std::sqrt((double)n3_);
+  HTRY(launch_gmres(handle_ops(h_), handle_mixed(h_), gw_, ggrid_, gspw_, opt_.krylov_max,
+                    gamma_, tol, ycur_, handle_Htotal(h_), ewt_,
+                    opt_.precond ? Minv_ : nullptr, r, x, s_));
+  HTRY(hipMemcpyAsync(hhost_, gw_.status, 3 * sizeof(double), hipMemcpyDeviceToHost, s_));
+  HTRY(hipStreamSynchronize(s_));
+  if (hhost_[2] != 0.0) {
+    (void)hipMemset(gw_.err, 0, sizeof(unsigned));
+    set_error("integrator: GMRES exchange timed out");
+    return -1;
+  }
+  *iters = (int)hhost_[0];
+  *xw2 = hhost_[1];
+  st_.linear += *iters;
+  return 0;
+}
+
 // Predictor into yp; E_p is the predictor's error constant (coefficient of
 // y^(q+1), with y_pred - y_exact = -E_p y^(q+1)).
 int Integrator::predictor(double h, int q, double* yp, double* E_p) {
@@ -295,9 +346,10 @@ int Integrator::step_attempt(double h, int q, bool* accepted, double* err_out) {
       HTRY(bjac_setup(s_, nn_, handle_coef_table(h_), handle_inst(h_), gamma_, ycur_,
                       handle_Htotal(h_), handle_kappa(h_), Minv_));
     int its = 0;
-    TRY(gmres(res_, del_, &its));
+    double xw2 = -1.0;
+    TRY(gmres(res_, del_, &its, &xw2));
     HTRY(vec_lincomb(s_, n3_, 1.0, ycur_, 1.0, del_, 0, nullptr, 0, nullptr, ycur_));
-    const double d = wrms(del_);
+    const double d = xw2 >= 0.0 ? std::sqrt(xw2 / n3_) : wrms(del_);
     if (!std::isfinite(d)) break;
     if (k > 0) crate = std::max(0.3 * crate, d / del_old);
     if (d * std::min(1.0, crate) * C <= kNewtonCoef) {

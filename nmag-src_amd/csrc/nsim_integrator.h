@@ -10,6 +10,8 @@
 #include <string>
 #include <vector>
 
+#include "nsim_gmres.h"
+
 struct nsim_handle;
 
 namespace nsim {
@@ -25,6 +27,9 @@ const double* handle_kappa(nsim_handle* h);
 struct Coeffs;
 const Coeffs* handle_coef_table(nsim_handle* h);  // device, one entry per magnet
 const int* handle_inst(nsim_handle* h);           // device node -> magnet, or null
+const DeviceOps& handle_ops(nsim_handle* h);
+bool handle_mixed(nsim_handle* h);
+int handle_device(nsim_handle* h);
 int handle_error_flags(nsim_handle* h, unsigned* out);
 void set_error(const std::string& msg);
 
@@ -33,6 +38,8 @@ struct TsOptions {
   int max_order = 2;                // timestepper_max_order (main.py:838)
   int krylov_max = 300;             // timestepper_krylov_max (main.py:839)
   int precond = 1;                  // 0 none, 1 block Jacobi of I - gamma J
+  int device_gmres = 1;             // 1: one persistent GMRES launch per solve,
+                                    // 0: host-driven Arnoldi steps
 };
 
 struct TsStats {
@@ -56,7 +63,9 @@ class Integrator {
 
  private:
   int step_attempt(double h, int q, bool* accepted, double* err_out);
-  int gmres(const double* r, double* x, int* iters);
+  // x = (I - gamma J)^-1 r; *xw2 = sum (x ewt)^2
+  int gmres(const double* r, double* x, int* iters, double* xw2);
+  int gmres_device(const double* r, double* x, int* iters, double* xw2);
   int apply_A(const double* z, double* w);
   double wrms(const double* x);
   int reduce(int nparts, int k, std::vector<double>* out);
@@ -85,6 +94,8 @@ class Integrator {
   double* hdev_ = nullptr;  // device-reduced Gram-Schmidt coefficients and norm
   double* hhost_ = nullptr; // their pinned host copy
   int np_ = 0;              // partials per reduction at this vector length
+  GmresWork gw_;            // persistent GMRES buffers (V shared with V_)
+  int ggrid_ = 0, gspw_ = 0;  // its grid; 0: not available
   std::vector<double> host_part_;
   double gamma_ = 0.0;      // current Newton matrix coefficient
   double crate_ = 1.0;      // Newton convergence-rate estimate

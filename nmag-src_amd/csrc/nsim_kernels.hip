@@ -207,15 +207,9 @@ __global__ __launch_bounds__(256) void k_jtimes(int n, const int* __restrict__ o
   const double mi[3] = {m[3 * row], m[3 * row + 1], m[3 * row + 2]};
   const double vi[3] = {v[3 * row], v[3 * row + 1], v[3 * row + 2]};
   const double H[3] = {Ht[3 * row], Ht[3 * row + 1], Ht[3 * row + 2]};
-  double vxH[3], mxh[3], mxH[3], vxmxH[3], mxvxH[3], mxmxh[3];
-  cross(vi, H, vxH);
-  cross(mi, h, mxh);
-  cross(mi, H, mxH);
-  cross(vi, mxH, vxmxH);
-  cross(mi, vxH, mxvxH);
-  cross(mi, mxh, mxmxh);
-  for (int d = 0; d < 3; d++)
-    Jv[3 * row + d] = c.c1 * (vxH[d] + mxh[d]) + c.c2 * (vxmxH[d] + mxvxH[d] + mxmxh[d]);
+  double out[3];
+  llg_jv_row(c.c1, c.c2, h, mi, vi, H, out);
+  for (int d = 0; d < 3; d++) Jv[3 * row + d] = out[d];
 }
 
 }  // namespace

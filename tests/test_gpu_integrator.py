@@ -148,3 +148,29 @@ def test_psetup_psolve_blocks_match_the_jacobian_diagonal():
         assert np.allclose(M @ z[i], r[i], rtol=1e-12, atol=1e-12)
     with pytest.raises(Exception):
         g.psolve(r.reshape(-1), gamma, lr=2)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_persistent_gmres_matches_host_driven_gmres(precision):
+    """The one-launch GMRES and the host-driven Arnoldi loop solve the same
+    Newton systems to the same tolerance: the trajectories agree to far below
+    the integration tolerance and take the same numbers of steps."""
+    mat = material(alpha=0.1)
+    coords, tets = kuhn_cuboid(10, 6, 3, 2.5)
+    g = NativeMagnet(coords, tets, mat.su_coeffs(), precision=precision, potential="dense")
+    n = len(coords)
+    m0 = random_m(n, 11)
+    g.set_field("H_ext", np.tile([0.0, 0.02, 0.01], (n, 1)))
+    res = {}
+    for dev in (0, 1):
+        g.tune("gmres_device", dev)
+        g.set_field("m", m0)
+        g.ts_init(0.0, rtol=1e-7, atol=1e-7)
+        assert g.advance(20.0) == pytest.approx(20.0)
+        res[dev] = (g.get_field("m"),
+                    {k: g.stat("ts_" + k) for k in ("steps", "newton", "linear")})
+    s0, s1 = res[0][1], res[1][1]
+    assert s0["steps"] > 20
+    assert abs(s0["steps"] - s1["steps"]) <= max(2, 0.02 * s0["steps"])
+    assert abs(s0["linear"] - s1["linear"]) <= 0.05 * s0["linear"] + 5
+    assert np.abs(res[0][0] - res[1][0]).max() < 1e-6
