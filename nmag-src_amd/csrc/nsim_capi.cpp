@@ -681,6 +681,11 @@ int nsim_set_tuning(nsim_handle* h, const char* key, int value) {
     if (h->ts) h->ts->options().device_gmres = h->ts_opt.device_gmres;
     return 0;
   }
+  if (k == "gmres_slices_per_wg") {  // takes effect at the next nsim_ts_init
+    if (value < 1 || value > 64) return fail("gmres_slices_per_wg must be 1..64");
+    h->ts_opt.gmres_spw = value;
+    return 0;
+  }
   if (k == "rhs_graph") {
     h->rhs_graph = value != 0;
     if (!h->rhs_graph) drop_rhs_graphs(h);
@@ -979,6 +984,7 @@ int nsim_get_stat(nsim_handle* h, const char* name, double* out) {
     else if (s == "ts_order") *out = t.q_last;
     else if (s == "ts_t") *out = t.t;
     else if (s == "ts_max_dm") *out = t.max_dm;
+    else if (s == "ts_gmres_grid") *out = h->ts->gmres_grid();
     else return fail("nsim_get_stat: unknown statistic '" + s + "'");
     return 0;
   }
@@ -1044,7 +1050,8 @@ int nsim_ts_init(nsim_handle* h, double t0, double rtol, double atol, int max_or
   o.atol = atol;
   if (max_order > 0) o.max_order = max_order;
   if (krylov_max > 0) o.krylov_max = krylov_max;
-  if (!h->ts || o.krylov_max != h->ts_opt.krylov_max) {
+  if (!h->ts || o.krylov_max != h->ts->options().krylov_max ||
+      o.gmres_spw != h->ts->options().gmres_spw) {
     delete h->ts;
     h->ts = new Integrator(h, o);
   } else {

@@ -73,18 +73,6 @@ struct Shared {
 };
 
 constexpr unsigned kSpinLimit = 1u << 20;
-constexpr int kBufferConfig = 0x00020000;  // raw buffer, gfx9 family
-constexpr int kSc1 = 16;                   // cache policy: device-coherent
-
-__device__ inline __amdgpu_buffer_rsrc_t make_rsrc(const void* p, unsigned bytes) {
-  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes,
-                                           kBufferConfig);
-}
-
-// Device-coherent load of element `idx` (out-of-range indices read 0).
-__device__ inline double load_coh(__amdgpu_buffer_rsrc_t r, int idx) {
-  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, idx * 8, 0, kSc1));
-}
 
 // Row slots of one thread: kept in registers (MAXW > 0) or streamed (MAXW == 0).
 // `free_mask` bit k marks a slot whose column is an unknown (receives the

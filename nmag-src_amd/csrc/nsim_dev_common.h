@@ -64,6 +64,21 @@ __device__ inline unsigned ld_coh(const unsigned* p) {
   return __hip_atomic_load(const_cast<unsigned*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Raw buffer resources for device-coherent loads that the compiler may batch
+// (unlike atomic loads, which it issues one at a time).
+constexpr int kBufferConfig = 0x00020000;  // raw buffer, gfx9 family
+constexpr int kSc1 = 16;                   // cache policy: device-coherent
+
+__device__ inline __amdgpu_buffer_rsrc_t make_rsrc(const void* p, unsigned bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes,
+                                           kBufferConfig);
+}
+
+// Device-coherent load of element `idx` (out-of-range indices read 0).
+__device__ inline double load_coh(__amdgpu_buffer_rsrc_t r, int idx) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, idx * 8, 0, kSc1));
+}
+
 // One row of the jacobi_ts1 Jacobian-times-vector (nmag_lam.py:432-436,
 // src/nsim.ml:1004-1042): with H_total frozen except for its exchange part h,
 //   Jv = c1 (v x H + m x h) + c2 (v x (m x H) + m x (v x H) + m x (m x h)).
@@ -78,6 +93,35 @@ __device__ inline void llg_jv_row(double c1, double c2, const double* h, const d
   cross(m, mxh, mxmxh);
   for (int d = 0; d < 3; d++)
     out[d] = c1 * (vxH[d] + mxh[d]) + c2 * (vxmxH[d] + mxvxH[d] + mxmxh[d]);
+}
+
+// Block-Jacobi preconditioner of one node: M = I - gamma J_ii, J_ii the 3x3
+// diagonal block of the linearisation above (H frozen, dH_i/dm_i = kappa I,
+// kappa = negJ K_ii / V_i); writes M^-1 row-major to o[9].
+__device__ inline void bjac_block(double c1, double c2, double gamma, double kappa,
+                                  const double* m, const double* H, double* o) {
+  double M[3][3];
+  for (int col = 0; col < 3; col++) {
+    double v[3] = {0.0, 0.0, 0.0};
+    v[col] = 1.0;
+    const double h[3] = {kappa * v[0], kappa * v[1], kappa * v[2]};
+    double J[3];
+    llg_jv_row(c1, c2, h, m, v, H, J);
+    for (int r = 0; r < 3; r++) M[r][col] = (r == col ? 1.0 : 0.0) - gamma * J[r];
+  }
+  const double det = M[0][0] * (M[1][1] * M[2][2] - M[1][2] * M[2][1]) -
+                     M[0][1] * (M[1][0] * M[2][2] - M[1][2] * M[2][0]) +
+                     M[0][2] * (M[1][0] * M[2][1] - M[1][1] * M[2][0]);
+  const double id = 1.0 / det;
+  o[0] = (M[1][1] * M[2][2] - M[1][2] * M[2][1]) * id;
+  o[1] = (M[0][2] * M[2][1] - M[0][1] * M[2][2]) * id;
+  o[2] = (M[0][1] * M[1][2] - M[0][2] * M[1][1]) * id;
+  o[3] = (M[1][2] * M[2][0] - M[1][0] * M[2][2]) * id;
+  o[4] = (M[0][0] * M[2][2] - M[0][2] * M[2][0]) * id;
+  o[5] = (M[0][2] * M[1][0] - M[0][0] * M[1][2]) * id;
+  o[6] = (M[1][0] * M[2][1] - M[1][1] * M[2][0]) * id;
+  o[7] = (M[0][1] * M[2][0] - M[0][0] * M[2][1]) * id;
+  o[8] = (M[0][0] * M[1][1] - M[0][1] * M[1][0]) * id;
 }
 
 // FP32 -> BF16 with round-to-nearest-even (NaN kept a quiet NaN).

@@ -15,7 +15,7 @@ constexpr int kGmresMaxl = 512;  // largest Krylov dimension the kernel holds
 // Device buffers of the solver (owned by the caller).
 struct GmresWork {
   double* V = nullptr;        // (maxl + 1) * n3 Krylov basis, stride n3
-  double* zp = nullptr;       // n3: the preconditioned direction, exchanged
+  double* Z = nullptr;        // maxl * n3 preconditioned directions, exchanged
   double* part = nullptr;     // 2 * gmax * pstride partial sums
   unsigned long long* tag = nullptr;  // 2 * gmax phase tags
   double* R = nullptr;        // maxl * maxl triangular factor (workgroup 0)
@@ -27,17 +27,30 @@ struct GmresWork {
 };
 
 // Grid of the kernel for a handle with `nslices` SELL slices: workgroups
-// (all co-resident) and slices per workgroup.  Returns false if no grid fits.
-bool gmres_grid(int nslices, int maxl, int device, int gmax, int* grid, int* spw);
+// (all co-resident, at most gmax) and slices per workgroup (at least
+// min_spw).  Returns false if no grid fits.
+bool gmres_grid(int nslices, int maxl, int device, int gmax, int min_spw, int* grid, int* spw);
 
-// x = (I - gamma J)^-1 r by right-preconditioned GMRES in the space scaled by
-// ewt, J the jacobi_ts1 linearisation at (m, Ht); Minv: 3x3 block-Jacobi
-// inverse per node, or null.  Stops at |residual| <= tol (scaled Euclidean) or
-// after maxl iterations.  Writes status[0] = iterations, status[1] =
-// sum_i (x_i ewt_i)^2; status[2] != 0 if an exchange timed out.
-hipError_t launch_gmres(const DeviceOps& d, bool mixed, GmresWork& w, int grid, int spw,
-                        int maxl, double gamma, double tol, const double* m, const double* Ht,
-                        const double* ewt, const double* Minv, const double* r, double* x,
-                        hipStream_t st);
+// One Newton iteration of a BDF step, y - gamma f(y) - avec = 0.
+struct NewtonSystem {
+  double* ycur;         // iterate; receives ycur + x
+  const double* Ht;     // H_total of the rhs evaluated at ycur
+  const double* f;      // that rhs
+  const double* avec;
+  const double* yp;     // the step's predictor
+  const double* ewt;    // error weights
+  const double* kappa;  // negJ K_ii / V_i per node
+  double* Minv;         // 9 per node: block-Jacobi inverses set up here, or null
+  double* x;            // the correction
+  double gamma, tol;
+};
+
+// Solves (I - gamma J) x = avec + gamma f - ycur by right-preconditioned
+// GMRES in the space scaled by ewt (J the jacobi_ts1 linearisation at ycur)
+// to |residual| <= tol (scaled Euclidean) or maxl iterations, then sets
+// ycur += x.  Writes w.status: [0] iterations, [1] sum (x ewt)^2,
+// [2] != 0 if an exchange timed out, [3] sum ((ycur - yp) ewt)^2.
+hipError_t launch_newton_gmres(const DeviceOps& d, bool mixed, GmresWork& w, int grid, int spw,
+                               int maxl, const NewtonSystem& s, hipStream_t st);
 
 }  // namespace nsim

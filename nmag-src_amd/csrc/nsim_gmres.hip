@@ -1,25 +1,32 @@
-// Persistent GMRES for the Newton systems (I - gamma J) x = r of the BDF
-// integrator: the work of CVODE's SPGMR (src/sundials_sp_stubs.c:700-1003
-// registers it with the jacobi_ts1 J.v and the preconditioner) in a single
-// launch per linear solve.
+// Persistent Newton iteration for the BDF integrator: residual, block-Jacobi
+// set-up, the GMRES solve of (I - gamma J) x = r, the update and its norms in
+// one launch.  It does the work of CVODE's SPGMR with the jacobi_ts1 J.v and
+// the preconditioner callbacks (src/sundials_sp_stubs.c:700-1003,
+// src/nsim.ml:2273-2484).
 //
-// Right-preconditioned GMRES in the space scaled by the error weights (so the
-// Euclidean residual is sqrt(n) times its WRMS norm), classical Gram-Schmidt
-// applied twice, Givens rotations on the Hessenberg columns.  Each workgroup
-// owns whole SELL slices (64 nodes each) and every vector element of their
-// nodes; everything it computes on them is local except the exchange part of
-// J.z, which gathers the preconditioned direction z at neighbouring nodes.
-// Grid-wide data meet in three exchanges per iteration: the two Gram-Schmidt
-// coefficient columns and the new vector's norm, the last one carrying the
-// next direction z' = M^-1 (v / ewt) (still unnormalised; J is linear, so the
-// gather scales by 1/norm).  An exchange publishes per-workgroup partial sums
-// with agent-coherent stores, then a phase tag; every workgroup polls all
-// tags and sums the partials in the same fixed order, so all of them hold
-// bit-identical totals, take the same Givens rotations and stop at the same
-// iteration without further communication.  Workgroup 0 keeps the triangular
-// factor, solves the least-squares problem at the end and broadcasts its
-// solution; the final correction x and sum (x ewt)^2 (the Newton update's
-// WRMS norm) are formed in the same launch.
+// GMRES runs right-preconditioned in the space scaled by the error weights
+// (the Euclidean residual is sqrt(n) times its WRMS norm) in its flexible
+// form: the preconditioned directions z_j are stored, the Krylov basis V is
+// orthonormalised by classical Gram-Schmidt applied twice, and x = Z y.  Each
+// workgroup owns whole SELL slices (64 nodes each) and every vector element
+// of their nodes; everything it computes on them is local except the exchange
+// part of J.z, which gathers z at neighbouring nodes.  Grid-wide data meet in
+// two exchanges per iteration:
+//   1. the first Gram-Schmidt column;
+//   2. the second column together with |v'|^2 of the once-projected vector
+//      v', whose direction z_{j+1} = M^-1 (v' / ewt) is published at the same
+//      time.  The new norm follows as |v'|^2 - |h2|^2 (v' - V h2 is v'
+//      without its components along the orthonormal V); when that difference
+//      loses more than two digits, a third exchange measures it directly.
+// Using z_{j+1} from v' rather than from the twice-projected vector is what
+// flexible GMRES permits: the Arnoldi relation A Z = V H holds for the z
+// actually used, and the solution is formed from the same Z.
+// An exchange publishes per-workgroup partial sums with agent-coherent stores,
+// then a phase tag; every workgroup polls all tags and sums the partials in
+// the same fixed order, so all of them hold bit-identical totals, take the
+// same Givens rotations and stop at the same iteration without further
+// communication.  Workgroup 0 keeps the triangular factor, solves the
+// least-squares problem at the end and broadcasts its solution.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -34,7 +41,7 @@ namespace {
 
 using namespace dev;
 
-constexpr int kSplit = 4;                 // wavefronts per slice in the J.z gather
+constexpr int kSplit = 4;                  // wavefronts per slice in the J.z gather
 constexpr unsigned kSpinLimit = 1u << 20;  // polls before an exchange gives up
 constexpr int kMaxGrid = 1024;
 
@@ -47,16 +54,19 @@ struct GmresArgs {
   const double* Vinv;
   const Coeffs* ct;
   const int* inst;
-  const double* m;
-  const double* Ht;
+  double* m;           // Newton iterate ycur; receives ycur + x
+  const double* Ht;    // H_total of the rhs at m
   const double* ewt;
-  const double* Minv;  // null: no preconditioner
+  const double* kappa;
+  double* Minv;        // block-Jacobi inverses, set up here (null: none)
   double gamma, tol;
-  const double* r;
-  double* x;
+  const double* avec;  // residual r = avec + gamma f - m
+  const double* f;
+  const double* yp;    // predictor (for the local error sum)
+  double* x;           // the correction
   double* V;
+  double* Z;           // maxl directions, published coherently
   size_t ld;
-  double* zp;
   double* part;
   unsigned long long* tag;
   double* R;
@@ -68,14 +78,16 @@ struct GmresArgs {
 
 struct GShared {
   double cs[kGmresMaxl], sn[kGmresMaxl], g[kGmresMaxl + 1];
+  double zs[kGmresMaxl];       // z_j = zs[j] * Z_j
   double col[kGmresMaxl + 1];  // Hessenberg column of this iteration
-  double loc[kGmresMaxl + 1];  // this workgroup's partial sums
-  double tot[kGmresMaxl + 1];  // grid totals
+  double loc[kGmresMaxl + 2];  // this workgroup's partial sums
+  double tot[kGmresMaxl + 2];  // grid totals
   double gp[3 * kSplit][64];   // J.z gather partials
   double red[8];
+  double hn2, vn2;  // |v|^2 from |v'|^2 - |h2|^2, and |v'|^2
 };
 
-// z = M^-1 t on one node (3x3 block), t = v / ewt.
+// z = M^-1 t on one node (3x3 block).
 __device__ inline void pc_node(const double* __restrict__ Minv, int i, const double* t,
                                double* z) {
   if (!Minv) {
@@ -90,13 +102,19 @@ __device__ inline void pc_node(const double* __restrict__ Minv, int i, const dou
   z[2] = o[6] * t[0] + o[7] * t[1] + o[8] * t[2];
 }
 
+template <typename TV>
+__device__ inline __amdgpu_buffer_rsrc_t z_rsrc(const GmresArgs<TV>& a, int j) {
+  return make_rsrc(a.Z + (size_t)j * a.ld, (unsigned)(a.ld * 8));
+}
+
 // Publishes sm.loc[0, cnt) for phase `ph` and, unless this workgroup is not
 // the root of a root_only exchange, sums every workgroup's values into
 // sm.tot[0, cnt): wavefront w takes the entries w, w + 4, ..., lane l the
 // workgroups l, l + 64, ..., then a fixed butterfly; the same order in every
-// workgroup.  Returns false if the exchange timed out or another workgroup
-// reported an error (the spin is bounded, so every workgroup leaves and the
-// grid drains).
+// workgroup.  Every wave's earlier coherent stores (the published direction)
+// complete before the tag.  Returns false if the exchange timed out or another
+// workgroup reported an error (the spin is bounded, so every workgroup leaves
+// and the grid drains).
 template <typename TV>
 __device__ bool exchange(const GmresArgs<TV>& a, GShared& sm, unsigned ph, int cnt,
                          bool root_only) {
@@ -105,7 +123,7 @@ __device__ bool exchange(const GmresArgs<TV>& a, GShared& sm, unsigned ph, int c
   __syncthreads();  // sm.loc complete
   double* mine = a.part + ((size_t)par * a.gmax + blockIdx.x) * a.pstride;
   for (int i = threadIdx.x; i < cnt; i += 256) st_coh(mine + i, sm.loc[i]);
-  wait_all_memory();  // this wave's exchanged stores (z', partials) completed
+  wait_all_memory();
   __syncthreads();
   if (threadIdx.x == 0) st_coh(&a.tag[par * a.gmax + blockIdx.x], tg);
   if (root_only && blockIdx.x != 0) return true;
@@ -122,10 +140,12 @@ __device__ bool exchange(const GmresArgs<TV>& a, GShared& sm, unsigned ph, int c
   }
   if (__syncthreads_or(bad)) return false;
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
-  const double* base = a.part + (size_t)par * a.gmax * a.pstride;
+  const __amdgpu_buffer_rsrc_t rp =
+      make_rsrc(a.part + (size_t)par * a.gmax * a.pstride, (unsigned)(a.gmax * a.pstride * 8));
   for (int i = w; i < cnt; i += 4) {
     double s = 0.0;
-    for (int t = l; t < G; t += 64) s += ld_coh(base + (size_t)t * a.pstride + i);
+#pragma unroll 8
+    for (int t = l; t < G; t += 64) s += load_coh(rp, t * a.pstride + i);
     s = wave_sum(s);
     if (l == 0) sm.tot[i] = s;
   }
@@ -151,24 +171,37 @@ __device__ bool wait_root(const GmresArgs<TV>& a, unsigned ph) {
   return !__syncthreads_or(bad);
 }
 
-// Sum over the workgroup into sm.loc[0] (fixed order).
-__device__ inline void block_total(double s, GShared& sm) {
+// Sum over the workgroup into sm.loc[slot] (fixed order).
+__device__ inline void block_total(double s, GShared& sm, int slot) {
   double dummy = 0.0;
   block_sum2(s, dummy, sm.red);
-  if (threadIdx.x == 0) sm.loc[0] = s;
+  if (threadIdx.x == 0) sm.loc[slot] = s;
 }
 
-// z' = M^-1 (src / ewt) on this workgroup's nodes, published coherently; with
-// src_is_t the source is already t = v / ewt.
+// Z_j = M^-1 (v / ewt) on this workgroup's nodes, published coherently.
 template <typename TV>
-__device__ void publish_direction(const GmresArgs<TV>& a, const double* src, bool src_is_t,
-                                  int r0, int r1) {
+__device__ void publish_direction(const GmresArgs<TV>& a, const double* v, int j, int r0,
+                                  int r1) {
+  double* Zj = a.Z + (size_t)j * a.ld;
   for (int i = r0 + threadIdx.x; i < r1; i += 256) {
     double t[3], z[3];
-    for (int d = 0; d < 3; d++)
-      t[d] = src_is_t ? src[3 * i + d] : src[3 * i + d] / a.ewt[3 * i + d];
+    for (int d = 0; d < 3; d++) t[d] = v[3 * i + d] / a.ewt[3 * i + d];
     pc_node(a.Minv, i, t, z);
-    for (int d = 0; d < 3; d++) st_coh(a.zp + 3 * i + d, z[d]);
+    for (int d = 0; d < 3; d++) st_coh(Zj + 3 * i + d, z[d]);
+  }
+}
+
+// sm.loc[i] = this workgroup's part of V_i . v for i <= j.
+template <typename TV>
+__device__ void local_dots(const GmresArgs<TV>& a, GShared& sm, const double* v, int j, int e0,
+                           int e1) {
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int i = wv; i <= j; i += 4) {
+    const double* Vi = a.V + (size_t)i * a.ld;
+    double s = 0.0;
+    for (int e = e0 + lane; e < e1; e += 64) s += Vi[e] * v[e];
+    s = wave_sum(s);
+    if (lane == 0) sm.loc[i] = s;
   }
 }
 
@@ -184,36 +217,57 @@ __global__ __launch_bounds__(256) void k_gmres(GmresArgs<TV> a) {
   bool ok = true;
   int k = 0;
 
-  // v0 = r ewt, beta = |v0|; z'_0 = M^-1 r
+  // Newton residual r = avec + gamma f - ycur and this workgroup's
+  // block-Jacobi inverses at the current iterate (its own nodes only);
+  // v0 = r ewt, beta = |v0|; Z_0 = M^-1 r
   {
+    double* Z0 = a.Z;
+    for (int i = r0 + tid; i < r1; i += 256) {
+      double ri[3];
+      for (int d = 0; d < 3; d++) {
+        double v = 1.0 * a.avec[3 * i + d];
+        v += a.gamma * a.f[3 * i + d];
+        v += -1.0 * a.m[3 * i + d];
+        ri[d] = v;
+      }
+      if (a.Minv) {
+        const Coeffs& cf = a.ct[a.inst ? a.inst[i] : 0];
+        const double mi[3] = {a.m[3 * i], a.m[3 * i + 1], a.m[3 * i + 2]};
+        const double Hi[3] = {a.Ht[3 * i], a.Ht[3 * i + 1], a.Ht[3 * i + 2]};
+        bjac_block(cf.c1, cf.c2, a.gamma, a.kappa[i], mi, Hi, a.Minv + 9 * i);
+      }
+      double z[3];
+      pc_node(a.Minv, i, ri, z);
+      for (int d = 0; d < 3; d++) st_coh(Z0 + 3 * i + d, z[d]);
+    }
     double s = 0.0;
     for (int e = e0 + tid; e < e1; e += 256) {
-      const double v = a.r[e] * a.ewt[e];
+      double r = 1.0 * a.avec[e];
+      r += a.gamma * a.f[e];
+      r += -1.0 * a.m[e];
+      const double v = r * a.ewt[e];
       a.V[e] = v;
       s += v * v;
     }
-    publish_direction(a, a.r, true, r0, r1);
-    block_total(s, sm);
+    block_total(s, sm, 0);
     ok = exchange(a, sm, ++ph, 1, false);
   }
   const double beta = ok ? sqrt(sm.tot[0]) : 0.0;
-  if (ok && !(beta > tol)) {  // already solved: x = 0
-    for (int e = e0 + tid; e < e1; e += 256) a.x[e] = 0.0;
-    if (blockIdx.x == 0 && tid == 0) {
-      a.status[0] = 0.0;
-      a.status[1] = 0.0;
-      a.status[2] = 0.0;
+  const bool solved = !(beta > tol);  // then x = 0 (k stays 0)
+  if (ok && !solved) {
+    const double sc = 1.0 / beta;
+    for (int e = e0 + tid; e < e1; e += 256) a.V[e] *= sc;
+    if (tid == 0) {
+      sm.g[0] = beta;
+      sm.zs[0] = sc;
     }
-    return;
+    __syncthreads();
   }
-  double scale = ok ? 1.0 / beta : 0.0;
-  if (ok) {
-    for (int e = e0 + tid; e < e1; e += 256) a.V[e] *= scale;
-    if (tid == 0) sm.g[0] = beta;
-  }
-  for (int j = 0; ok && j < a.maxl; j++) {
+  for (int j = 0; ok && !solved && j < a.maxl; j++) {
     double* Vn = a.V + (size_t)(j + 1) * a.ld;
-    // Vn = (z - gamma J z) ewt, z = scale z'
+    const __amdgpu_buffer_rsrc_t rz = z_rsrc(a, j);
+    const double sc = sm.zs[j];
+    // Vn = (z - gamma J z) ewt, z = sc Z_j
     for (int s = s0; s < s1; s++) {
       const int row = 64 * s + lane;
       const bool live = row < a.n;
@@ -224,9 +278,9 @@ __global__ __launch_bounds__(256) void k_gmres(GmresArgs<TV> a) {
           const int e = b + (q << 6) + lane;
           const int c = a.col[e];
           const double kv = (double)a.K[e];
-          hv[0] += kv * ld_coh(a.zp + 3 * c);
-          hv[1] += kv * ld_coh(a.zp + 3 * c + 1);
-          hv[2] += kv * ld_coh(a.zp + 3 * c + 2);
+          hv[0] += kv * load_coh(rz, 3 * c);
+          hv[1] += kv * load_coh(rz, 3 * c + 1);
+          hv[2] += kv * load_coh(rz, 3 * c + 2);
         }
       for (int q = 0; q < 3; q++) sm.gp[wv * 3 + q][lane] = hv[q];
       __syncthreads();
@@ -237,55 +291,72 @@ __global__ __launch_bounds__(256) void k_gmres(GmresArgs<TV> a) {
           hv[q] = acc;
         }
         const Coeffs& cf = a.ct[a.inst ? a.inst[row] : 0];
-        const double fct = cf.negJ * a.Vinv[row] * scale;
+        const double fct = cf.negJ * a.Vinv[row] * sc;
         const double h[3] = {fct * hv[0], fct * hv[1], fct * hv[2]};
         const double mi[3] = {a.m[3 * row], a.m[3 * row + 1], a.m[3 * row + 2]};
         const double Hi[3] = {a.Ht[3 * row], a.Ht[3 * row + 1], a.Ht[3 * row + 2]};
         double zi[3], jv[3];
-        for (int d = 0; d < 3; d++) zi[d] = scale * ld_coh(a.zp + 3 * row + d);
+        for (int d = 0; d < 3; d++) zi[d] = sc * load_coh(rz, 3 * row + d);
         llg_jv_row(cf.c1, cf.c2, h, mi, zi, Hi, jv);
         for (int d = 0; d < 3; d++)
           Vn[3 * row + d] = (zi[d] - a.gamma * jv[d]) * a.ewt[3 * row + d];
       }
       __syncthreads();
     }
-    // classical Gram-Schmidt, twice
-    for (int pass = 0; pass < 2 && ok; pass++) {
-      for (int i = wv; i <= j; i += 4) {
-        const double* Vi = a.V + (size_t)i * a.ld;
-        double s = 0.0;
-        for (int e = e0 + lane; e < e1; e += 64) s += Vi[e] * Vn[e];
-        s = wave_sum(s);
-        if (lane == 0) sm.loc[i] = s;
-      }
-      ok = exchange(a, sm, ++ph, j + 1, false);
-      if (!ok) break;
-      for (int e = e0 + tid; e < e1; e += 256) {
-        double v = Vn[e];
-        for (int i = 0; i <= j; i++) v -= sm.tot[i] * a.V[(size_t)i * a.ld + e];
-        Vn[e] = v;
-      }
-      for (int i = tid; i <= j; i += 256) sm.col[i] = pass == 0 ? sm.tot[i] : sm.col[i] + sm.tot[i];
-      __syncthreads();
-    }
+    // Gram-Schmidt pass 1
+    local_dots(a, sm, Vn, j, e0, e1);
+    ok = exchange(a, sm, ++ph, j + 1, false);
     if (!ok) break;
-    // norm of the new vector; its direction z' travels in the same exchange
+    for (int e = e0 + tid; e < e1; e += 256) {
+      double v = Vn[e];
+      for (int i = 0; i <= j; i++) v -= sm.tot[i] * a.V[(size_t)i * a.ld + e];
+      Vn[e] = v;
+    }
+    for (int i = tid; i <= j; i += 256) sm.col[i] = sm.tot[i];
+    __syncthreads();
+    // pass 2 with |v'|^2 and the next direction in the same exchange
+    local_dots(a, sm, Vn, j, e0, e1);
     {
       double s = 0.0;
       for (int e = e0 + tid; e < e1; e += 256) s += Vn[e] * Vn[e];
-      publish_direction(a, Vn, false, r0, r1);
-      block_total(s, sm);
+      if (j + 1 < a.maxl) publish_direction(a, Vn, j + 1, r0, r1);
+      block_total(s, sm, j + 1);
+    }
+    ok = exchange(a, sm, ++ph, j + 2, false);
+    if (!ok) break;
+    for (int e = e0 + tid; e < e1; e += 256) {
+      double v = Vn[e];
+      for (int i = 0; i <= j; i++) v -= sm.tot[i] * a.V[(size_t)i * a.ld + e];
+      Vn[e] = v;
+    }
+    if (tid == 0) {
+      double h2 = 0.0;
+      for (int i = 0; i <= j; i++) {
+        sm.col[i] += sm.tot[i];
+        h2 += sm.tot[i] * sm.tot[i];
+      }
+      sm.hn2 = sm.tot[j + 1] - h2;
+      sm.vn2 = sm.tot[j + 1];
+    }
+    __syncthreads();
+    double hn2 = sm.hn2;
+    if (!(hn2 > 1e-2 * sm.vn2)) {  // cancellation: measure |v| directly
+      double s = 0.0;
+      for (int e = e0 + tid; e < e1; e += 256) s += Vn[e] * Vn[e];
+      block_total(s, sm, 0);
       ok = exchange(a, sm, ++ph, 1, false);
       if (!ok) break;
+      hn2 = sm.tot[0];
     }
-    const double hn = sqrt(sm.tot[0]);
+    const double hn = sqrt(fmax(hn2, 0.0));
     if (hn > 0.0) {
-      scale = 1.0 / hn;
-      for (int e = e0 + tid; e < e1; e += 256) Vn[e] *= scale;
+      const double inv = 1.0 / hn;
+      for (int e = e0 + tid; e < e1; e += 256) Vn[e] *= inv;
     }
     if (tid == 0) {
       double* c = sm.col;
       c[j + 1] = hn;
+      if (j + 1 < a.maxl) sm.zs[j + 1] = hn > 0.0 ? 1.0 / hn : 0.0;
       for (int i = 0; i < j; i++) {  // previous rotations
         const double p = c[i], q = c[i + 1];
         c[i] = sm.cs[i] * p + sm.sn[i] * q;
@@ -305,8 +376,8 @@ __global__ __launch_bounds__(256) void k_gmres(GmresArgs<TV> a) {
     k = j + 1;
     if (fabs(sm.g[j + 1]) <= tol || !(hn > 0.0) || !isfinite(hn)) break;
   }
-  // y = R^-1 g (workgroup 0), broadcast; x = M^-1 (sum_i y_i V_i / ewt)
-  if (ok) {
+  // y = R^-1 g (workgroup 0), broadcast into sm.tot as y_i zs_i
+  if (ok && k > 0) {
     ++ph;
     if (blockIdx.x == 0) {
       if (tid == 0) {
@@ -323,36 +394,47 @@ __global__ __launch_bounds__(256) void k_gmres(GmresArgs<TV> a) {
     } else {
       ok = wait_root(a, ph);
       if (ok) {
-        for (int i = tid; i < k; i += 256) sm.tot[i] = ld_coh(a.y + i);
+        const __amdgpu_buffer_rsrc_t ry = make_rsrc(a.y, (unsigned)(a.maxl * 8));
+        for (int i = tid; i < k; i += 256) sm.tot[i] = load_coh(ry, i);
         __syncthreads();
       }
     }
+    if (ok) {
+      for (int i = tid; i < k; i += 256) sm.tot[i] *= sm.zs[i];
+      __syncthreads();
+    }
   }
   if (ok) {
+    // x = Z (y zs); the Newton update ycur += x, with the weighted sums of
+    // squares of the correction and of the distance from the predictor
+    double sx = 0.0, se = 0.0;
     for (int e = e0 + tid; e < e1; e += 256) {
-      double v = 0.0;
-      for (int i = 0; i < k; i++) v += sm.tot[i] * a.V[(size_t)i * a.ld + e];
-      a.x[e] = v / a.ewt[e];
+      double xv = 0.0;
+      for (int i = 0; i < k; i++) xv += sm.tot[i] * load_coh(z_rsrc(a, i), e);
+      a.x[e] = xv;
+      const double w = a.ewt[e];
+      const double u = xv * w;
+      sx += u * u;
+      double yv = 1.0 * a.m[e];
+      yv += 1.0 * xv;
+      a.m[e] = yv;
+      double dy = 1.0 * yv;
+      dy += -1.0 * a.yp[e];
+      const double v = dy * w;
+      se += v * v;
     }
-    __syncthreads();
-    double s = 0.0;
-    for (int i = r0 + tid; i < r1; i += 256) {
-      const double t[3] = {a.x[3 * i], a.x[3 * i + 1], a.x[3 * i + 2]};
-      double z[3];
-      pc_node(a.Minv, i, t, z);
-      for (int d = 0; d < 3; d++) {
-        a.x[3 * i + d] = z[d];
-        const double u = z[d] * a.ewt[3 * i + d];
-        s += u * u;
-      }
+    block_sum2(sx, se, sm.red);
+    if (tid == 0) {
+      sm.loc[0] = sx;
+      sm.loc[1] = se;
     }
-    block_total(s, sm);
-    ok = exchange(a, sm, ++ph, 1, true);
+    ok = exchange(a, sm, ++ph, 2, true);
   }
   if (blockIdx.x == 0 && tid == 0) {
     a.status[0] = (double)k;
     a.status[1] = ok ? sm.tot[0] : NAN;
     a.status[2] = ok ? 0.0 : 1.0;
+    a.status[3] = ok ? sm.tot[1] : NAN;
   }
 }
 
@@ -361,51 +443,50 @@ const void* kernel_ptr() {
   return reinterpret_cast<const void*>(&k_gmres<TV>);
 }
 
+template <typename TV>
+hipError_t launch_typed(const DeviceOps& d, const TV* K, GmresWork& w, int grid, int spw,
+                        int maxl, const NewtonSystem& s, hipStream_t st) {
+  GmresArgs<TV> a{d.n,     d.nslices, spw,      maxl,     w.gmax,  w.pstride, d.sell_off,
+                  d.sell_col, K,      d.Vinv,   d.coefs,  d.inst,  s.ycur,    s.Ht,
+                  s.ewt,   s.kappa,   s.Minv,   s.gamma,  s.tol,   s.avec,    s.f,
+                  s.yp,    s.x,       w.V,      w.Z,      3 * (size_t)d.n,    w.part,
+                  w.tag,   w.R,       w.y,      w.status, w.err,   ++w.epoch};
+  void* args[] = {&a};
+  return hipLaunchCooperativeKernel(kernel_ptr<TV>(), dim3(grid), dim3(256), args, 0, st);
+}
+
 }  // namespace
 
-bool gmres_grid(int nslices, int maxl, int device, int gmax, int* grid, int* spw) {
+bool gmres_grid(int nslices, int maxl, int device, int gmax, int min_spw, int* grid,
+                int* spw) {
   if (nslices <= 0 || maxl < 1 || maxl > kGmresMaxl) return false;
-  int per_cu = 0, cus = 0;
+  int cus = 0, pf = 0, pd = 0;
   // the FP32 and FP64 variants differ only in one load; take the smaller
-  int pf = 0, pd = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&pf, kernel_ptr<float>(), 256, 0) !=
           hipSuccess ||
       hipOccupancyMaxActiveBlocksPerMultiprocessor(&pd, kernel_ptr<double>(), 256, 0) !=
           hipSuccess ||
       hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
     return false;
-  per_cu = pf < pd ? pf : pd;
-  int cap = per_cu * cus;
+  int cap = (pf < pd ? pf : pd) * cus;
   if (cap > kMaxGrid) cap = kMaxGrid;
   if (cap > gmax) cap = gmax;
   if (cap < 1) return false;
-  const int s = (nslices + cap - 1) / cap;
+  int s = (nslices + cap - 1) / cap;
+  if (s < min_spw) s = min_spw;
   *spw = s;
   *grid = (nslices + s - 1) / s;
   return true;
 }
 
-hipError_t launch_gmres(const DeviceOps& d, bool mixed, GmresWork& w, int grid, int spw,
-                        int maxl, double gamma, double tol, const double* m, const double* Ht,
-                        const double* ewt, const double* Minv, const double* r, double* x,
-                        hipStream_t st) {
-  if (grid < 1 || grid > w.gmax || maxl < 1 || maxl > kGmresMaxl || w.pstride < maxl + 1 ||
-      (long long)grid * spw < d.nslices || (long long)(grid - 1) * spw >= d.nslices)
+hipError_t launch_newton_gmres(const DeviceOps& d, bool mixed, GmresWork& w, int grid, int spw,
+                               int maxl, const NewtonSystem& s, hipStream_t st) {
+  if (grid < 1 || grid > w.gmax || maxl < 1 || maxl > kGmresMaxl || w.pstride < maxl + 2 ||
+      (long long)grid * spw < d.nslices || (long long)(grid - 1) * spw >= d.nslices ||
+      3LL * d.n * 8 >= (1LL << 32) || (long long)w.gmax * w.pstride * 8 >= (1LL << 32))
     return hipErrorInvalidConfiguration;
-  const unsigned long long epoch = ++w.epoch;
-  const size_t ld = 3 * (size_t)d.n;
-  if (mixed) {
-    GmresArgs<float> a{d.n, d.nslices, spw, maxl, w.gmax, w.pstride, d.sell_off, d.sell_col,
-                       d.Kf, d.Vinv, d.coefs, d.inst, m, Ht, ewt, Minv, gamma, tol, r, x,
-                       w.V, ld, w.zp, w.part, w.tag, w.R, w.y, w.status, w.err, epoch};
-    void* args[] = {&a};
-    return hipLaunchCooperativeKernel(kernel_ptr<float>(), dim3(grid), dim3(256), args, 0, st);
-  }
-  GmresArgs<double> a{d.n, d.nslices, spw, maxl, w.gmax, w.pstride, d.sell_off, d.sell_col,
-                      d.K, d.Vinv, d.coefs, d.inst, m, Ht, ewt, Minv, gamma, tol, r, x,
-                      w.V, ld, w.zp, w.part, w.tag, w.R, w.y, w.status, w.err, epoch};
-  void* args[] = {&a};
-  return hipLaunchCooperativeKernel(kernel_ptr<double>(), dim3(grid), dim3(256), args, 0, st);
+  if (mixed) return launch_typed<float>(d, d.Kf, w, grid, spw, maxl, s, st);
+  return launch_typed<double>(d, d.K, w, grid, spw, maxl, s, st);
 }
 
 }  // namespace nsim

@@ -95,12 +95,13 @@ Integrator::Integrator(nsim_handle* h, const TsOptions& o) : h_(h), opt_(o) {
   // persistent GMRES: one co-resident grid over the SELL slices
   const DeviceOps& d = handle_ops(h);
   int grid = 0, spw = 0;
-  if (ok_ && gmres_grid(d.nslices, opt_.krylov_max, handle_device(h), 1 << 20, &grid, &spw)) {
+  if (ok_ && nsim::gmres_grid(d.nslices, opt_.krylov_max, handle_device(h), 1 << 20,
+                        std::max(1, opt_.gmres_spw), &grid, &spw)) {
     gw_.V = V_;
     gw_.gmax = grid;
-    gw_.pstride = (opt_.krylov_max + 8) & ~7;
+    gw_.pstride = (opt_.krylov_max + 2 + 7) & ~7;
     const size_t maxl = opt_.krylov_max;
-    bool g = alloc(&gw_.zp, n) && alloc(&gw_.part, 2 * (size_t)grid * gw_.pstride) &&
+    bool g = alloc(&gw_.Z, n * maxl) && alloc(&gw_.part, 2 * (size_t)grid * gw_.pstride) &&
              alloc(&gw_.R, maxl * maxl) && alloc(&gw_.y, maxl) && alloc(&gw_.status, 8);
     if (g && hipMalloc(reinterpret_cast<void**>(&gw_.tag),
                        2 * (size_t)grid * sizeof(unsigned long long)) == hipSuccess) {
@@ -174,9 +175,7 @@ int Integrator::apply_A(const double* z, double* w) {
 
 // Right-preconditioned GMRES in the space scaled by the error weights, so the
 // Euclidean residual is sqrt(n) times its WRMS norm.  x = (I - gamma J)^-1 r.
-int Integrator::gmres(const double* r, double* x, int* iters, double* xw2) {
-  if (opt_.device_gmres && ggrid_ > 0) return gmres_device(r, x, iters, xw2);
-  *xw2 = -1.0;  // not formed here
+int Integrator::gmres(const double* r, double* x, int* iters) {
   const int maxl = opt_.krylov_max;
   const size_t ld = n3_;
   double* V0 = V_;
@@ -266,38 +265,49 @@ int Integrator::gmres(const double* r, double* x, int* iters, double* xw2) {
   return 0;
 }
 
-// The same solve as one persistent launch (nsim_gmres.hip); the host reads
-// the iteration count and the correction's weighted norm with one
+// One Newton iteration after the rhs at ycur: residual, block-Jacobi set-up,
+// GMRES, ycur += del and the weighted norms in one persistent launch
+// (nsim_gmres.hip); the host reads the iteration count, the correction's
+// WRMS norm *d and the sum of squares of the local error with one
 // synchronisation.
-int Integrator::gmres_device(const double* r, double* x, int* iters, double* xw2) {
-  const double tol = kLinCoef * kNewtonCoef * std::sqrt((double)n3_);
-  HTRY(launch_gmres(handle_ops(h_), handle_mixed(h_), gw_, ggrid_, gspw_, opt_.krylov_max,
-                    gamma_, tol, ycur_, handle_Htotal(h_), ewt_,
-                    opt_.precond ? Minv_ : nullptr, r, x, s_));
-  HTRY(hipMemcpyAsync(hhost_, gw_.status, 3 * sizeof(double), hipMemcpyDeviceToHost, s_));
+int Integrator::newton_device(double* d, double* errsq) {
+  NewtonSystem sys;
+  sys.ycur = ycur_;
+  sys.Ht = handle_Htotal(h_);
+  sys.f = f_;
+  sys.avec = avec_;
+  sys.yp = yp_;
+  sys.ewt = ewt_;
+  sys.kappa = handle_kappa(h_);
+  sys.Minv = opt_.precond ? Minv_ : nullptr;
+  sys.x = del_;
+  sys.gamma = gamma_;
+  sys.tol = kLinCoef * kNewtonCoef * std::sqrt((double)n3_);
+  HTRY(launch_newton_gmres(handle_ops(h_), handle_mixed(h_), gw_, ggrid_, gspw_,
+                           opt_.krylov_max, sys, s_));
+  HTRY(hipMemcpyAsync(hhost_, gw_.status, 4 * sizeof(double), hipMemcpyDeviceToHost, s_));
   HTRY(hipStreamSynchronize(s_));
   if (hhost_[2] != 0.0) {
     (void)hipMemset(gw_.err, 0, sizeof(unsigned));
     set_error("integrator: GMRES exchange timed out");
     return -1;
   }
-  *iters = (int)hhost_[0];
-  *xw2 = hhost_[1];
-  st_.linear += *iters;
+  st_.linear += (long)hhost_[0];
+  *d = std::sqrt(hhost_[1] / n3_);
+  *errsq = hhost_[3];
   return 0;
 }
 
 // Predictor into yp; E_p is the predictor's error constant (coefficient of
 // y^(q+1), with y_pred - y_exact = -E_p y^(q+1)).
-int Integrator::predictor(double h, int q, double* yp, double* E_p) {
+// yp = L0 y_n + L1 p1 + L2 p2 (null p drop their terms).
+void Integrator::predictor(double h, int q, Predictor* p) const {
   const double t1 = tn_ + h;
   if (nhist_ == 0) {  // explicit Euler from f(y_n)
-    HTRY(vec_lincomb(s_, n3_, 1.0, yn_, h, fn_, 0, nullptr, 0, nullptr, yp));
-    *E_p = 0.5 * h * h;
-    return 0;
+    *p = Predictor{{1.0, h, 0.0}, fn_, nullptr, 0.5 * h * h};
+    return;
   }
   const double P[3] = {tn_, tnm1_, tnm2_};
-  const double* Y[3] = {yn_, ynm1_, ynm2_};
   double L[3] = {0, 0, 0}, ep = 1.0;
   for (int j = 0; j <= q; j++) {
     double w = 1.0;
@@ -306,10 +316,8 @@ int Integrator::predictor(double h, int q, double* yp, double* E_p) {
     L[j] = w;
     ep *= t1 - P[j];
   }
-  *E_p = ep / factorial(q + 1);
-  HTRY(vec_lincomb(s_, n3_, L[0], Y[0], L[1], Y[1], q >= 2 ? L[2] : 0.0,
-                   q >= 2 ? Y[2] : nullptr, 0, nullptr, yp));
-  return 0;
+  *p = Predictor{{L[0], L[1], q >= 2 ? L[2] : 0.0}, ynm1_, q >= 2 ? ynm2_ : nullptr,
+                 ep / factorial(q + 1)};
 }
 
 int Integrator::step_attempt(double h, int q, bool* accepted, double* err_out) {
@@ -322,13 +330,16 @@ int Integrator::step_attempt(double h, int q, bool* accepted, double* err_out) {
   double R = 0.0;
   for (int j = 1; j <= q; j++) R += dl[j] * std::pow(T[j] - T[0], q + 1) / factorial(q + 1);
   const double E_c = -R / dl[0];
-  double E_p = 0.0;
-  TRY(predictor(h, q, yp_, &E_p));
-  const double C = E_c / (E_c + E_p);
+  Predictor pr;
+  predictor(h, q, &pr);
+  const double C = E_c / (E_c + pr.E_p);
+  // one pass: error weights of y_n, the predictor yp (= ycur), and
   // a = -gamma sum_{j>=1} l_j' y_j
-  HTRY(vec_lincomb(s_, n3_, -gamma_ * dl[1], Y[1], q >= 2 ? -gamma_ * dl[2] : 0.0,
-                   q >= 2 ? Y[2] : nullptr, 0, nullptr, 0, nullptr, avec_));
-  HTRY(hipMemcpyAsync(ycur_, yp_, sizeof(double) * n3_, hipMemcpyDeviceToDevice, s_));
+  HTRY(vec_step_begin(s_, n3_, yn_, pr.L[0], pr.p1, pr.L[1], pr.p2, pr.L[2], -gamma_ * dl[1],
+                      q >= 2 ? Y[2] : nullptr, q >= 2 ? -gamma_ * dl[2] : 0.0, opt_.rtol,
+                      opt_.atol, ewt_, yp_, ycur_, avec_));
+  const bool dev = opt_.device_gmres && ggrid_ > 0;
+  double errsq = NAN;  // sum ((ycur - yp) ewt)^2 after the last Newton update
   // The convergence-rate estimate carries over between steps (as CVODE's
   // crate does) unless gamma changed a lot, so a small first correction can
   // be accepted without a second rhs evaluation.
@@ -340,16 +351,21 @@ int Integrator::step_attempt(double h, int q, bool* accepted, double* err_out) {
     TRY(device_rhs(h_, ycur_, f_));
     st_.rhs++;
     st_.newton++;
-    // res = -G = a + gamma f - y
-    HTRY(vec_lincomb(s_, n3_, 1.0, avec_, gamma_, f_, -1.0, ycur_, 0, nullptr, res_));
-    if (opt_.precond)
-      HTRY(bjac_setup(s_, nn_, handle_coef_table(h_), handle_inst(h_), gamma_, ycur_,
-                      handle_Htotal(h_), handle_kappa(h_), Minv_));
-    int its = 0;
-    double xw2 = -1.0;
-    TRY(gmres(res_, del_, &its, &xw2));
-    HTRY(vec_lincomb(s_, n3_, 1.0, ycur_, 1.0, del_, 0, nullptr, 0, nullptr, ycur_));
-    const double d = xw2 >= 0.0 ? std::sqrt(xw2 / n3_) : wrms(del_);
+    double d = 0.0;
+    if (dev) {
+      // residual, preconditioner, solve, update and both norms in one launch
+      TRY(newton_device(&d, &errsq));
+    } else {
+      // res = -G = a + gamma f - y
+      HTRY(vec_lincomb(s_, n3_, 1.0, avec_, gamma_, f_, -1.0, ycur_, 0, nullptr, res_));
+      if (opt_.precond)
+        HTRY(bjac_setup(s_, nn_, handle_coef_table(h_), handle_inst(h_), gamma_, ycur_,
+                        handle_Htotal(h_), handle_kappa(h_), Minv_));
+      int its = 0;
+      TRY(gmres(res_, del_, &its));
+      HTRY(vec_lincomb(s_, n3_, 1.0, ycur_, 1.0, del_, 0, nullptr, 0, nullptr, ycur_));
+      d = wrms(del_);
+    }
     if (!std::isfinite(d)) break;
     if (k > 0) crate = std::max(0.3 * crate, d / del_old);
     if (d * std::min(1.0, crate) * C <= kNewtonCoef) {
@@ -366,8 +382,13 @@ int Integrator::step_attempt(double h, int q, bool* accepted, double* err_out) {
     hnext_ = 0.25 * h;
     return 0;
   }
-  HTRY(vec_lincomb(s_, n3_, 1.0, ycur_, -1.0, yp_, 0, nullptr, 0, nullptr, tmp_));
-  const double err = C * wrms(tmp_);
+  double err = 0.0;
+  if (dev) {
+    err = C * std::sqrt(errsq / n3_);
+  } else {
+    HTRY(vec_lincomb(s_, n3_, 1.0, ycur_, -1.0, yp_, 0, nullptr, 0, nullptr, tmp_));
+    err = C * wrms(tmp_);
+  }
   *err_out = err;
   if (!std::isfinite(err)) {
     st_.conv_fails++;
@@ -413,8 +434,8 @@ int Integrator::advance(double t_final, int max_steps, bool exact_tstop, double*
   while (true) {
     if (max_steps > 0 && steps >= max_steps) break;
     if (exact_tstop ? (t_final - tn_ <= eps_t) : (tn_ >= t_final)) break;
-    HTRY(vec_ewt(s_, n3_, yn_, opt_.rtol, opt_.atol, ewt_));
-    if (hnext_ <= 0.0) {  // first step: h0 from ||f(y0)||
+    if (hnext_ <= 0.0) {  // first step: h0 from ||f(y0)|| (each attempt sets ewt again)
+      HTRY(vec_ewt(s_, n3_, yn_, opt_.rtol, opt_.atol, ewt_));
       TRY(device_rhs(h_, yn_, fn_));
       st_.rhs++;
       const double fnorm = wrms(fn_);

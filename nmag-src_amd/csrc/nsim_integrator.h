@@ -40,6 +40,7 @@ struct TsOptions {
   int precond = 1;                  // 0 none, 1 block Jacobi of I - gamma J
   int device_gmres = 1;             // 1: one persistent GMRES launch per solve,
                                     // 0: host-driven Arnoldi steps
+  int gmres_spw = 1;                // least SELL slices per GMRES workgroup
 };
 
 struct TsStats {
@@ -59,17 +60,27 @@ class Integrator {
   // The handle's m receives y at *t_reached.
   int advance(double t_final, int max_steps, bool exact_tstop, double* t_reached);
   const TsStats& stats() const { return st_; }
+  // workgroups of the persistent GMRES launch (0: host-driven GMRES only)
+  int gmres_grid() const { return ggrid_; }
   TsOptions& options() { return opt_; }
 
  private:
   int step_attempt(double h, int q, bool* accepted, double* err_out);
-  // x = (I - gamma J)^-1 r; *xw2 = sum (x ewt)^2
-  int gmres(const double* r, double* x, int* iters, double* xw2);
-  int gmres_device(const double* r, double* x, int* iters, double* xw2);
+  // x = (I - gamma J)^-1 r, host-driven Arnoldi steps
+  int gmres(const double* r, double* x, int* iters);
+  int newton_device(double* d, double* errsq);
   int apply_A(const double* z, double* w);
   double wrms(const double* x);
   int reduce(int nparts, int k, std::vector<double>* out);
-  int predictor(double h, int q, double* yp, double* E_p);
+  // Predictor yp = L0 y_n + L1 p1 + L2 p2 and its error constant E_p
+  // (y_pred - y_exact = -E_p y^(q+1)).
+  struct Predictor {
+    double L[3];
+    const double* p1;
+    const double* p2;
+    double E_p;
+  };
+  void predictor(double h, int q, Predictor* p) const;
   int check(hipError_t e, const char* what);
 
   nsim_handle* h_;

@@ -16,6 +16,12 @@ hipError_t vec_lincomb(hipStream_t st, int n, double c0, const double* x0, doubl
                        const double* x3, double* y);
 hipError_t vec_mul(hipStream_t st, int n, const double* x, const double* w, bool inverse,
                    double* y);
+// ewt = 1/(rtol |yn| + atol); yp = ycur = L0 yn + L1 p1 + L2 p2; avec = a1 yn + a2 a2v
+// (null vectors drop their terms).
+hipError_t vec_step_begin(hipStream_t st, int n, const double* yn, double L0, const double* p1,
+                          double L1, const double* p2, double L2, double a1, const double* a2v,
+                          double a2, double rtol, double atol, double* ewt, double* yp,
+                          double* ycur, double* avec);
 hipError_t vec_ewt(hipStream_t st, int n, const double* y, double rtol, double atol,
                    double* ewt);
 hipError_t vec_wsq_partials(hipStream_t st, int n, const double* x, const double* w,

@@ -35,6 +35,29 @@ __global__ __launch_bounds__(256) void k_lincomb(int n, double c0, const double*
   }
 }
 
+// Start of a BDF step attempt in one pass: ewt = 1/(rtol |yn| + atol), the
+// predictor yp = L0 yn + L1 p1 + L2 p2, ycur = yp, and avec = a1 yn + a2 a2v
+// (null p1, p2, a2v drop their terms).
+__global__ __launch_bounds__(256) void k_step_begin(int n, const double* yn, double L0,
+                                                    const double* p1, double L1,
+                                                    const double* p2, double L2, double a1,
+                                                    const double* a2v, double a2, double rtol,
+                                                    double atol, double* ewt, double* yp,
+                                                    double* ycur, double* avec) {
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+    const double y = yn[i];
+    ewt[i] = 1.0 / (rtol * fabs(y) + atol);
+    double v = L0 * y;
+    if (p1) v += L1 * p1[i];
+    if (p2) v += L2 * p2[i];
+    yp[i] = v;
+    ycur[i] = v;
+    double a = a1 * y;
+    if (a2v) a += a2 * a2v[i];
+    avec[i] = a;
+  }
+}
+
 // y = x * w (inverse: y = x / w)
 __global__ __launch_bounds__(256) void k_vmul(int n, const double* x, const double* w,
                                               int inverse, double* y) {
@@ -120,38 +143,7 @@ __global__ __launch_bounds__(256) void k_bjac_setup(int nn, const Coeffs* __rest
   const Coeffs& c = ct[inst ? inst[i] : 0];
   const double mi[3] = {m[3 * i], m[3 * i + 1], m[3 * i + 2]};
   const double Hi[3] = {H[3 * i], H[3 * i + 1], H[3 * i + 2]};
-  const double kp = kappa[i];
-  double M[3][3];
-  for (int col = 0; col < 3; col++) {
-    double v[3] = {0.0, 0.0, 0.0};
-    v[col] = 1.0;
-    const double h[3] = {kp * v[0], kp * v[1], kp * v[2]};
-    double vxH[3], mxh[3], mxH[3], vxmxH[3], mxvxH[3], mxmxh[3];
-    cross(v, Hi, vxH);
-    cross(mi, h, mxh);
-    cross(mi, Hi, mxH);
-    cross(v, mxH, vxmxH);
-    cross(mi, vxH, mxvxH);
-    cross(mi, mxh, mxmxh);
-    for (int r = 0; r < 3; r++) {
-      const double J = c.c1 * (vxH[r] + mxh[r]) + c.c2 * (vxmxH[r] + mxvxH[r] + mxmxh[r]);
-      M[r][col] = (r == col ? 1.0 : 0.0) - gamma * J;
-    }
-  }
-  const double det = M[0][0] * (M[1][1] * M[2][2] - M[1][2] * M[2][1]) -
-                     M[0][1] * (M[1][0] * M[2][2] - M[1][2] * M[2][0]) +
-                     M[0][2] * (M[1][0] * M[2][1] - M[1][1] * M[2][0]);
-  const double id = 1.0 / det;
-  double* o = Minv + 9 * i;
-  o[0] = (M[1][1] * M[2][2] - M[1][2] * M[2][1]) * id;
-  o[1] = (M[0][2] * M[2][1] - M[0][1] * M[2][2]) * id;
-  o[2] = (M[0][1] * M[1][2] - M[0][2] * M[1][1]) * id;
-  o[3] = (M[1][2] * M[2][0] - M[1][0] * M[2][2]) * id;
-  o[4] = (M[0][0] * M[2][2] - M[0][2] * M[2][0]) * id;
-  o[5] = (M[0][2] * M[1][0] - M[0][0] * M[1][2]) * id;
-  o[6] = (M[1][0] * M[2][1] - M[1][1] * M[2][0]) * id;
-  o[7] = (M[0][1] * M[2][0] - M[0][0] * M[2][1]) * id;
-  o[8] = (M[0][0] * M[1][1] - M[0][1] * M[1][0]) * id;
+  bjac_block(c.c1, c.c2, gamma, kappa[i], mi, Hi, Minv + 9 * i);
 }
 
 __global__ __launch_bounds__(256) void k_bjac_solve(int nn, const double* Minv, const double* r,
@@ -198,6 +190,15 @@ hipError_t vec_lincomb(hipStream_t st, int n, double c0, const double* x0, doubl
                        const double* x1, double c2, const double* x2, double c3,
                        const double* x3, double* y) {
   k_lincomb<<<grid_for(n), 256, 0, st>>>(n, c0, x0, c1, x1, c2, x2, c3, x3, y);
+  return hipGetLastError();
+}
+
+hipError_t vec_step_begin(hipStream_t st, int n, const double* yn, double L0, const double* p1,
+                          double L1, const double* p2, double L2, double a1, const double* a2v,
+                          double a2, double rtol, double atol, double* ewt, double* yp,
+                          double* ycur, double* avec) {
+  k_step_begin<<<grid_for(n), 256, 0, st>>>(n, yn, L0, p1, L1, p2, L2, a1, a2v, a2, rtol, atol,
+                                            ewt, yp, ycur, avec);
   return hipGetLastError();
 }
 
