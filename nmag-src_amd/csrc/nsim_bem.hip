@@ -111,7 +111,9 @@ __global__ __launch_bounds__(256) void k_bem_assemble(int nb, int ldb, int col_b
 
 // Storage type TB of B, type TX of the LDS-staged input vector, accumulation
 // type TA.  FP32 storage is accumulated in FP64 (the kernel is HBM-bound, the
-// FP64 FMAs are free); BF16 storage uses FP32 inputs and accumulation.
+// FP64 FMAs are free; an FP32 input, which would halve the LDS footprint,
+// measured no faster at the 20k-node slab and is not used); BF16 storage uses
+// FP32 inputs and accumulation.
 template <typename TB>
 struct BemVec;
 template <>
@@ -238,6 +240,38 @@ __global__ __launch_bounds__(64 * WAVES) void k_dense_gemv(
   for (; rb < r1; rb++) gemv_rows<TB, 1, X>(B, ldb, rb, xs, lane, map, y);
 }
 
+// Single-segment variant with a few workgroups per CU: x is staged once per
+// workgroup, then every wavefront claims rows from a device counter, the
+// claim of its next row issued before the current row is streamed.  Each
+// wavefront makes exactly one claim past the last row, so the counter ends at
+// n + total_waves; the last wavefront out (counted in ctr[1]) resets both
+// words, and launches need no host-side reset (graph replays included).
+template <typename TB, int WAVES, int KIND>
+__global__ __launch_bounds__(64 * WAVES) void k_dense_gemv_dyn(
+    const TB* __restrict__ B, int ldb, int nb, const double* __restrict__ x,
+    const int* __restrict__ map, double* __restrict__ y, unsigned* ctr, unsigned total_waves) {
+  using X = typename BemVec<TB>::X;
+  constexpr int T = 64 * WAVES;
+  extern __shared__ __align__(16) unsigned char smem[];
+  X* xs = reinterpret_cast<X*>(smem);
+  for (int j = threadIdx.x; j < ldb; j += T) xs[j] = j < nb ? (X)x[j] : (X)0;
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  unsigned r = 0;
+  if (lane == 0) r = atomicAdd(&ctr[0], 1u);
+  r = __shfl(r, 0, 64);
+  while (r < (unsigned)nb) {
+    unsigned next = 0;
+    if (lane == 0) next = atomicAdd(&ctr[0], 1u);
+    gemv_rows<TB, 1, X>(B, ldb, (int)r, xs, lane, map, y);
+    r = __shfl(next, 0, 64);
+  }
+  if (lane == 0 && atomicAdd(&ctr[1], 1u) == total_waves - 1) {
+    __hip_atomic_store(&ctr[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&ctr[1], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
 constexpr size_t kMaxLds = 160 * 1024;  // LDS per workgroup on gfx950
 
 template <typename TB, int ROWS, int WAVES, int KIND>
@@ -253,6 +287,30 @@ hipError_t launch_shape(const DenseOp& op, size_t lds, const double* x, double* 
       (const TB*)op.A, op.aoff, op.ld, op.n, op.off, op.wg_seg, op.wg_row0, op.rpw, x, op.map,
       y);
   return hipGetLastError();
+}
+
+template <typename TB, int WAVES, int KIND>
+hipError_t launch_dyn_shape(const DenseOp& op, size_t lds, const double* x, double* y,
+                            hipStream_t st) {
+  const void* k = reinterpret_cast<const void*>(&k_dense_gemv_dyn<TB, WAVES, KIND>);
+  if (lds > 64 * 1024) {
+    const hipError_t e =
+        hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+  }
+  k_dense_gemv_dyn<TB, WAVES, KIND><<<op.grid, 64 * WAVES, lds, st>>>(
+      (const TB*)op.A, op.h_ld[0], op.h_n[0], x + op.h_off[0], op.map + op.h_off[0], y, op.ctr,
+      (unsigned)(op.grid * WAVES));
+  return hipGetLastError();
+}
+
+template <typename TB, int KIND>
+hipError_t launch_dyn(const DenseOp& op, size_t lds, const double* x, double* y,
+                      hipStream_t st) {
+  if (!op.ctr || op.h_n.size() != 1 || op.h_aoff[0] != 0) return hipErrorInvalidConfiguration;
+  if (op.waves == 4) return launch_dyn_shape<TB, 4, KIND>(op, lds, x, y, st);
+  if (op.waves == 16) return launch_dyn_shape<TB, 16, KIND>(op, lds, x, y, st);
+  return hipErrorInvalidConfiguration;
 }
 
 template <typename TB, int ROWS, int KIND>
@@ -276,6 +334,9 @@ hipError_t gemv_typed(const DenseOp& op, const double* x, double* y, hipStream_t
   const size_t lds = (size_t)op.max_ld * sizeof(X);
   if (lds > kMaxLds) return hipErrorInvalidValue;
   switch (op.rows) {
+    case 0:
+      return op.kind == 1 ? launch_dyn<TB, 1>(op, lds, x, y, st)
+                          : launch_dyn<TB, 0>(op, lds, x, y, st);
     case 1: return launch_rows<TB, 1>(op, lds, x, y, st);
     case 2: return launch_rows<TB, 2>(op, lds, x, y, st);
     case 4: return launch_rows<TB, 4>(op, lds, x, y, st);
@@ -316,9 +377,23 @@ int dense_gemv_max_order(int type) {
 
 bool plan_dense_gemv(DenseOp& op, int num_cus, int rows_per_pass, int wg_per_cu, int waves,
                      std::vector<int>* wg_seg, std::vector<int>* wg_row0) {
-  if (rows_per_pass != 1 && rows_per_pass != 2 && rows_per_pass != 4) return false;
+  if (rows_per_pass != 0 && rows_per_pass != 1 && rows_per_pass != 2 && rows_per_pass != 4)
+    return false;
   if (wg_per_cu < 1 || wg_per_cu > 8) return false;
   if (waves != 4 && waves != 16) return false;
+  if (rows_per_pass == 0) {
+    // dynamic row claims: one segment, wg_per_cu workgroups per CU
+    if (op.h_n.size() != 1 || op.h_aoff[0] != 0) return false;
+    op.max_ld = op.h_ld[0];
+    op.rpw = 0;
+    op.rows = 0;
+    op.wg_per_cu = wg_per_cu;
+    op.waves = waves;
+    op.grid = wg_per_cu * (num_cus > 0 ? num_cus : 256);
+    wg_seg->assign(1, 0);
+    wg_row0->assign(1, 0);
+    return true;
+  }
   // wg_per_cu workgroups of `waves` wavefronts per CU; every wavefront gets
   // the same number of rows, every workgroup one segment
   long long rows = 0;

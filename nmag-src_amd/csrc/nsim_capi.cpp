@@ -93,11 +93,14 @@ void flush_timers(nsim_handle* h) {
 // tile shape and uploads its workgroup table.
 int setup_gemv(nsim_handle* h, DenseOp& op, int rows_per_pass, int wg_per_cu, int waves) {
   if (op.h_n.empty()) return 0;
+  if (!op.ctr) {  // row counter of the dynamic variant; it resets itself after use
+    HIPCHK(zeros(&op.ctr, 2));
+  }
   DenseOp trial = op;
   std::vector<int> wg_seg, wg_row0;
   if (!plan_dense_gemv(trial, h->num_cus, rows_per_pass, wg_per_cu, waves, &wg_seg, &wg_row0))
-    return fail("GEMV tile shape out of range (rows 1/2/4, workgroups per CU 1..8, "
-                "wavefronts per workgroup 4/16)");
+    return fail("GEMV tile shape out of range (rows 0/1/2/4 with 0 = dynamic rows on one "
+                "segment, workgroups per CU 1..8, wavefronts per workgroup 4/16)");
   int *ds = nullptr, *dr = nullptr;
   hipError_t e = upload(&ds, wg_seg.data(), wg_seg.size());
   if (e == hipSuccess) e = upload(&dr, wg_row0.data(), wg_row0.size());
@@ -647,6 +650,7 @@ void nsim_destroy(nsim_handle* h) {
   for (DenseOp* op : {&h->d.bem, &h->d.pot}) {
     if (op->wg_seg) (void)hipFree(op->wg_seg);
     if (op->wg_row0) (void)hipFree(op->wg_row0);
+    if (op->ctr) (void)hipFree(op->ctr);
   }
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
@@ -714,7 +718,7 @@ double tune_dense(nsim_handle* h, DenseOp& op, const double* x, double* y, hipEv
   double best = 1e30;
   int br = op.rows, bw = op.wg_per_cu, bv = op.waves;
   for (int v : {4, 16})
-    for (int r : {1, 2, 4})
+    for (int r : {0, 1, 2, 4})
       for (int w : {1, 2, 4, 8}) {
         if (setup_gemv(h, op, r, w, v) != 0) continue;
         bool ok = true;

@@ -35,8 +35,10 @@ struct DenseOp {
   int* wg_seg = nullptr;     // plan: workgroup -> segment
   int* wg_row0 = nullptr;    //       workgroup -> first row in the segment
   int grid = 0, rpw = 0, max_ld = 0;
-  int rows = 2, wg_per_cu = 2, waves = 4;  // tile shape: rows per pass, workgroups per
+  int rows = 2, wg_per_cu = 2, waves = 4;  // tile shape: rows per pass (0: rows
+                                           // claimed dynamically), workgroups per
                                            // CU, wavefronts per workgroup
+  unsigned* ctr = nullptr;                 // row counter of the dynamic variant
 };
 
 // Operators resident in HBM (one copy per precision actually used).

@@ -95,6 +95,14 @@ def lib():
             raise ImportError(
                 "libnsim_hip.so is not built (run __graft_entry__.build() or "
                 "make -C nmag-src_amd/csrc); there is no fallback compute path")
+        # PyTorch ships its own HIP runtime under the same soname.  Whichever
+        # is loaded first serves the whole process; if the library's came
+        # first, a later torch finds no GPU.  Load torch's first when present,
+        # so device tensors and the library share one runtime.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = ctypes.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(L, name)

@@ -83,6 +83,39 @@ def test_trajectory_matches_oracle_rhs_integrated_by_scipy():
     assert np.abs(avg(m) - avg(ref)).max() < 1e-6
 
 
+def test_bdf_order_from_step_counts():
+    """Local-error control at order q takes h ~ tol^(1/(q+1)), so cutting the
+    tolerance 100-fold multiplies the step count by about 100^(1/2) = 10 at
+    q = 1 and 100^(1/3) = 4.6 at q = 2; the global error of the macrospin
+    closed form also shrinks with the tolerance at each order."""
+    alpha, H = 0.1, 0.1
+    mat = material(alpha=alpha)
+    _, _, g = _magnet(1, 1, 1, 2.5, mat, do_demag=False)
+    th0, ph0 = 1.2, 0.3
+    m0 = [math.sin(th0) * math.cos(ph0), math.sin(th0) * math.sin(ph0), math.cos(th0)]
+    gp = mat.su_llg_gamma_G / (1 + alpha ** 2)
+    T = 200.0
+    th = 2 * math.atan(math.tan(th0 / 2) * math.exp(-alpha * gp * H * T))
+    ph = ph0 + gp * H * T
+    ref = [math.sin(th) * math.cos(ph), math.sin(th) * math.sin(ph), math.cos(th)]
+    ratio, err = {}, {}
+    for q in (1, 2):
+        steps = {}
+        for tol in (1e-6, 1e-8):
+            g.set_field("m", np.tile(m0, (g.n, 1)))
+            g.set_field("H_ext", np.tile([0.0, 0.0, H], (g.n, 1)))
+            g.ts_init(0.0, rtol=tol, atol=tol, max_order=q)
+            g.advance(T)
+            steps[tol] = g.stat("ts_steps")
+            err[(q, tol)] = np.abs(g.get_field("m") - ref).max()
+            assert g.stat("ts_order") == q
+        ratio[q] = steps[1e-8] / steps[1e-6]
+    assert 7.0 < ratio[1] < 14.0, ratio
+    assert 3.3 < ratio[2] < 6.5, ratio
+    for q in (1, 2):
+        assert err[(q, 1e-8)] < err[(q, 1e-6)] / 10
+
+
 def test_stop_semantics_and_statistics():
     mat = material(alpha=0.5)
     _, _, g = _magnet(4, 3, 2, 2.5, mat, rtol=1e-8)

@@ -169,3 +169,53 @@ def test_dense_potential_operator_properties():
     assert np.max(np.abs(gpu.get_field("phi") - phi_it)) <= 1e-10 * np.max(np.abs(phi_it))
     with pytest.raises(Exception):
         gpu.get_field("phi1")
+
+
+@pytest.mark.parametrize("route,precision", [("dense", "fp64"), ("dense", "mixed"),
+                                             ("iterative", "fp64"), ("iterative", "bf16")])
+def test_dynamic_row_gemv_matches_static_plan(route, precision):
+    """Rows claimed from a device counter (rows = 0) go through the same
+    per-row dot product as the static one-row plan, so every shape gives
+    bit-identical fields; the counter resets itself, so repeated launches and
+    graph replays keep doing so."""
+    coords, tets, orc, gpu, mat = make_pair(12, 8, 3, 2.5, precision=precision,
+                                            potential=route, build_bem=False)
+    op = "pot" if route == "dense" else "bem"
+    m = random_m(gpu.n, 17)
+    gpu.set_field("m", m)
+
+    def fields():
+        if route == "iterative":
+            gpu.set_field("phi1", np.zeros(gpu.n))
+            gpu.set_field("phi2", np.zeros(gpu.n))
+        gpu.execute()
+        return gpu.get_field("H_demag")
+
+    gpu.tune(op + "_rows", 1)
+    ref = fields()
+    for w in (1, 2, 8):
+        for v in (4, 16):
+            gpu.tune(op + "_rows", 0)
+            gpu.tune(op + "_wg_per_cu", w)
+            gpu.tune(op + "_waves", v)
+            assert gpu.stat(op + "_rows") == 0
+            for _ in range(3):
+                assert np.array_equal(fields(), ref), (w, v)
+    if route == "dense":  # graph replays of the dynamic variant
+        import torch
+        n3 = 3 * gpu.n
+        y = torch.tensor(m.reshape(-1), device="cuda:0")
+        o = torch.zeros(n3, dtype=torch.float64, device="cuda:0")
+        outs = []
+        for _ in range(4):  # seen, captured, replayed, replayed
+            o.zero_()
+            gpu.rhs_device(y.data_ptr(), o.data_ptr())
+            gpu.synchronize()
+            outs.append(o.cpu().numpy())
+        assert gpu.stat("graph_launches") >= 2
+        gpu.tune("rhs_graph", 0)
+        o.zero_()
+        gpu.rhs_device(y.data_ptr(), o.data_ptr())
+        gpu.synchronize()
+        for a in outs:
+            assert np.array_equal(a, o.cpu().numpy())
