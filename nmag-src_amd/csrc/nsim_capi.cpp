@@ -591,7 +591,7 @@ int create_impl(const char* who, int ninst, const nsim_mesh* meshes,
   // every layout pads by less than one part of at most 1024 rows
   ZEROS(h->cgw.pub, 2 * (n + 1024));
   ZEROS(h->cgw.part, 8 * (size_t)h->cgw.gmax);
-  ZEROS(h->cgw.tag, 2 * (size_t)h->cgw.gmax);
+  ZEROS(h->cgw.tag, 2 * (size_t)h->cgw.gmax * dev::kTagStride);
   ZEROS(h->sync_nbc, 8);
   ZEROS(h->sync_dbc, 8);
   h->cgw.sync = h->sync_nbc;

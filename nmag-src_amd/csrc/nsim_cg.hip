@@ -168,13 +168,13 @@ __device__ bool exchange(const CgArgs<TV>& a, __amdgpu_buffer_rsrc_t rpart, unsi
     st_coh(slot + 2, c);
     st_coh(slot + 3, cb);
     wait_all_memory();
-    st_coh(&a.tag[par * a.gmax + blockIdx.x], tg);
+    st_coh(&a.tag[tag_slot(par, a.gmax, blockIdx.x)], tg);
   }
   double v1 = 0.0, v2 = 0.0;
   int bad = 0;
   for (int t = threadIdx.x; t < G && !bad; t += 256) {
     unsigned spins = 0;
-    while (ld_coh(&a.tag[par * a.gmax + t]) != tg) {
+    while (ld_coh(&a.tag[tag_slot(par, a.gmax, t)]) != tg) {
       if ((++spins & 255u) == 0u && (ld_coh(&a.sync[2]) != 0u || spins > kSpinLimit)) {
         st_coh(&a.sync[2], 1u);
         bad = 1;

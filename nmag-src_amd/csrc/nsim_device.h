@@ -79,7 +79,7 @@ struct DeviceOps {
 struct CgWork {
   double* pub = nullptr;            // 2*n published vector (by phase parity)
   double* part = nullptr;           // 2*gmax*2 per-workgroup partial sums
-  unsigned long long* tag = nullptr;  // 2*gmax phase tags
+  unsigned long long* tag = nullptr;  // 2*gmax phase tags, dev::kTagStride words apart
   unsigned* sync = nullptr;         // [2] error flag [3] last iterations
                                     // [4] total iterations [5] solves
                                     // [6] solves stopped at maxit

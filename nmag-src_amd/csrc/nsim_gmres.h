@@ -6,18 +6,20 @@
 
 #include <cstddef>
 
+#include "nsim_dev_common.h"
 #include "nsim_device.h"
 
 namespace nsim {
 
 constexpr int kGmresMaxl = 512;  // largest Krylov dimension the kernel holds
+constexpr int kGmresTagStride = dev::kTagStride;  // tag words per workgroup and parity
 
 // Device buffers of the solver (owned by the caller).
 struct GmresWork {
   double* V = nullptr;        // (maxl + 1) * n3 Krylov basis, stride n3
   double* Z = nullptr;        // maxl * n3 preconditioned directions, exchanged
   double* part = nullptr;     // 2 * gmax * pstride partial sums
-  unsigned long long* tag = nullptr;  // 2 * gmax phase tags
+  unsigned long long* tag = nullptr;  // 2 * gmax phase tags, kGmresTagStride apart
   double* R = nullptr;        // maxl * maxl triangular factor (workgroup 0)
   double* y = nullptr;        // maxl: the least-squares solution, broadcast
   double* status = nullptr;   // [0] iterations [1] sum (x ewt)^2 [2] error flag

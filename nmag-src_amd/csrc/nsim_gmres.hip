@@ -125,12 +125,12 @@ __device__ bool exchange(const GmresArgs<TV>& a, GShared& sm, unsigned ph, int c
   for (int i = threadIdx.x; i < cnt; i += 256) st_coh(mine + i, sm.loc[i]);
   wait_all_memory();
   __syncthreads();
-  if (threadIdx.x == 0) st_coh(&a.tag[par * a.gmax + blockIdx.x], tg);
+  if (threadIdx.x == 0) st_coh(&a.tag[tag_slot(par, a.gmax, blockIdx.x)], tg);
   if (root_only && blockIdx.x != 0) return true;
   int bad = 0;
   for (int t = threadIdx.x; t < G && !bad; t += 256) {
     unsigned spins = 0;
-    while (ld_coh(&a.tag[par * a.gmax + t]) != tg) {
+    while (ld_coh(&a.tag[tag_slot(par, a.gmax, t)]) != tg) {
       if ((++spins & 255u) == 0u && (ld_coh(a.err) != 0u || spins > kSpinLimit)) {
         st_coh(a.err, 1u);
         bad = 1;
@@ -160,7 +160,7 @@ __device__ bool wait_root(const GmresArgs<TV>& a, unsigned ph) {
   int bad = 0;
   if (threadIdx.x == 0) {
     unsigned spins = 0;
-    while (ld_coh(&a.tag[(ph & 1) * a.gmax]) != tg) {
+    while (ld_coh(&a.tag[tag_slot(ph & 1, a.gmax, 0)]) != tg) {
       if ((++spins & 255u) == 0u && (ld_coh(a.err) != 0u || spins > kSpinLimit)) {
         st_coh(a.err, 1u);
         bad = 1;
@@ -388,7 +388,7 @@ __global__ __launch_bounds__(256) void k_gmres(GmresArgs<TV> a) {
           st_coh(a.y + i, sm.tot[i]);
         }
         wait_all_memory();
-        st_coh(&a.tag[(ph & 1) * a.gmax], (a.epoch << 24) | ph);
+        st_coh(&a.tag[tag_slot(ph & 1, a.gmax, 0)], (a.epoch << 24) | ph);
       }
       __syncthreads();
     } else {

@@ -103,10 +103,10 @@ Integrator::Integrator(nsim_handle* h, const TsOptions& o) : h_(h), opt_(o) {
     const size_t maxl = opt_.krylov_max;
     bool g = alloc(&gw_.Z, n * maxl) && alloc(&gw_.part, 2 * (size_t)grid * gw_.pstride) &&
              alloc(&gw_.R, maxl * maxl) && alloc(&gw_.y, maxl) && alloc(&gw_.status, 8);
-    if (g && hipMalloc(reinterpret_cast<void**>(&gw_.tag),
-                       2 * (size_t)grid * sizeof(unsigned long long)) == hipSuccess) {
+    const size_t tag_bytes = 2 * (size_t)grid * kGmresTagStride * sizeof(unsigned long long);
+    if (g && hipMalloc(reinterpret_cast<void**>(&gw_.tag), tag_bytes) == hipSuccess) {
       allocs_.push_back(gw_.tag);
-      g = hipMemset(gw_.tag, 0, 2 * (size_t)grid * sizeof(unsigned long long)) == hipSuccess;
+      g = hipMemset(gw_.tag, 0, tag_bytes) == hipSuccess;
     } else {
       g = false;
     }
