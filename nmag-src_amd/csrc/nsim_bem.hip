@@ -4,6 +4,8 @@
 
 #include <cmath>
 #include <cstdint>
+#include <mutex>
+#include <vector>
 
 #include "nsim_dev_common.h"
 #include "nsim_device.h"
@@ -169,6 +171,27 @@ __device__ inline void load_row_chunk<uint16_t, float>(const uint16_t* p, float*
   }
 }
 
+// xs[j] = x[j] for j < nb, 0 up to ldb, by T threads: eight loads in flight
+// per thread before any LDS store (a plain loop issues one load at a time
+// and waits on L2 latency before every LDS store).
+template <typename X, int T>
+__device__ inline void stage_x(X* xs, const double* __restrict__ x, int nb, int ldb) {
+  constexpr int U = 8;
+  for (int j0 = threadIdx.x; j0 < ldb; j0 += T * U) {
+    X v[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int j = j0 + u * T;
+      v[u] = j < nb ? (X)x[j] : (X)0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int j = j0 + u * T;
+      if (j < ldb) xs[j] = v[u];
+    }
+  }
+}
+
 // NR consecutive rows rb.. of a row-major segment (stride ldb) against the
 // staged x: y[map[rb + k]] = row . x.  Every lane loads 16 bytes of each row
 // per step; the partial sums are reduced across the wavefront.
@@ -230,7 +253,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_dense_gemv(
   const TB* B = Abase + seg_aoff[sg];
   const double* xb = xall + boff;
   const int* map = mapall + boff;
-  for (int j = threadIdx.x; j < ldb; j += T) xs[j] = j < nb ? (X)xb[j] : (X)0;
+  stage_x<X, T>(xs, xb, nb, ldb);
   __syncthreads();
   const int lane = threadIdx.x & 63;
   const int r0 = wg_row0[blockIdx.x] + (threadIdx.x >> 6) * rows_per_wave;
@@ -254,7 +277,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_dense_gemv_dyn(
   constexpr int T = 64 * WAVES;
   extern __shared__ __align__(16) unsigned char smem[];
   X* xs = reinterpret_cast<X*>(smem);
-  for (int j = threadIdx.x; j < ldb; j += T) xs[j] = j < nb ? (X)x[j] : (X)0;
+  stage_x<X, T>(xs, x, nb, ldb);
   __syncthreads();
   const int lane = threadIdx.x & 63;
   unsigned r = 0;
@@ -274,15 +297,35 @@ __global__ __launch_bounds__(64 * WAVES) void k_dense_gemv_dyn(
 
 constexpr size_t kMaxLds = 160 * 1024;  // LDS per workgroup on gfx950
 
+// Raises a kernel's dynamic-LDS limit above the 64 KiB default once per
+// kernel, device and size, so that a launch makes no runtime call besides
+// the launch itself.
+inline hipError_t allow_lds(const void* k, size_t lds) {
+  if (lds <= 64 * 1024) return hipSuccess;
+  struct Done {
+    const void* k;
+    int dev;
+    size_t lds;
+  };
+  static std::mutex mu;
+  static std::vector<Done> done;
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  std::lock_guard<std::mutex> lock(mu);
+  for (const Done& d : done)
+    if (d.k == k && d.dev == dev && d.lds >= lds) return hipSuccess;
+  e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e == hipSuccess) done.push_back(Done{k, dev, lds});
+  return e;
+}
+
 template <typename TB, int ROWS, int WAVES, int KIND>
 hipError_t launch_shape(const DenseOp& op, size_t lds, const double* x, double* y,
                         hipStream_t st) {
   const void* k = reinterpret_cast<const void*>(&k_dense_gemv<TB, ROWS, WAVES, KIND>);
-  if (lds > 64 * 1024) {
-    const hipError_t e =
-        hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    if (e != hipSuccess) return e;
-  }
+  const hipError_t e = allow_lds(k, lds);
+  if (e != hipSuccess) return e;
   k_dense_gemv<TB, ROWS, WAVES, KIND><<<op.grid, 64 * WAVES, lds, st>>>(
       (const TB*)op.A, op.aoff, op.ld, op.n, op.off, op.wg_seg, op.wg_row0, op.rpw, x, op.map,
       y);
@@ -293,11 +336,8 @@ template <typename TB, int WAVES, int KIND>
 hipError_t launch_dyn_shape(const DenseOp& op, size_t lds, const double* x, double* y,
                             hipStream_t st) {
   const void* k = reinterpret_cast<const void*>(&k_dense_gemv_dyn<TB, WAVES, KIND>);
-  if (lds > 64 * 1024) {
-    const hipError_t e =
-        hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    if (e != hipSuccess) return e;
-  }
+  const hipError_t e = allow_lds(k, lds);
+  if (e != hipSuccess) return e;
   k_dense_gemv_dyn<TB, WAVES, KIND><<<op.grid, 64 * WAVES, lds, st>>>(
       (const TB*)op.A, op.h_ld[0], op.h_n[0], x + op.h_off[0], op.map + op.h_off[0], y, op.ctr,
       (unsigned)(op.grid * WAVES));

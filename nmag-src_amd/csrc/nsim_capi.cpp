@@ -685,6 +685,12 @@ int nsim_set_tuning(nsim_handle* h, const char* key, int value) {
     if (h->ts) h->ts->options().device_gmres = h->ts_opt.device_gmres;
     return 0;
   }
+  if (k == "gmres_pc_sweeps") {
+    if (value < 0 || value > 8) return fail("gmres_pc_sweeps must be 0..8");
+    h->ts_opt.gmres_pc_sweeps = value;
+    if (h->ts) h->ts->options().gmres_pc_sweeps = value;
+    return 0;
+  }
   if (k == "gmres_slices_per_wg") {  // takes effect at the next nsim_ts_init
     if (value < 1 || value > 64) return fail("gmres_slices_per_wg must be 1..64");
     h->ts_opt.gmres_spw = value;

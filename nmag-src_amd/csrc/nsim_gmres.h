@@ -45,6 +45,7 @@ struct NewtonSystem {
   double* Minv;         // 9 per node: block-Jacobi inverses set up here, or null
   double* x;            // the correction
   double gamma, tol;
+  int sweeps = 0;       // local Jacobi sweeps over each workgroup's own nodes
 };
 
 // Solves (I - gamma J) x = avec + gamma f - ycur by right-preconditioned

@@ -44,10 +44,12 @@ using namespace dev;
 constexpr int kSplit = 4;                  // wavefronts per slice in the J.z gather
 constexpr unsigned kSpinLimit = 1u << 20;  // polls before an exchange gives up
 constexpr int kMaxGrid = 1024;
+constexpr int kPcNodes = 256;              // largest node block of the local sweeps
 
 template <typename TV>
 struct GmresArgs {
   int n, nslices, spw, maxl, gmax, pstride;
+  int sweeps;          // local Jacobi sweeps of the preconditioner (0: block Jacobi)
   const int* off;
   const int* col;
   const TV* K;
@@ -83,6 +85,8 @@ struct GShared {
   double loc[kGmresMaxl + 2];  // this workgroup's partial sums
   double tot[kGmresMaxl + 2];  // grid totals
   double gp[3 * kSplit][64];   // J.z gather partials
+  double pt[3 * kPcNodes];     // preconditioner: right-hand side on own nodes
+  double pz[2][3 * kPcNodes];  //                 sweep iterates
   double red[8];
   double hn2, vn2;  // |v|^2 from |v'|^2 - |h2|^2, and |v'|^2
 };
@@ -178,17 +182,85 @@ __device__ inline void block_total(double s, GShared& sm, int slot) {
   if (threadIdx.x == 0) sm.loc[slot] = s;
 }
 
-// Z_j = M^-1 (v / ewt) on this workgroup's nodes, published coherently.
+// Z_j = M^-1 t on this workgroup's nodes, published coherently, with
+// t = v / ewt (or, for the first direction, t = the Newton residual).  M is
+// the 3x3 block-Jacobi splitting A = D - N of A = I - gamma J, improved by
+// `sweeps` Jacobi sweeps z <- D^-1 (t + N z) over the couplings inside the
+// workgroup's own nodes (an additive-Schwarz block solve; couplings to other
+// workgroups' nodes are dropped, so M stays local and fixed).  N z at node i
+// is gamma (c1 m x h + c2 m x (m x h)) with h = negJ/V_i sum_{j != i} K_ij z_j.
 template <typename TV>
-__device__ void publish_direction(const GmresArgs<TV>& a, const double* v, int j, int r0,
-                                  int r1) {
+__device__ void publish_direction(const GmresArgs<TV>& a, GShared& sm, const double* v,
+                                  bool residual, int j, int s0, int s1, int r0, int r1) {
   double* Zj = a.Z + (size_t)j * a.ld;
+  const bool local = a.Minv && a.sweeps > 0 && r1 - r0 <= kPcNodes;
   for (int i = r0 + threadIdx.x; i < r1; i += 256) {
     double t[3], z[3];
-    for (int d = 0; d < 3; d++) t[d] = v[3 * i + d] / a.ewt[3 * i + d];
+    for (int d = 0; d < 3; d++) {
+      if (residual) {
+        double r = 1.0 * a.avec[3 * i + d];
+        r += a.gamma * a.f[3 * i + d];
+        r += -1.0 * a.m[3 * i + d];
+        t[d] = r;
+      } else {
+        t[d] = v[3 * i + d] / a.ewt[3 * i + d];
+      }
+    }
     pc_node(a.Minv, i, t, z);
-    for (int d = 0; d < 3; d++) st_coh(Zj + 3 * i + d, z[d]);
+    if (local) {
+      for (int d = 0; d < 3; d++) {
+        sm.pt[3 * (i - r0) + d] = t[d];
+        sm.pz[0][3 * (i - r0) + d] = z[d];
+      }
+    } else {
+      for (int d = 0; d < 3; d++) st_coh(Zj + 3 * i + d, z[d]);
+    }
   }
+  if (!local) return;
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  int cur = 0;
+  for (int sw = 0; sw < a.sweeps; sw++) {
+    __syncthreads();
+    for (int s = s0; s < s1; s++) {
+      const int row = 64 * s + lane;
+      const bool live = row < a.n;
+      const int b = a.off[s], w = (a.off[s + 1] - b) >> 6;
+      double hv[3] = {0.0, 0.0, 0.0};
+      if (live)
+        for (int q = wv; q < w; q += kSplit) {
+          const int e = b + (q << 6) + lane;
+          const int c = a.col[e];
+          if (c == row || c < r0 || c >= r1) continue;
+          const double kv = (double)a.K[e];
+          hv[0] += kv * sm.pz[cur][3 * (c - r0)];
+          hv[1] += kv * sm.pz[cur][3 * (c - r0) + 1];
+          hv[2] += kv * sm.pz[cur][3 * (c - r0) + 2];
+        }
+      for (int q = 0; q < 3; q++) sm.gp[wv * 3 + q][lane] = hv[q];
+      __syncthreads();
+      if (wv == 0 && live) {
+        for (int q = 0; q < 3; q++) {
+          double acc = sm.gp[q][lane];
+          for (int u = 1; u < kSplit; u++) acc += sm.gp[u * 3 + q][lane];
+          hv[q] = acc;
+        }
+        const Coeffs& cf = a.ct[a.inst ? a.inst[row] : 0];
+        const double fct = cf.negJ * a.Vinv[row];
+        const double h[3] = {fct * hv[0], fct * hv[1], fct * hv[2]};
+        const double mi[3] = {a.m[3 * row], a.m[3 * row + 1], a.m[3 * row + 2]};
+        const double zero[3] = {0.0, 0.0, 0.0};
+        double nz[3], t[3], z[3];
+        llg_jv_row(cf.c1, cf.c2, h, mi, zero, zero, nz);
+        for (int d = 0; d < 3; d++) t[d] = sm.pt[3 * (row - r0) + d] + a.gamma * nz[d];
+        pc_node(a.Minv, row, t, z);
+        for (int d = 0; d < 3; d++) sm.pz[cur ^ 1][3 * (row - r0) + d] = z[d];
+      }
+      __syncthreads();
+    }
+    cur ^= 1;
+  }
+  for (int i = r0 + threadIdx.x; i < r1; i += 256)
+    for (int d = 0; d < 3; d++) st_coh(Zj + 3 * i + d, sm.pz[cur][3 * (i - r0) + d]);
 }
 
 // sm.loc[i] = this workgroup's part of V_i . v for i <= j.
@@ -221,25 +293,14 @@ __global__ __launch_bounds__(256) void k_gmres(GmresArgs<TV> a) {
   // block-Jacobi inverses at the current iterate (its own nodes only);
   // v0 = r ewt, beta = |v0|; Z_0 = M^-1 r
   {
-    double* Z0 = a.Z;
-    for (int i = r0 + tid; i < r1; i += 256) {
-      double ri[3];
-      for (int d = 0; d < 3; d++) {
-        double v = 1.0 * a.avec[3 * i + d];
-        v += a.gamma * a.f[3 * i + d];
-        v += -1.0 * a.m[3 * i + d];
-        ri[d] = v;
-      }
-      if (a.Minv) {
+    if (a.Minv)
+      for (int i = r0 + tid; i < r1; i += 256) {
         const Coeffs& cf = a.ct[a.inst ? a.inst[i] : 0];
         const double mi[3] = {a.m[3 * i], a.m[3 * i + 1], a.m[3 * i + 2]};
         const double Hi[3] = {a.Ht[3 * i], a.Ht[3 * i + 1], a.Ht[3 * i + 2]};
         bjac_block(cf.c1, cf.c2, a.gamma, a.kappa[i], mi, Hi, a.Minv + 9 * i);
       }
-      double z[3];
-      pc_node(a.Minv, i, ri, z);
-      for (int d = 0; d < 3; d++) st_coh(Z0 + 3 * i + d, z[d]);
-    }
+    publish_direction(a, sm, nullptr, true, 0, s0, s1, r0, r1);
     double s = 0.0;
     for (int e = e0 + tid; e < e1; e += 256) {
       double r = 1.0 * a.avec[e];
@@ -319,7 +380,7 @@ __global__ __launch_bounds__(256) void k_gmres(GmresArgs<TV> a) {
     {
       double s = 0.0;
       for (int e = e0 + tid; e < e1; e += 256) s += Vn[e] * Vn[e];
-      if (j + 1 < a.maxl) publish_direction(a, Vn, j + 1, r0, r1);
+      if (j + 1 < a.maxl) publish_direction(a, sm, Vn, false, j + 1, s0, s1, r0, r1);
       block_total(s, sm, j + 1);
     }
     ok = exchange(a, sm, ++ph, j + 2, false);
@@ -446,8 +507,8 @@ const void* kernel_ptr() {
 template <typename TV>
 hipError_t launch_typed(const DeviceOps& d, const TV* K, GmresWork& w, int grid, int spw,
                         int maxl, const NewtonSystem& s, hipStream_t st) {
-  GmresArgs<TV> a{d.n,     d.nslices, spw,      maxl,     w.gmax,  w.pstride, d.sell_off,
-                  d.sell_col, K,      d.Vinv,   d.coefs,  d.inst,  s.ycur,    s.Ht,
+  GmresArgs<TV> a{d.n,     d.nslices, spw,      maxl,     w.gmax,  w.pstride, s.sweeps,
+                  d.sell_off, d.sell_col, K,  d.Vinv,   d.coefs,  d.inst,  s.ycur,    s.Ht,
                   s.ewt,   s.kappa,   s.Minv,   s.gamma,  s.tol,   s.avec,    s.f,
                   s.yp,    s.x,       w.V,      w.Z,      3 * (size_t)d.n,    w.part,
                   w.tag,   w.R,       w.y,      w.status, w.err,   ++w.epoch};

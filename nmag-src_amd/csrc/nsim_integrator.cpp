@@ -283,6 +283,7 @@ int Integrator::newton_device(double* d, double* errsq) {
   sys.x = del_;
   sys.gamma = gamma_;
   sys.tol = kLinCoef * kNewtonCoef * std::sqrt((double)n3_);
+  sys.sweeps = opt_.gmres_pc_sweeps;
   HTRY(launch_newton_gmres(handle_ops(h_), handle_mixed(h_), gw_, ggrid_, gspw_,
                            opt_.krylov_max, sys, s_));
   HTRY(hipMemcpyAsync(hhost_, gw_.status, 4 * sizeof(double), hipMemcpyDeviceToHost, s_));

@@ -41,6 +41,8 @@ struct TsOptions {
   int device_gmres = 1;             // 1: one persistent GMRES launch per solve,
                                     // 0: host-driven Arnoldi steps
   int gmres_spw = 1;                // least SELL slices per GMRES workgroup
+  int gmres_pc_sweeps = 1;          // local Jacobi sweeps of the persistent GMRES's
+                                    // preconditioner (0: 3x3 block Jacobi only)
 };
 
 struct TsStats {

@@ -195,15 +195,21 @@ def test_persistent_gmres_matches_host_driven_gmres(precision):
     m0 = random_m(n, 11)
     g.set_field("H_ext", np.tile([0.0, 0.02, 0.01], (n, 1)))
     res = {}
-    for dev in (0, 1):
+    # host-driven; persistent with the 3x3 block-Jacobi preconditioner; and
+    # persistent with two local Schwarz sweeps (fewer Krylov iterations)
+    for key, dev, sweeps in (("host", 0, 0), ("dev", 1, 0), ("dev_sweeps", 1, 2)):
         g.tune("gmres_device", dev)
+        g.tune("gmres_pc_sweeps", sweeps)
         g.set_field("m", m0)
         g.ts_init(0.0, rtol=1e-7, atol=1e-7)
         assert g.advance(20.0) == pytest.approx(20.0)
-        res[dev] = (g.get_field("m"),
+        res[key] = (g.get_field("m"),
                     {k: g.stat("ts_" + k) for k in ("steps", "newton", "linear")})
-    s0, s1 = res[0][1], res[1][1]
+    s0 = res["host"][1]
     assert s0["steps"] > 20
-    assert abs(s0["steps"] - s1["steps"]) <= max(2, 0.02 * s0["steps"])
-    assert abs(s0["linear"] - s1["linear"]) <= 0.05 * s0["linear"] + 5
-    assert np.abs(res[0][0] - res[1][0]).max() < 1e-6
+    for key in ("dev", "dev_sweeps"):
+        s1 = res[key][1]
+        assert abs(s0["steps"] - s1["steps"]) <= max(2, 0.02 * s0["steps"]), key
+        assert np.abs(res["host"][0] - res[key][0]).max() < 1e-6, key
+    assert abs(s0["linear"] - res["dev"][1]["linear"]) <= 0.05 * s0["linear"] + 5
+    assert res["dev_sweeps"][1]["linear"] < s0["linear"]
