@@ -243,7 +243,8 @@ __global__ __launch_bounds__(64 * WAVES) void k_dense_gemv(
     const int* __restrict__ seg_ld, const int* __restrict__ seg_n,
     const int* __restrict__ seg_off, const int* __restrict__ wg_seg,
     const int* __restrict__ wg_row0, int rows_per_wave, const double* __restrict__ xall,
-    const int* __restrict__ mapall, double* __restrict__ y) {
+    const int* __restrict__ mapall, double* __restrict__ y, const int* __restrict__ halt) {
+  if (halt && *halt) return;  // queued by a finished integrator run
   using X = typename BemVec<TB>::X;
   constexpr int T = 64 * WAVES;
   extern __shared__ __align__(16) unsigned char smem[];
@@ -272,7 +273,9 @@ __global__ __launch_bounds__(64 * WAVES) void k_dense_gemv(
 template <typename TB, int WAVES, int KIND>
 __global__ __launch_bounds__(64 * WAVES) void k_dense_gemv_dyn(
     const TB* __restrict__ B, int ldb, int nb, const double* __restrict__ x,
-    const int* __restrict__ map, double* __restrict__ y, unsigned* ctr, unsigned total_waves) {
+    const int* __restrict__ map, double* __restrict__ y, unsigned* ctr, unsigned total_waves,
+    const int* __restrict__ halt) {
+  if (halt && *halt) return;  // every wavefront: the counter stays untouched
   using X = typename BemVec<TB>::X;
   constexpr int T = 64 * WAVES;
   extern __shared__ __align__(16) unsigned char smem[];
@@ -328,7 +331,7 @@ hipError_t launch_shape(const DenseOp& op, size_t lds, const double* x, double* 
   if (e != hipSuccess) return e;
   k_dense_gemv<TB, ROWS, WAVES, KIND><<<op.grid, 64 * WAVES, lds, st>>>(
       (const TB*)op.A, op.aoff, op.ld, op.n, op.off, op.wg_seg, op.wg_row0, op.rpw, x, op.map,
-      y);
+      y, op.halt);
   return hipGetLastError();
 }
 
@@ -340,7 +343,7 @@ hipError_t launch_dyn_shape(const DenseOp& op, size_t lds, const double* x, doub
   if (e != hipSuccess) return e;
   k_dense_gemv_dyn<TB, WAVES, KIND><<<op.grid, 64 * WAVES, lds, st>>>(
       (const TB*)op.A, op.h_ld[0], op.h_n[0], x + op.h_off[0], op.map + op.h_off[0], y, op.ctr,
-      (unsigned)(op.grid * WAVES));
+      (unsigned)(op.grid * WAVES), op.halt);
   return hipGetLastError();
 }
 

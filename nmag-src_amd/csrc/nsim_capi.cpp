@@ -331,6 +331,13 @@ const int* handle_inst(nsim_handle* h) { return h->d.inst; }
 const DeviceOps& handle_ops(nsim_handle* h) { return h->d; }
 bool handle_mixed(nsim_handle* h) { return h->mixed; }
 int handle_device(nsim_handle* h) { return h->device; }
+void handle_set_halt(nsim_handle* h, const int* halt) {
+  drop_rhs_graphs(h);  // their kernel arguments hold the old word
+  h->d.halt = halt;
+  h->d.pot.halt = halt;
+  h->d.bem.halt = halt;
+  h->cgw.halt = halt;
+}
 int handle_error_flags(nsim_handle* h, unsigned* out) {
   unsigned a[6], b[6];
   HIPCHK(hipMemcpyAsync(a, h->sync_nbc, sizeof(a), hipMemcpyDeviceToHost, h->stream));
@@ -678,6 +685,12 @@ int nsim_set_tuning(nsim_handle* h, const char* key, int value) {
   }
   if (k == "cg_coarse") {
     h->cg_coarse = value != 0;
+    return 0;
+  }
+  if (k == "cooperative_launch") {  // persistent CG / GMRES launches
+    h->cgw.cooperative = value != 0;
+    h->ts_opt.cooperative = value != 0;
+    if (h->ts) h->ts->options().cooperative = value != 0;
     return 0;
   }
   if (k == "gmres_device") {

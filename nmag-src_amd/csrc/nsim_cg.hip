@@ -62,6 +62,7 @@ struct CgArgs {
   double* xsum;
   double* rout;             // optional: rout[blocal[i]] = x_i on boundary nodes
   const int* blocal;
+  const int* halt;          // nonzero word: every workgroup returns at once
 };
 
 struct Shared {
@@ -248,6 +249,7 @@ __device__ double load_norm_coarse(const CgArgs<TV>& a, __amdgpu_buffer_rsrc_t r
 // measured against the load vector.
 template <typename TV, int RPT, int MAXW>
 __global__ __launch_bounds__(256) void k_cg(CgArgs<TV> a) {
+  if (a.halt && *a.halt) return;  // queued by a finished integrator run
   __shared__ Shared sm;
   const int n = a.n;
   const bool coarse = a.naggs > 0;
@@ -428,9 +430,10 @@ hipError_t launch_cg(const CgDevice& c, CgWork& w, bool dirichlet, bool coarse,
   const int naggs = coarse ? c.naggs : 0;
   CgArgs<double> a{c.npad, w.gmax, maxit, c.cap_shift, c.agg_shift, naggs, c.sell_off,
                    c.sell_col, c.K, c.p2o, dinv, ainv, b, x, w.pub, w.part, w.tag, w.sync,
-                   epoch, rtol * rtol, xadd, xsum, restrict_out, blocal};
+                   epoch, rtol * rtol, xadd, xsum, restrict_out, blocal, w.halt};
   void* args[] = {&a};
-  return hipLaunchCooperativeKernel(k, dim3(grid), dim3(256), args, 0, st);
+  if (w.cooperative) return hipLaunchCooperativeKernel(k, dim3(grid), dim3(256), args, 0, st);
+  return hipLaunchKernel(k, dim3(grid), dim3(256), args, 0, st);
 }
 
 }  // namespace nsim

@@ -39,6 +39,7 @@ struct DenseOp {
                                            // claimed dynamically), workgroups per
                                            // CU, wavefronts per workgroup
   unsigned* ctr = nullptr;                 // row counter of the dynamic variant
+  const int* halt = nullptr;               // as DeviceOps::halt
 };
 
 // Operators resident in HBM (one copy per precision actually used).
@@ -61,6 +62,9 @@ struct DeviceOps {
   std::vector<double> h_bem_geo_eps;  // per segment geometric tolerance
   DenseOp pot;                      // dense demag potential operator (A null: iterative)
   int* iota = nullptr;              // identity map of the nodes
+  const int* halt = nullptr;        // nonzero word: the field kernels return at once
+                                    // (set by a device-driven integrator run that
+                                    // has finished while kernels are still queued)
   // coefficients: one entry per magnet; inst[i] = magnet of node i (null
   // when the handle holds a single magnet)
   Coeffs* coefs = nullptr;
@@ -85,6 +89,12 @@ struct CgWork {
                                     // [6] solves stopped at maxit
   int gmax = 0;
   unsigned long long epoch = 0;     // launch counter; makes every phase tag unique
+  const int* halt = nullptr;        // as DeviceOps::halt
+  // The CG grid never exceeds what the device holds at once and the stream
+  // orders it after and before other work, so a plain launch keeps its
+  // workgroups co-resident; a cooperative launch guarantees it at about
+  // 20 us more dispatch cost.  Every exchange wait is bounded either way.
+  bool cooperative = false;
 };
 
 // Field inputs of one evaluation (device pointers; optional ones may be null).

@@ -26,6 +26,12 @@ struct GmresWork {
   unsigned* err = nullptr;    // exchange error flag
   int gmax = 0, pstride = 0;
   unsigned long long epoch = 0;  // launch counter; makes every phase tag unique
+  // The grid never exceeds what the device holds at once (gmres_grid), and
+  // the launches before and after it on the stream wait for it, so its
+  // workgroups are co-resident with a plain launch; a cooperative launch
+  // guarantees the same at about 20 us more dispatch cost per launch
+  // (measured on MI355X).  Every exchange wait is bounded either way.
+  bool cooperative = false;
 };
 
 // Grid of the kernel for a handle with `nslices` SELL slices: workgroups
@@ -46,6 +52,8 @@ struct NewtonSystem {
   double* x;            // the correction
   double gamma, tol;
   int sweeps = 0;       // local Jacobi sweeps over each workgroup's own nodes
+  const double* gamma_p = nullptr;  // device word holding gamma (overrides gamma)
+  const int* halt = nullptr;        // nonzero device word: the launch does nothing
 };
 
 // Solves (I - gamma J) x = avec + gamma f - ycur by right-preconditioned

@@ -76,6 +76,8 @@ struct GmresArgs {
   double* status;
   unsigned* err;
   unsigned long long epoch;
+  const double* gamma_p;  // if set, gamma is read here (set by the device controller)
+  const int* halt;        // nonzero word: every workgroup returns at once
 };
 
 struct GShared {
@@ -279,6 +281,8 @@ __device__ void local_dots(const GmresArgs<TV>& a, GShared& sm, const double* v,
 
 template <typename TV>
 __global__ __launch_bounds__(256) void k_gmres(GmresArgs<TV> a) {
+  if (a.halt && *a.halt) return;  // queued by a finished integrator run
+  if (a.gamma_p) a.gamma = *a.gamma_p;
   __shared__ GShared sm;
   const int s0 = blockIdx.x * a.spw, s1 = min(a.nslices, s0 + a.spw);
   const int r0 = 64 * s0, r1 = min(a.n, 64 * s1);
@@ -511,9 +515,12 @@ hipError_t launch_typed(const DeviceOps& d, const TV* K, GmresWork& w, int grid,
                   d.sell_off, d.sell_col, K,  d.Vinv,   d.coefs,  d.inst,  s.ycur,    s.Ht,
                   s.ewt,   s.kappa,   s.Minv,   s.gamma,  s.tol,   s.avec,    s.f,
                   s.yp,    s.x,       w.V,      w.Z,      3 * (size_t)d.n,    w.part,
-                  w.tag,   w.R,       w.y,      w.status, w.err,   ++w.epoch};
+                  w.tag,   w.R,       w.y,      w.status, w.err,   ++w.epoch,
+                  s.gamma_p, s.halt};
   void* args[] = {&a};
-  return hipLaunchCooperativeKernel(kernel_ptr<TV>(), dim3(grid), dim3(256), args, 0, st);
+  if (w.cooperative)
+    return hipLaunchCooperativeKernel(kernel_ptr<TV>(), dim3(grid), dim3(256), args, 0, st);
+  return hipLaunchKernel(kernel_ptr<TV>(), dim3(grid), dim3(256), args, 0, st);
 }
 
 }  // namespace

@@ -1,7 +1,7 @@
 // Device-resident BDF / Newton-GMRES time integrator (see nsim_integrator.h).
 //
-// Step controller (host) -- all vectors stay in HBM, the host sees only the
-// reduction partials of norms and inner products:
+// All vectors stay in HBM.  The step controller (nsim_tsctl.h) is the same
+// code on host and device:
 //  * BDF of order q <= 2 on the actual (variable) step history: the corrector
 //    is P'(t_{n+1}) = f(y_{n+1}) for the polynomial P through y_{n+1..n+1-q}.
 //  * predictor = extrapolation of the polynomial through y_{n..n-q} (explicit
@@ -9,8 +9,12 @@
 //    err = E_c/(E_c+E_p) ||y - y_pred||_WRMS, with the variable-step error
 //    constants of both formulas computed from the node positions.
 //  * Newton on y - gamma f(y) - a = 0 with (I - gamma J) solved by scaled GMRES
-//    (CGS2 orthogonalisation), J.v from the jacobi_ts1 linearisation and a 3x3
+//    (CGS2 orthogonalisation), J.v from the jacobi_ts1 linearisation and a
 //    block-Jacobi right preconditioner; WRMS weights 1/(rtol|y| + atol).
+// With the persistent GMRES an advance call queues Newton cycles in batches
+// (k_ts_begin, the field evaluation, k_gmres, k_ts_ctrl); a finished run
+// sets a halt word that turns the cycles still queued into empty launches.
+// The host-driven GMRES runs one cycle at a time and reads the state back.
 #include "nsim_integrator.h"
 
 #include <algorithm>
@@ -23,49 +27,18 @@
 
 namespace nsim {
 
-namespace {
-
-constexpr int kMaxCor = 3;          // Newton iterations per attempt (CVODE NLS_MAXCOR)
-constexpr double kNewtonCoef = 0.1; // Newton convergence coefficient
-constexpr double kLinCoef = 0.05;   // linear-solve tolerance factor (CVODE eplifac)
-constexpr int kMaxAttempts = 20;
-
-// l_j'(T[0]) of the Lagrange basis polynomial of node j among T[0..m].
-double lagrange_deriv0(const double* T, int m, int j) {
-  if (j == 0) {
-    double s = 0.0;
-    for (int i = 1; i <= m; i++) s += 1.0 / (T[0] - T[i]);
-    return s;
-  }
-  double num = 1.0, den = 1.0;
-  for (int i = 0; i <= m; i++) {
-    if (i == j) continue;
-    den *= T[j] - T[i];
-    if (i != 0) num *= T[0] - T[i];
-  }
-  return num / den;
-}
-
-double factorial(int k) {
-  double f = 1.0;
-  for (int i = 2; i <= k; i++) f *= i;
-  return f;
-}
-
-}  // namespace
-
-int Integrator::check(hipError_t e, const char* what) {
-  if (e == hipSuccess) return 0;
-  set_error(std::string("integrator: ") + what + ": " + hipGetErrorString(e));
-  return -1;
-}
-
 #define TRY(expr)                     \
   do {                                \
     int _rc = (expr);                 \
     if (_rc) return _rc;              \
   } while (0)
 #define HTRY(expr) TRY(check((expr), #expr))
+
+int Integrator::check(hipError_t e, const char* what) {
+  if (e == hipSuccess) return 0;
+  set_error(std::string("integrator: ") + what + ": " + hipGetErrorString(e));
+  return -1;
+}
 
 Integrator::Integrator(nsim_handle* h, const TsOptions& o) : h_(h), opt_(o) {
   s_ = handle_stream(h);
@@ -80,10 +53,10 @@ Integrator::Integrator(nsim_handle* h, const TsOptions& o) : h_(h), opt_(o) {
   };
   const size_t n = n3_;
   np_ = vec_reduce_grid(n3_);
-  ok_ = alloc(&yn_, n) && alloc(&ynm1_, n) && alloc(&ynm2_, n) && alloc(&fn_, n) &&
+  ok_ = alloc(&slot_[0], n) && alloc(&slot_[1], n) && alloc(&slot_[2], n) && alloc(&fn_, n) &&
         alloc(&yp_, n) && alloc(&ycur_, n) && alloc(&avec_, n) && alloc(&f_, n) &&
         alloc(&res_, n) && alloc(&del_, n) && alloc(&tmp_, n) && alloc(&tmp2_, n) &&
-        alloc(&ewt_, n) && alloc(&ystart_, n) &&
+        alloc(&ewt_, n) && alloc(&ystart_, n) && alloc(&status_, 8) &&
         alloc(&V_, n * (size_t)(opt_.krylov_max + 1)) && alloc(&Minv_, 9 * (size_t)nn_) &&
         alloc(&coef_, (size_t)opt_.krylov_max + 8) &&
         alloc(&hdev_, 2 * (size_t)opt_.krylov_max + 8) &&
@@ -92,11 +65,24 @@ Integrator::Integrator(nsim_handle* h, const TsOptions& o) : h_(h), opt_(o) {
   if (ok_ && hipHostMalloc(reinterpret_cast<void**>(&hhost_),
                            (2 * (size_t)opt_.krylov_max + 8) * sizeof(double)) != hipSuccess)
     ok_ = false;
+  // controller state: device copy, pinned mirrors, the ring's pointer table
+  if (ok_) {
+    ok_ = hipMalloc(reinterpret_cast<void**>(&ds_), sizeof(TsState)) == hipSuccess &&
+          hipMemset(ds_, 0, sizeof(TsState)) == hipSuccess &&
+          hipHostMalloc(reinterpret_cast<void**>(&pin_), 3 * sizeof(TsState)) == hipSuccess &&
+          hipMalloc(reinterpret_cast<void**>(&slots_dev_), 3 * sizeof(double*)) == hipSuccess &&
+          hipMemcpy(slots_dev_, slot_, 3 * sizeof(double*), hipMemcpyHostToDevice) ==
+              hipSuccess &&
+          hipEventCreateWithFlags(&ev_[0], hipEventDisableTiming) == hipSuccess &&
+          hipEventCreateWithFlags(&ev_[1], hipEventDisableTiming) == hipSuccess;
+    if (ds_) allocs_.push_back(ds_);
+    if (slots_dev_) allocs_.push_back(slots_dev_);
+  }
   // persistent GMRES: one co-resident grid over the SELL slices
   const DeviceOps& d = handle_ops(h);
   int grid = 0, spw = 0;
   if (ok_ && nsim::gmres_grid(d.nslices, opt_.krylov_max, handle_device(h), 1 << 20,
-                        std::max(1, opt_.gmres_spw), &grid, &spw)) {
+                              std::max(1, opt_.gmres_spw), &grid, &spw)) {
     gw_.V = V_;
     gw_.gmax = grid;
     gw_.pstride = (opt_.krylov_max + 2 + 7) & ~7;
@@ -121,12 +107,17 @@ Integrator::Integrator(nsim_handle* h, const TsOptions& o) : h_(h), opt_(o) {
       gspw_ = spw;
     }
   }
+  if (ok_) handle_set_halt(h_, &ds_->halt);
 }
 
 Integrator::~Integrator() {
   (void)hipStreamSynchronize(s_);
+  if (ok_) handle_set_halt(h_, nullptr);
   for (void* p : allocs_) (void)hipFree(p);
   if (hhost_) (void)hipHostFree(hhost_);
+  if (pin_) (void)hipHostFree(pin_);
+  for (hipEvent_t e : ev_)
+    if (e) (void)hipEventDestroy(e);
 }
 
 int Integrator::reduce(int nparts, int k, std::vector<double>* out) {
@@ -152,17 +143,36 @@ int Integrator::init(double t0) {
     set_error("integrator: device allocation failed");
     return -1;
   }
-  HTRY(hipMemcpyAsync(yn_, handle_m(h_), sizeof(double) * n3_, hipMemcpyDeviceToDevice, s_));
+  HTRY(hipMemcpyAsync(slot_[0], handle_m(h_), sizeof(double) * n3_, hipMemcpyDeviceToDevice,
+                      s_));
   HTRY(hipStreamSynchronize(s_));
-  tn_ = t0;
-  nhist_ = 0;
-  hnext_ = 0.0;
-  qnext_ = 1;
-  gamma_pc_ = -1.0;
-  crate_ = 1.0;
-  gamma_crate_ = 0.0;
+  hs_ = TsState{};
+  hs_.tn = hs_.tnm1 = hs_.tnm2 = t0;
+  hs_.sn = 0;
+  hs_.snm1 = 1;
+  hs_.snm2 = 2;
+  hs_.hnext = 0.0;
+  hs_.qnext = 1;
+  hs_.max_order = opt_.max_order;
+  hs_.crate = 1.0;
+  hs_.gamma_crate = 0.0;
+  hs_.copy = kSlotNone;
+  hs_.mode = kTsDone;
   st_ = TsStats();
   st_.t = t0;
+  return 0;
+}
+
+int Integrator::upload_state() {
+  pin_[2] = hs_;
+  HTRY(hipMemcpyAsync(ds_, &pin_[2], sizeof(TsState), hipMemcpyHostToDevice, s_));
+  return 0;
+}
+
+int Integrator::download_state() {
+  HTRY(hipMemcpyAsync(&pin_[2], ds_, sizeof(TsState), hipMemcpyDeviceToHost, s_));
+  HTRY(hipStreamSynchronize(s_));
+  hs_ = pin_[2];
   return 0;
 }
 
@@ -242,7 +252,6 @@ int Integrator::gmres(const double* r, double* x, int* iters) {
     g[j + 1] = -sn[j] * g[j];
     g[j] = cs[j] * g[j];
     k = j + 1;
-    st_.linear++;
     if (std::fabs(g[j + 1]) <= tol || hn == 0.0) break;
   }
   // back substitution, then x = M^-1 (sum y_i V_i / ewt)
@@ -265,12 +274,12 @@ int Integrator::gmres(const double* r, double* x, int* iters) {
   return 0;
 }
 
-// One Newton iteration after the rhs at ycur: residual, block-Jacobi set-up,
-// GMRES, ycur += del and the weighted norms in one persistent launch
-// (nsim_gmres.hip); the host reads the iteration count, the correction's
-// WRMS norm *d and the sum of squares of the local error with one
-// synchronisation.
-int Integrator::newton_device(double* d, double* errsq) {
+// One Newton cycle of the device-driven run: start of attempt (when the
+// controller asks for one), field evaluation at ycur, the persistent GMRES
+// (residual, preconditioner, solve, ycur += delta, norms) and the controller.
+int Integrator::enqueue_cycle() {
+  HTRY(launch_ts_begin(s_, n3_, ds_, slots_dev_, fn_, ycur_, yp_, avec_, ewt_));
+  TRY(device_rhs(h_, ycur_, f_));
   NewtonSystem sys;
   sys.ycur = ycur_;
   sys.Ht = handle_Htotal(h_);
@@ -281,146 +290,80 @@ int Integrator::newton_device(double* d, double* errsq) {
   sys.kappa = handle_kappa(h_);
   sys.Minv = opt_.precond ? Minv_ : nullptr;
   sys.x = del_;
-  sys.gamma = gamma_;
+  sys.gamma = 0.0;
+  sys.gamma_p = &ds_->gamma;
+  sys.halt = &ds_->halt;
   sys.tol = kLinCoef * kNewtonCoef * std::sqrt((double)n3_);
   sys.sweeps = opt_.gmres_pc_sweeps;
+  gw_.cooperative = opt_.cooperative;
   HTRY(launch_newton_gmres(handle_ops(h_), handle_mixed(h_), gw_, ggrid_, gspw_,
                            opt_.krylov_max, sys, s_));
-  HTRY(hipMemcpyAsync(hhost_, gw_.status, 4 * sizeof(double), hipMemcpyDeviceToHost, s_));
-  HTRY(hipStreamSynchronize(s_));
-  if (hhost_[2] != 0.0) {
-    (void)hipMemset(gw_.err, 0, sizeof(unsigned));
-    set_error("integrator: GMRES exchange timed out");
-    return -1;
-  }
-  st_.linear += (long)hhost_[0];
-  *d = std::sqrt(hhost_[1] / n3_);
-  *errsq = hhost_[3];
+  HTRY(launch_ts_ctrl(s_, ds_, gw_.status, n3_));
   return 0;
 }
 
-// Predictor into yp; E_p is the predictor's error constant (coefficient of
-// y^(q+1), with y_pred - y_exact = -E_p y^(q+1)).
-// yp = L0 y_n + L1 p1 + L2 p2 (null p drop their terms).
-void Integrator::predictor(double h, int q, Predictor* p) const {
-  const double t1 = tn_ + h;
-  if (nhist_ == 0) {  // explicit Euler from f(y_n)
-    *p = Predictor{{1.0, h, 0.0}, fn_, nullptr, 0.5 * h * h};
-    return;
+// Queues batches of cycles, two in flight: while one runs the host waits for
+// the other's copy of the state, stops when it reads "done", and sizes the
+// next batch from the steps left at the current step size.
+int Integrator::run_device_cycles() {
+  const int maxb = std::max(1, std::min(opt_.max_batch, 64));
+  int batches = 0, waited = 0;
+  int batch = std::min(maxb, 4);
+  auto enqueue_batch = [&](int cycles) -> int {
+    for (int c = 0; c < cycles; c++) TRY(enqueue_cycle());
+    TsState* mirror = &pin_[batches & 1];
+    HTRY(hipMemcpyAsync(mirror, ds_, sizeof(TsState), hipMemcpyDeviceToHost, s_));
+    HTRY(hipEventRecord(ev_[batches & 1], s_));
+    batches++;
+    return 0;
+  };
+  TRY(enqueue_batch(batch));
+  TRY(enqueue_batch(batch));
+  while (true) {
+    HTRY(hipEventSynchronize(ev_[waited & 1]));
+    const TsState m = pin_[waited & 1];
+    waited++;
+    if (m.mode == kTsDone || m.mode == kTsError || m.halt) break;
+    // cycles still needed: about one per step left (a few more for retries)
+    double left = 1.0;
+    if (m.max_steps > 0) left = m.max_steps - m.steps_call;
+    if (m.h > 0.0) {
+      const double steps = std::ceil((m.t_final - m.tn) / m.h);
+      left = m.max_steps > 0 ? std::min(left, steps) : steps;
+    }
+    batch = (int)std::min<double>(maxb, std::max(1.0, left + 1.0));
+    TRY(enqueue_batch(batch));
   }
-  const double P[3] = {tn_, tnm1_, tnm2_};
-  double L[3] = {0, 0, 0}, ep = 1.0;
-  for (int j = 0; j <= q; j++) {
-    double w = 1.0;
-    for (int i = 0; i <= q; i++)
-      if (i != j) w *= (t1 - P[i]) / (P[j] - P[i]);
-    L[j] = w;
-    ep *= t1 - P[j];
-  }
-  *p = Predictor{{L[0], L[1], q >= 2 ? L[2] : 0.0}, ynm1_, q >= 2 ? ynm2_ : nullptr,
-                 ep / factorial(q + 1)};
+  return download_state();
 }
 
-int Integrator::step_attempt(double h, int q, bool* accepted, double* err_out) {
-  *accepted = false;
-  const double T[3] = {tn_ + h, tn_, tnm1_};
-  const double* Y[3] = {nullptr, yn_, ynm1_};
-  double dl[3];
-  for (int j = 0; j <= q; j++) dl[j] = lagrange_deriv0(T, q, j);
-  gamma_ = 1.0 / dl[0];
-  double R = 0.0;
-  for (int j = 1; j <= q; j++) R += dl[j] * std::pow(T[j] - T[0], q + 1) / factorial(q + 1);
-  const double E_c = -R / dl[0];
-  Predictor pr;
-  predictor(h, q, &pr);
-  const double C = E_c / (E_c + pr.E_p);
-  // one pass: error weights of y_n, the predictor yp (= ycur), and
-  // a = -gamma sum_{j>=1} l_j' y_j
-  HTRY(vec_step_begin(s_, n3_, yn_, pr.L[0], pr.p1, pr.L[1], pr.p2, pr.L[2], -gamma_ * dl[1],
-                      q >= 2 ? Y[2] : nullptr, q >= 2 ? -gamma_ * dl[2] : 0.0, opt_.rtol,
-                      opt_.atol, ewt_, yp_, ycur_, avec_));
-  const bool dev = opt_.device_gmres && ggrid_ > 0;
-  double errsq = NAN;  // sum ((ycur - yp) ewt)^2 after the last Newton update
-  // The convergence-rate estimate carries over between steps (as CVODE's
-  // crate does) unless gamma changed a lot, so a small first correction can
-  // be accepted without a second rhs evaluation.
-  if (gamma_crate_ <= 0.0 || std::fabs(gamma_ / gamma_crate_ - 1.0) > 0.3) crate_ = 1.0;
-  gamma_crate_ = gamma_;
-  double del_old = 0.0, crate = crate_;
-  bool converged = false;
-  for (int k = 0; k < kMaxCor; k++) {
+// Host-driven GMRES: one cycle at a time, the controller's state read back
+// after each.
+int Integrator::run_host_cycles() {
+  for (;;) {
+    HTRY(launch_ts_begin(s_, n3_, ds_, slots_dev_, fn_, ycur_, yp_, avec_, ewt_));
     TRY(device_rhs(h_, ycur_, f_));
-    st_.rhs++;
-    st_.newton++;
-    double d = 0.0;
-    if (dev) {
-      // residual, preconditioner, solve, update and both norms in one launch
-      TRY(newton_device(&d, &errsq));
-    } else {
-      // res = -G = a + gamma f - y
-      HTRY(vec_lincomb(s_, n3_, 1.0, avec_, gamma_, f_, -1.0, ycur_, 0, nullptr, res_));
-      if (opt_.precond)
-        HTRY(bjac_setup(s_, nn_, handle_coef_table(h_), handle_inst(h_), gamma_, ycur_,
-                        handle_Htotal(h_), handle_kappa(h_), Minv_));
-      int its = 0;
-      TRY(gmres(res_, del_, &its));
-      HTRY(vec_lincomb(s_, n3_, 1.0, ycur_, 1.0, del_, 0, nullptr, 0, nullptr, ycur_));
-      d = wrms(del_);
-    }
-    if (!std::isfinite(d)) break;
-    if (k > 0) crate = std::max(0.3 * crate, d / del_old);
-    if (d * std::min(1.0, crate) * C <= kNewtonCoef) {
-      converged = true;
-      break;
-    }
-    if (k > 0 && d > 2.0 * del_old) break;
-    del_old = d;
-  }
-  crate_ = crate;
-  if (!converged) {
-    st_.conv_fails++;
-    crate_ = 1.0;
-    hnext_ = 0.25 * h;
-    return 0;
-  }
-  double err = 0.0;
-  if (dev) {
-    err = C * std::sqrt(errsq / n3_);
-  } else {
+    gamma_ = hs_.gamma;
+    // res = -G = a + gamma f - y
+    HTRY(vec_lincomb(s_, n3_, 1.0, avec_, gamma_, f_, -1.0, ycur_, 0, nullptr, res_));
+    if (opt_.precond)
+      HTRY(bjac_setup(s_, nn_, handle_coef_table(h_), handle_inst(h_), gamma_, ycur_,
+                      handle_Htotal(h_), handle_kappa(h_), Minv_));
+    int its = 0;
+    TRY(gmres(res_, del_, &its));
+    HTRY(vec_lincomb(s_, n3_, 1.0, ycur_, 1.0, del_, 0, nullptr, 0, nullptr, ycur_));
+    const double d = wrms(del_);
     HTRY(vec_lincomb(s_, n3_, 1.0, ycur_, -1.0, yp_, 0, nullptr, 0, nullptr, tmp_));
-    err = C * wrms(tmp_);
+    const double e = wrms(tmp_);
+    hhost_[0] = its;
+    hhost_[1] = d * d * n3_;
+    hhost_[2] = 0.0;
+    hhost_[3] = e * e * n3_;
+    HTRY(hipMemcpyAsync(status_, hhost_, 4 * sizeof(double), hipMemcpyHostToDevice, s_));
+    HTRY(launch_ts_ctrl(s_, ds_, status_, n3_));
+    TRY(download_state());
+    if (hs_.mode == kTsDone || hs_.mode == kTsError) return 0;
   }
-  *err_out = err;
-  if (!std::isfinite(err)) {
-    st_.conv_fails++;
-    hnext_ = 0.25 * h;
-    return 0;
-  }
-  if (err > 1.0) {
-    st_.err_fails++;
-    hnext_ = h * std::max(0.2, 0.9 * std::pow(err, -1.0 / (q + 1)));
-    return 0;
-  }
-  // accept: rotate the history
-  std::swap(ynm2_, ynm1_);
-  std::swap(ynm1_, yn_);
-  std::swap(yn_, ycur_);
-  tnm2_ = tnm1_;
-  tnm1_ = tn_;
-  tn_ = tn_ + h;
-  nhist_ = std::min(nhist_ + 1, 2);
-  st_.steps++;
-  st_.h_last = h;
-  st_.q_last = q;
-  st_.t = tn_;
-  const double etamax = st_.steps <= 10 ? 100.0 : 10.0;
-  double eta = err > 0.0 ? 0.9 * std::pow(err, -1.0 / (q + 1)) : etamax;
-  eta = std::min(etamax, std::max(0.2, eta));
-  if (eta > 1.0 && eta < 1.5) eta = 1.0;
-  hnext_ = h * eta;
-  qnext_ = std::min(opt_.max_order, std::max(1, nhist_));
-  *accepted = true;
-  return 0;
 }
 
 int Integrator::advance(double t_final, int max_steps, bool exact_tstop, double* t_reached) {
@@ -430,44 +373,69 @@ int Integrator::advance(double t_final, int max_steps, bool exact_tstop, double*
   }
   double* m = handle_m(h_);
   HTRY(hipMemcpyAsync(ystart_, m, sizeof(double) * n3_, hipMemcpyDeviceToDevice, s_));
-  const double eps_t = 1e-12 * std::max(1.0, std::fabs(t_final));
-  int steps = 0;
-  while (true) {
-    if (max_steps > 0 && steps >= max_steps) break;
-    if (exact_tstop ? (t_final - tn_ <= eps_t) : (tn_ >= t_final)) break;
-    if (hnext_ <= 0.0) {  // first step: h0 from ||f(y0)|| (each attempt sets ewt again)
-      HTRY(vec_ewt(s_, n3_, yn_, opt_.rtol, opt_.atol, ewt_));
-      TRY(device_rhs(h_, yn_, fn_));
-      st_.rhs++;
+  hs_.t_final = t_final;
+  hs_.exact_tstop = exact_tstop ? 1 : 0;
+  hs_.max_steps = max_steps;
+  hs_.steps_call = 0;
+  hs_.halt = 0;
+  hs_.err_code = 0;
+  hs_.rtol = opt_.rtol;
+  hs_.atol = opt_.atol;
+  hs_.max_order = opt_.max_order;
+  if (!ts_finished(hs_)) {
+    double* yn = slot_[hs_.sn];
+    if (hs_.hnext <= 0.0) {  // first step: h0 from ||f(y0)||
+      HTRY(vec_ewt(s_, n3_, yn, opt_.rtol, opt_.atol, ewt_));
+      TRY(device_rhs(h_, yn, fn_));
+      hs_.rhs++;
       const double fnorm = wrms(fn_);
       double h0 = fnorm > 0 ? 0.1 / fnorm : 1.0;
-      h0 = std::min(h0, 0.1 * std::fabs(t_final - tn_));
-      hnext_ = h0 > 0 ? h0 : 1e-6;
+      h0 = std::min(h0, 0.1 * std::fabs(t_final - hs_.tn));
+      hs_.hnext = h0 > 0 ? h0 : 1e-6;
+      hs_.qnext = 1;
     }
-    double h = hnext_;
-    int q = qnext_;
-    int fails = 0;
-    bool accepted = false;
-    double err = 0.0;
-    while (!accepted) {
-      if (exact_tstop && tn_ + h > t_final - 0.01 * h) h = t_final - tn_;
-      if (fails >= 2) q = 1;
-      TRY(step_attempt(h, q, &accepted, &err));
-      if (accepted) break;
-      if (++fails >= kMaxAttempts || hnext_ < 1e-14 * std::max(1.0, std::fabs(tn_))) {
-        set_error("integrator: step size underflow or repeated failures at t=" +
-                  std::to_string(tn_));
-        return 1;
+    ts_prepare(hs_, hs_.hnext, hs_.qnext);
+    TRY(upload_state());
+    const bool dev = opt_.device_gmres && ggrid_ > 0;
+    const int rc = dev ? run_device_cycles() : run_host_cycles();
+    // cycles queued after the end have run as empty launches; lift the halt
+    // word so that later field evaluations run
+    (void)hipStreamSynchronize(s_);
+    (void)hipMemsetAsync(&ds_->halt, 0, sizeof(int), s_);
+    (void)hipStreamSynchronize(s_);
+    TRY(rc);
+    if (hs_.copy >= 0) {  // the accepted iterate of the last step becomes y_n
+      HTRY(hipMemcpyAsync(slot_[hs_.copy], ycur_, sizeof(double) * n3_,
+                          hipMemcpyDeviceToDevice, s_));
+      hs_.copy = kSlotNone;
+    }
+    hs_.halt = 0;
+    st_.steps = hs_.steps;
+    st_.rhs = hs_.rhs;
+    st_.newton = hs_.newton;
+    st_.linear = hs_.linear;
+    st_.err_fails = hs_.err_fails;
+    st_.conv_fails = hs_.conv_fails;
+    st_.h_last = hs_.h_last;
+    st_.q_last = hs_.q_last;
+    st_.t = hs_.tn;
+    if (hs_.mode == kTsError) {
+      if (hs_.err_code == 1) {
+        (void)hipMemset(gw_.err, 0, sizeof(unsigned));
+        set_error("integrator: GMRES exchange timed out");
+        return -1;
       }
-      h = hnext_;
+      set_error("integrator: step size underflow or repeated failures at t=" +
+                std::to_string(hs_.tn));
+      return 1;
     }
-    steps++;
   }
-  if (!exact_tstop && tn_ > t_final && nhist_ >= 1) {
+  const double* yn = slot_[hs_.sn];
+  if (!exact_tstop && hs_.tn > t_final && hs_.nhist >= 1) {
     // interpolate the step polynomial back to t_final
-    const int q = nhist_;
-    const double P[3] = {tn_, tnm1_, tnm2_};
-    const double* Y[3] = {yn_, ynm1_, ynm2_};
+    const int q = hs_.nhist;
+    const double P[3] = {hs_.tn, hs_.tnm1, hs_.tnm2};
+    const double* Y[3] = {yn, slot_[hs_.snm1], slot_[hs_.snm2]};
     double L[3] = {0, 0, 0};
     for (int j = 0; j <= q; j++) {
       double w = 1.0;
@@ -479,8 +447,8 @@ int Integrator::advance(double t_final, int max_steps, bool exact_tstop, double*
                      q >= 2 ? Y[2] : nullptr, 0, nullptr, m));
     *t_reached = t_final;
   } else {
-    HTRY(hipMemcpyAsync(m, yn_, sizeof(double) * n3_, hipMemcpyDeviceToDevice, s_));
-    *t_reached = tn_;
+    HTRY(hipMemcpyAsync(m, yn, sizeof(double) * n3_, hipMemcpyDeviceToDevice, s_));
+    *t_reached = hs_.tn;
   }
   int np = 0;
   HTRY(vec_maxdist_partials(s_, nn_, m, ystart_, part_, &np));

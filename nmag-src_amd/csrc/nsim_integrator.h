@@ -1,8 +1,13 @@
 // Device-resident time integrator: variable-step BDF (order <= 2) with a
 // Newton iteration whose linear systems (I - gamma J) delta = -G are solved by
-// scaled GMRES with the analytic jacobi_ts1 J.v and a 3x3 block-Jacobi right
+// scaled GMRES with the analytic jacobi_ts1 J.v and a block-Jacobi right
 // preconditioner.  It stands in for CVODE (BDF + Newton + SPGMR) behind the
 // reference's timestepper (src/nsim.ml:2485-2580, src/sundials_sp_stubs.c:700-1308).
+//
+// The step controller (nsim_tsctl.h) runs on the device after every Newton
+// iteration, so an advance call queues whole Newton cycles -- start of
+// attempt, field evaluation, persistent GMRES, controller -- in batches and
+// the host only watches for the end.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -11,6 +16,7 @@
 #include <vector>
 
 #include "nsim_gmres.h"
+#include "nsim_tsctl.h"
 
 struct nsim_handle;
 
@@ -30,6 +36,8 @@ const int* handle_inst(nsim_handle* h);           // device node -> magnet, or n
 const DeviceOps& handle_ops(nsim_handle* h);
 bool handle_mixed(nsim_handle* h);
 int handle_device(nsim_handle* h);
+// Points the field kernels of the handle at a halt word (null: none).
+void handle_set_halt(nsim_handle* h, const int* halt);
 int handle_error_flags(nsim_handle* h, unsigned* out);
 void set_error(const std::string& msg);
 
@@ -38,11 +46,14 @@ struct TsOptions {
   int max_order = 2;                // timestepper_max_order (main.py:838)
   int krylov_max = 300;             // timestepper_krylov_max (main.py:839)
   int precond = 1;                  // 0 none, 1 block Jacobi of I - gamma J
-  int device_gmres = 1;             // 1: one persistent GMRES launch per solve,
+  int device_gmres = 1;             // 1: one persistent GMRES launch per Newton
+                                    // iteration, queued without host round trips;
                                     // 0: host-driven Arnoldi steps
   int gmres_spw = 1;                // least SELL slices per GMRES workgroup
   int gmres_pc_sweeps = 1;          // local Jacobi sweeps of the persistent GMRES's
                                     // preconditioner (0: 3x3 block Jacobi only)
+  int max_batch = 16;               // Newton cycles queued per batch
+  bool cooperative = false;         // persistent GMRES as a cooperative launch
 };
 
 struct TsStats {
@@ -62,27 +73,20 @@ class Integrator {
   // The handle's m receives y at *t_reached.
   int advance(double t_final, int max_steps, bool exact_tstop, double* t_reached);
   const TsStats& stats() const { return st_; }
+  TsOptions& options() { return opt_; }
   // workgroups of the persistent GMRES launch (0: host-driven GMRES only)
   int gmres_grid() const { return ggrid_; }
-  TsOptions& options() { return opt_; }
 
  private:
-  int step_attempt(double h, int q, bool* accepted, double* err_out);
-  // x = (I - gamma J)^-1 r, host-driven Arnoldi steps
+  int run_device_cycles();
+  int run_host_cycles();
+  int enqueue_cycle();
+  int upload_state();
+  int download_state();
   int gmres(const double* r, double* x, int* iters);
-  int newton_device(double* d, double* errsq);
   int apply_A(const double* z, double* w);
   double wrms(const double* x);
   int reduce(int nparts, int k, std::vector<double>* out);
-  // Predictor yp = L0 y_n + L1 p1 + L2 p2 and its error constant E_p
-  // (y_pred - y_exact = -E_p y^(q+1)).
-  struct Predictor {
-    double L[3];
-    const double* p1;
-    const double* p2;
-    double E_p;
-  };
-  void predictor(double h, int q, Predictor* p) const;
   int check(hipError_t e, const char* what);
 
   nsim_handle* h_;
@@ -91,15 +95,16 @@ class Integrator {
   hipStream_t s_;
   int n3_ = 0, nn_ = 0;
   bool ok_ = false;
-  // state: y_n at t_n and up to two previous points
-  double *yn_ = nullptr, *ynm1_ = nullptr, *ynm2_ = nullptr;
-  double tn_ = 0.0, tnm1_ = 0.0, tnm2_ = 0.0;
-  int nhist_ = 0;   // valid previous points
-  double hnext_ = 0.0;
-  int qnext_ = 1;
+  TsState hs_{};              // controller state, host copy (current between calls)
+  TsState* ds_ = nullptr;     // its device copy, which the controller kernel updates
+  TsState* pin_ = nullptr;    // pinned: [0], [1] batch mirrors, [2] upload staging
+  hipEvent_t ev_[2] = {nullptr, nullptr};
+  double* slot_[3] = {nullptr, nullptr, nullptr};  // ring of y_n, y_{n-1}, y_{n-2}
+  double** slots_dev_ = nullptr;                   // the same pointers on the device
   double *fn_ = nullptr;    // f(y_n) (first step's predictor)
   double *yp_ = nullptr, *ycur_ = nullptr, *avec_ = nullptr, *f_ = nullptr, *res_ = nullptr,
          *del_ = nullptr, *tmp_ = nullptr, *tmp2_ = nullptr, *ewt_ = nullptr, *ystart_ = nullptr;
+  double* status_ = nullptr;  // Newton-iteration results for the controller (host GMRES)
   double* V_ = nullptr;     // (maxl+1) Krylov vectors
   double* Minv_ = nullptr;  // 9 per node
   double* coef_ = nullptr;  // device coefficients for multi-axpy
@@ -110,10 +115,7 @@ class Integrator {
   GmresWork gw_;            // persistent GMRES buffers (V shared with V_)
   int ggrid_ = 0, gspw_ = 0;  // its grid; 0: not available
   std::vector<double> host_part_;
-  double gamma_ = 0.0;      // current Newton matrix coefficient
-  double crate_ = 1.0;      // Newton convergence-rate estimate
-  double gamma_crate_ = 0.0;  // gamma at which crate_ was measured
-  double gamma_pc_ = -1.0;  // gamma the preconditioner was set up for
+  double gamma_ = 0.0;      // Newton matrix coefficient of the host-driven GMRES
   std::vector<void*> allocs_;
 };
 

@@ -72,7 +72,9 @@ __global__ __launch_bounds__(256) void k_exch_div(int n, const int* __restrict__
                                                   const int* __restrict__ inst,
                                                   const double* __restrict__ m,
                                                   double* __restrict__ Hexch,
-                                                  double* __restrict__ rho, int want_rho) {
+                                                  double* __restrict__ rho, int want_rho,
+                                                  const int* __restrict__ halt) {
+  if (halt && *halt) return;  // queued by a finished integrator run
   __shared__ double sh[4 * kSplit][64];
   const int s = blockIdx.x, t = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int row = s * 64 + lane;
@@ -116,7 +118,8 @@ __global__ __launch_bounds__(256) void k_demag_llg(int n, const int* __restrict_
                                                    const Coeffs* __restrict__ ct,
                                                    const int* __restrict__ inst, FieldArgs f,
                                                    FieldBufs bf, double* __restrict__ dmdt,
-                                                   int do_demag) {
+                                                   int do_demag, const int* __restrict__ halt) {
+  if (halt && *halt) return;  // queued by a finished integrator run
   __shared__ double sh[3 * kSplit][64];
   const int s = blockIdx.x, t = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int row = s * 64 + lane;
@@ -219,11 +222,11 @@ hipError_t launch_exch_div(const DeviceOps& d, bool mixed, const double* m, doub
   if (mixed)
     k_exch_div<float><<<d.nslices, 256, 0, st>>>(d.n, d.sell_off, d.sell_col, d.Kf,
                                                          d.D, d.Vinv, d.coefs, d.inst, m,
-                                                         Hexch, rho, want_rho ? 1 : 0);
+                                                         Hexch, rho, want_rho ? 1 : 0, d.halt);
   else
     k_exch_div<double><<<d.nslices, 256, 0, st>>>(d.n, d.sell_off, d.sell_col, d.K,
                                                           d.D, d.Vinv, d.coefs, d.inst, m,
-                                                          Hexch, rho, want_rho ? 1 : 0);
+                                                          Hexch, rho, want_rho ? 1 : 0, d.halt);
   return hipGetLastError();
 }
 
@@ -232,11 +235,11 @@ hipError_t launch_demag_llg(const DeviceOps& d, bool mixed, const FieldArgs& f,
   if (mixed)
     k_demag_llg<float><<<d.nslices, 256, 0, st>>>(d.n, d.sell_off, d.sell_col, d.Gf,
                                                           d.coefs, d.inst, f, b, dmdt,
-                                                          do_demag ? 1 : 0);
+                                                          do_demag ? 1 : 0, d.halt);
   else
     k_demag_llg<double><<<d.nslices, 256, 0, st>>>(d.n, d.sell_off, d.sell_col, d.G,
                                                            d.coefs, d.inst, f, b, dmdt,
-                                                           do_demag ? 1 : 0);
+                                                           do_demag ? 1 : 0, d.halt);
   return hipGetLastError();
 }
 

@@ -1,0 +1,80 @@
+// Device side of the BDF step controller (nsim_tsctl.h): the start-of-attempt
+// vector pass and the one-thread controller kernel that runs after every
+// Newton iteration.
+#include <hip/hip_runtime.h>
+
+#include "nsim_tsctl.h"
+#include "nsim_vec.h"
+
+namespace nsim {
+
+namespace {
+
+__device__ inline const double* src_of(double* const* slots, const double* fn, int s) {
+  return s == kSlotF ? fn : (s >= 0 ? slots[s] : nullptr);
+}
+
+__global__ __launch_bounds__(256) void k_ts_begin(int n3, const TsState* __restrict__ s,
+                                                  double* const* __restrict__ slots,
+                                                  const double* __restrict__ fn,
+                                                  double* __restrict__ ycur,
+                                                  double* __restrict__ yp,
+                                                  double* __restrict__ avec,
+                                                  double* __restrict__ ewt) {
+  if (s->mode != kTsBegin || s->halt) return;
+  const int copy = s->copy;
+  const double* yn = slots[s->sn];
+  double* dst = copy >= 0 ? slots[copy] : nullptr;
+  const double* p1 = src_of(slots, fn, s->p1);
+  const double* p2 = src_of(slots, fn, s->p2);
+  const double* a2v = src_of(slots, fn, s->a2s);
+  const double L0 = s->L[0], L1 = s->L[1], L2 = s->L[2], a1 = s->a1, a2 = s->a2;
+  const double rtol = s->rtol, atol = s->atol;
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n3; i += gridDim.x * 256) {
+    // the accepted iterate becomes y_n (same values as the ring slot gets)
+    const double y = dst ? ycur[i] : yn[i];
+    if (dst) dst[i] = y;
+    ewt[i] = 1.0 / (rtol * fabs(y) + atol);
+    double v = L0 * y;
+    if (p1) v += L1 * p1[i];
+    if (p2) v += L2 * p2[i];
+    yp[i] = v;
+    ycur[i] = v;
+    double a = a1 * y;
+    if (a2v) a += a2 * a2v[i];
+    avec[i] = a;
+  }
+}
+
+// One thread works on a register copy of the state (read and written back
+// whole, so the loads are in flight together rather than one field at a time).
+__global__ void k_ts_ctrl(TsState* s, const double* __restrict__ status, int n3) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  TsState st = *s;
+  if (st.mode == kTsDone || st.mode == kTsError || st.halt) return;
+  const double its = status[0], xw2 = status[1], bad = status[2], errsq = status[3];
+  if (bad != 0.0) {
+    st.mode = kTsError;
+    st.err_code = 1;
+    st.halt = 1;
+  } else {
+    ts_after_newton(st, sqrt(xw2 / n3), errsq, (int)its, n3);
+  }
+  *s = st;
+}
+
+}  // namespace
+
+hipError_t launch_ts_begin(hipStream_t st, int n3, const TsState* s, double* const* slots,
+                           const double* fn, double* ycur, double* yp, double* avec,
+                           double* ewt) {
+  k_ts_begin<<<vec_reduce_grid(n3), 256, 0, st>>>(n3, s, slots, fn, ycur, yp, avec, ewt);
+  return hipGetLastError();
+}
+
+hipError_t launch_ts_ctrl(hipStream_t st, TsState* s, const double* status, int n3) {
+  k_ts_ctrl<<<1, 64, 0, st>>>(s, status, n3);
+  return hipGetLastError();
+}
+
+}  // namespace nsim

@@ -87,7 +87,7 @@ def test_bdf_order_from_step_counts():
     """Local-error control at order q takes h ~ tol^(1/(q+1)), so cutting the
     tolerance 100-fold multiplies the step count by about 100^(1/2) = 10 at
     q = 1 and 100^(1/3) = 4.6 at q = 2; the global error of the macrospin
-    closed form also shrinks with the tolerance at each order."""
+    closed form shrinks with the tolerance at the matching rate."""
     alpha, H = 0.1, 0.1
     mat = material(alpha=alpha)
     _, _, g = _magnet(1, 1, 1, 2.5, mat, do_demag=False)
@@ -112,8 +112,10 @@ def test_bdf_order_from_step_counts():
         ratio[q] = steps[1e-8] / steps[1e-6]
     assert 7.0 < ratio[1] < 14.0, ratio
     assert 3.3 < ratio[2] < 6.5, ratio
-    for q in (1, 2):
-        assert err[(q, 1e-8)] < err[(q, 1e-6)] / 10
+    # the global error scales like tol^(q/(q+1)): 100^(1/2) = 10 at q = 1 and
+    # 100^(2/3) = 21.5 at q = 2
+    assert err[(1, 1e-8)] < err[(1, 1e-6)] / 5
+    assert err[(2, 1e-8)] < err[(2, 1e-6)] / 10
 
 
 def test_stop_semantics_and_statistics():
