@@ -308,7 +308,16 @@ int Integrator::enqueue_cycle() {
 int Integrator::run_device_cycles() {
   const int maxb = std::max(1, std::min(opt_.max_batch, 64));
   int batches = 0, waited = 0;
-  int batch = std::min(maxb, 4);
+  // cycles still needed: about one per step left at the current step size
+  auto cycles_left = [&](const TsState& m) {
+    double left = 1.0;
+    if (m.max_steps > 0) left = m.max_steps - m.steps_call;
+    if (m.h > 0.0) {
+      const double steps = std::ceil((m.t_final - m.tn) / m.h);
+      left = m.max_steps > 0 ? std::min(left, steps) : steps;
+    }
+    return (int)std::min<double>(maxb, std::max(1.0, left));
+  };
   auto enqueue_batch = [&](int cycles) -> int {
     for (int c = 0; c < cycles; c++) TRY(enqueue_cycle());
     TsState* mirror = &pin_[batches & 1];
@@ -317,24 +326,20 @@ int Integrator::run_device_cycles() {
     batches++;
     return 0;
   };
-  TRY(enqueue_batch(batch));
-  TRY(enqueue_batch(batch));
+  // the first batch covers the estimate, the second a couple of retries
+  TRY(enqueue_batch(cycles_left(hs_)));
+  TRY(enqueue_batch(2));
   while (true) {
     HTRY(hipEventSynchronize(ev_[waited & 1]));
     const TsState m = pin_[waited & 1];
     waited++;
-    if (m.mode == kTsDone || m.mode == kTsError || m.halt) break;
-    // cycles still needed: about one per step left (a few more for retries)
-    double left = 1.0;
-    if (m.max_steps > 0) left = m.max_steps - m.steps_call;
-    if (m.h > 0.0) {
-      const double steps = std::ceil((m.t_final - m.tn) / m.h);
-      left = m.max_steps > 0 ? std::min(left, steps) : steps;
+    if (m.mode == kTsDone || m.mode == kTsError || m.halt) {
+      // the state does not change after the end: this copy is final
+      hs_ = m;
+      return 0;
     }
-    batch = (int)std::min<double>(maxb, std::max(1.0, left + 1.0));
-    TRY(enqueue_batch(batch));
+    TRY(enqueue_batch(cycles_left(m)));
   }
-  return download_state();
 }
 
 // Host-driven GMRES: one cycle at a time, the controller's state read back
@@ -398,12 +403,14 @@ int Integrator::advance(double t_final, int max_steps, bool exact_tstop, double*
     TRY(upload_state());
     const bool dev = opt_.device_gmres && ggrid_ > 0;
     const int rc = dev ? run_device_cycles() : run_host_cycles();
-    // cycles queued after the end have run as empty launches; lift the halt
-    // word so that later field evaluations run
-    (void)hipStreamSynchronize(s_);
-    (void)hipMemsetAsync(&ds_->halt, 0, sizeof(int), s_);
-    (void)hipStreamSynchronize(s_);
-    TRY(rc);
+    // cycles queued after the end run as empty launches; behind them on the
+    // stream, lift the halt word so that later field evaluations run
+    const hipError_t me = hipMemsetAsync(&ds_->halt, 0, sizeof(int), s_);
+    if (rc) {
+      (void)hipStreamSynchronize(s_);
+      return rc;
+    }
+    HTRY(me);
     if (hs_.copy >= 0) {  // the accepted iterate of the last step becomes y_n
       HTRY(hipMemcpyAsync(slot_[hs_.copy], ycur_, sizeof(double) * n3_,
                           hipMemcpyDeviceToDevice, s_));
