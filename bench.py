@@ -78,6 +78,17 @@ def gemv_bytes(nb, esz):
 
 
 AUTOTUNE = True
+# --no-graph: eager launches only (rocprofv3's kernel trace cannot follow
+# HIP-graph capture in this image)
+GRAPHS = True
+
+
+def graphs(g):
+    if not GRAPHS:
+        g.tune("rhs_graph", 0)
+    return g
+
+
 # HBM bytes per launch of the potential GEMV at the 20k-node slab, measured
 # with rocprofv3 (FETCH_SIZE x 2.000, the factor a 1 GiB device copy gives,
 # plus WRITE_SIZE; profiles/potential_gemv_c2_slab_hbm_counters.md).  Counters
@@ -154,8 +165,8 @@ def run_dynamics(prec, route, coords, tets, mat, Hext, m0, window_ps, checkpoint
     """advance_time over window_ps from the S-state (BDF, rtol = atol = 1e-6);
     returns wall time, integrator statistics and <m>(t) at the checkpoints."""
     from nmag._native import NativeMagnet
-    g = NativeMagnet(coords, tets, mat.su_coeffs(), precision=prec, nbc_rtol=ksp_rtol,
-                     dbc_rtol=ksp_rtol, potential=route)
+    g = graphs(NativeMagnet(coords, tets, mat.su_coeffs(), precision=prec, nbc_rtol=ksp_rtol,
+                            dbc_rtol=ksp_rtol, potential=route))
     g.set_field("H_ext", Hext)
     g.set_field("m", m0)
     g.ts_init(0.0, 1e-6, 1e-6)
@@ -261,7 +272,7 @@ def run_batch(steps, warmup, rtol, window_ps, torch):
     # batched field evaluations
     for prec in ("fp64", "mixed"):
         t0 = time.perf_counter()
-        b = NativeBatch(meshes, mats, precision=prec, nbc_rtol=rtol, dbc_rtol=rtol)
+        b = graphs(NativeBatch(meshes, mats, precision=prec, nbc_rtol=rtol, dbc_rtol=rtol))
         t_setup = time.perf_counter() - t0
         setup(b, cat)
         b.set_field("m", states[0])
@@ -331,7 +342,7 @@ def run_batch(steps, warmup, rtol, window_ps, torch):
     # integrator per magnet, FP64, rtol = atol = 1e-6
     if window_ps > 0:
         t0 = time.perf_counter()
-        b = NativeBatch(meshes, mats, precision="fp64", nbc_rtol=rtol, dbc_rtol=rtol)
+        b = graphs(NativeBatch(meshes, mats, precision="fp64", nbc_rtol=rtol, dbc_rtol=rtol))
         setup(b, cat)
         b.set_field("m", np.concatenate([g["fields"]["m"] for g in mags]))
         b.ts_init(0.0, 1e-6, 1e-6)
@@ -471,12 +482,15 @@ def main():
     ap.add_argument("--adjoint-ns", type=float, default=2.0,
                     help="ns of the differentiation-form trajectory (0 skips)")
     ap.add_argument("--adjoint-rtol", type=float, default=1e-8)
+    ap.add_argument("--no-graph", action="store_true",
+                    help="no HIP-graph replay of field evaluations (for kernel traces)")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="also run a plain device copy of this many MiB (HBM counter "
                          "calibration under rocprofv3 --pmc)")
     args = ap.parse_args()
-    global AUTOTUNE
+    global AUTOTUNE, GRAPHS
     AUTOTUNE = not args.no_autotune
+    GRAPHS = not args.no_graph
     if not args.routes.startswith("dense"):
         print("--routes must start with dense (the headline route)", file=sys.stderr)
         sys.exit(2)
