@@ -371,6 +371,74 @@ def run_batch(steps, warmup, rtol, window_ps, torch):
     return out
 
 
+def run_precision(slab, window_ps=1000.0, checkpoint_ps=10.0, seed=1, rtol=1e-10):
+    """Precision form on the 12x12x4-cube cuboid (2.5 nm, 845 nodes, alpha 0.5,
+    seeded Gaussian m, no applied field): per-field max relative error of
+    H_exch, H_demag and dm/dt against FP64 at the seeded state, and the
+    <m>(t) deviation from FP64 over window_ps of relaxation; the same
+    per-field errors at one state of the 20k-node slab (slab = coords, tets,
+    mat, Hext, m)."""
+    from nmag import kuhn_cuboid
+    from nmag._native import NativeMagnet
+    coords, tets = kuhn_cuboid(12, 12, 4, 2.5)
+    mat = make_material(alpha=0.5)
+    rng = np.random.default_rng(seed)
+    m0 = rng.standard_normal((len(coords), 3))
+    m0 /= np.linalg.norm(m0, axis=1, keepdims=True)
+    fields = ("H_exch", "H_demag", "dmdt")
+
+    def rel(a, b):
+        return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+
+    out = {"workload": "Py 30x30x10 nm cuboid, 2.5 nm Kuhn mesh (845 nodes), alpha 0.5, "
+                       "seeded Gaussian m (seed %d), relaxed %.0f ps" % (seed, window_ps)}
+    ref, avg = {}, {}
+    # "fp64_tight": FP64 integrated at rtol = atol = 1e-8, the size of the
+    # integration error itself at the working tolerance 1e-6
+    for prec in ("fp64", "mixed", "bf16", "fp64_tight"):
+        g = graphs(NativeMagnet(coords, tets, mat.su_coeffs(),
+                                precision="fp64" if prec == "fp64_tight" else prec,
+                                nbc_rtol=rtol, dbc_rtol=rtol))
+        g.set_field("m", m0)
+        g.execute()
+        f = {k: g.get_field(k) for k in fields}
+        if prec == "fp64":
+            ref = f
+        elif prec != "fp64_tight":
+            out[prec] = {k + "_max_rel_err": rel(f[k], ref[k]) for k in fields}
+        tol = 1e-8 if prec == "fp64_tight" else 1e-6
+        g.ts_init(0.0, tol, tol)
+        V = g.get_field("volumes")
+        t, series = 0.0, []
+        while t < window_ps - 1e-9:
+            t = g.advance(min(window_ps, t + checkpoint_ps))
+            m = g.get_field("m")
+            series.append((m * V[:, None]).sum(0) / V.sum())
+        avg[prec] = np.array(series)
+        g.close()
+    for prec in ("mixed", "bf16"):
+        out[prec]["mean_m_max_dev_vs_fp64"] = float(np.abs(avg[prec] - avg["fp64"]).max())
+    out["fp64_rtol_1e-8_mean_m_max_dev_vs_fp64"] = float(
+        np.abs(avg["fp64_tight"] - avg["fp64"]).max())
+    # the slab at one state
+    coords, tets, mat, Hext, m = slab
+    sref = {}
+    out["slab"] = {}
+    for prec in ("fp64", "mixed", "bf16"):
+        g = graphs(NativeMagnet(coords, tets, mat.su_coeffs(), precision=prec))
+        g.set_field("H_ext", Hext)
+        g.set_field("m", m)
+        g.execute()
+        f = {k: g.get_field(k) for k in fields}
+        g.close()
+        if prec == "fp64":
+            sref = f
+        else:
+            out["slab"][prec] = {k + "_max_rel_err": rel(f[k], sref[k]) for k in fields}
+    log("precision form: %s" % json.dumps(out))
+    return out
+
+
 def run_adjoint(t_ns, rtol, fd_checks=True):
     """Differentiation form: permalloy slab 100 x 50 x 5 nm, 2.5 nm Kuhn mesh
     (40 x 20 x 2 cubes, 2583 nodes), alpha 0.02, per-element K ~ N(2e4, 5e3)
@@ -482,6 +550,8 @@ def main():
     ap.add_argument("--adjoint-ns", type=float, default=2.0,
                     help="ns of the differentiation-form trajectory (0 skips)")
     ap.add_argument("--adjoint-rtol", type=float, default=1e-8)
+    ap.add_argument("--no-precision", action="store_true",
+                    help="skip the precision-form leg (per-field errors, <m> deviation)")
     ap.add_argument("--no-graph", action="store_true",
                     help="no HIP-graph replay of field evaluations (for kernel traces)")
     ap.add_argument("--copy-mb", type=int, default=0,
@@ -604,6 +674,10 @@ def main():
     if not args.no_batch:
         batch = run_batch(args.batch_steps, 3, args.rtol, args.batch_window, torch)
 
+    precision = None
+    if not args.no_precision:
+        precision = run_precision((coords, tets, mat, Hext, states[args.warmup]))
+
     adjoint = None
     if args.adjoint_ns > 0:
         adjoint = run_adjoint(args.adjoint_ns, args.adjoint_rtol)
@@ -665,6 +739,7 @@ def main():
         "roofline_iterative_route": rooflines_iterative,
         "cpu_baseline": cpu,
         "dynamics": dyn,
+        "precision": precision,
         "batch": batch,
         "differentiation": adjoint,
     }
