@@ -8,13 +8,20 @@ in and come out in SI; internally everything is in nmag's simulation units.
 All computation runs in libnsim_hip.so through the C ABI.
 """
 import math
+import os
+import re
+import sys
+import time
+import zipfile
 
 import numpy as np
 
 from . import _native
+from . import datafile
 from .material import MagMaterial, NmagUserError
 from .mesh import read_nmesh
 from .units import SI, su_factor, to_su
+from .when import at, every, next_deltas, next_occurrence
 
 # SI units of each field (the reference's fieldunits_by_fieldname)
 FIELD_UNITS = {
@@ -29,18 +36,18 @@ PER_MATERIAL = ("m", "M", "H_total", "H_exch", "H_anis", "dmdt", "E_exch", "E_de
                 "E_ext", "E_anis", "E_total")
 DEGREES_PER_NS = SI(math.pi / 180.0 / 1e-9, "1/s")
 
+# the clock columns that open every ndt row (interface/nmag/main.py:233-240,
+# :2337-2345)
+NDT_CLOCK = (("time", SI(1, "s")), ("id", SI(1)), ("step", SI(1)), ("last_step_dt", SI(1, "s")),
+             ("stage_time", SI(1, "s")), ("stage_step", SI(1)), ("stage", SI(1)))
+# matching tolerance of time events (interface/nmag/hysteresis.py:421-424)
+_TIME_TOLS = {"time": 1e-20, "stage_time": 1e-20}
 
-class every:
-    """Schedule 'every n steps' (interface/nsim/when.py); only ('step', n) is used
-    by relax's convergence check."""
 
-    def __init__(self, what, value):
-        self.what, self.value = what, value
-
-
-class at:
-    def __init__(self, what, value=None):
-        self.what, self.value = what, value
+def _runid():
+    """Default simulation name: the running script's name without extension."""
+    base = os.path.splitext(os.path.basename(sys.argv[0] if sys.argv else ""))[0]
+    return base if base and base not in ("-c", "__main__", "-") else "simulation"
 
 
 class Clock(dict):
@@ -84,7 +91,7 @@ class Simulation:
             if k not in ("NBC.rtol", "NBC.atol", "NBC.dtol", "NBC.maxits",
                          "DBC.rtol", "DBC.atol", "DBC.dtol", "DBC.maxits"):
                 raise NmagUserError("unknown ksp_tolerances key '%s'" % k)
-        self.name = name or "simulation"
+        self.name = name or _runid()
         self.do_demag = do_demag
         self.precision = precision
         self.ksp_tolerances = dict(ksp_tolerances)
@@ -97,13 +104,46 @@ class Simulation:
         self._default_exact_tstop = True
         self.max_dm_dt = None
         self.convergence = Convergence()
-        self.clock = Clock(time=0.0, step=0, stage=1, stage_step=0, stage_time=0.0,
+        # SimulationClock (interface/nmag/clock.py:21-37); times in SI seconds
+        self.clock = Clock(id=-1, time=0.0, step=0, stage=1, stage_step=0, stage_time=0.0,
+                           real_time=0.0, stage_end=False, convergence=False,
+                           exit_hysteresis=False, zero_stage_step=0, zero_stage_time=0.0,
                            time_reached_si=0.0, time_reached_su=0.0, last_step_dt_si=0.0,
-                           last_step_dt_su=0.0, convergence=False)
+                           last_step_dt_su=0.0)
         self._magnet = None
         self._mat = None
         self._ts_initialised = False
+        self._stages_run = 0
+        self._saved_steps = set()
         self.max_time_reached = SI(1e-6, "s")
+        # relax/hysteresis 'save' and 'do' names (simulation_core.py:185-197)
+        self.action_abbreviations = {}
+        self.add_save_abbrev("save_averages", lambda s: s.save_data(avoid_same_step=True))
+        self.add_save_abbrev("save_fields",
+                             lambda s: s.save_data(fields="all", avoid_same_step=True))
+        self.add_save_abbrev("save_field_m",
+                             lambda s: s.save_data(fields=["m"], avoid_same_step=True))
+        self.add_save_abbrev("save_restart", lambda s: s.save_restart_file())
+        self.add_do_abbrev("do_next_stage", Simulation.hysteresis_next_stage)
+        self.add_do_abbrev("do_exit", Simulation.hysteresis_exit)
+        # existing data files: an error unless the script runs with --clean
+        # (old files renamed *.old) or --restart (continued); the command-line
+        # flags of the reference (simulation_core.py:143-166, snippets.py:151-156)
+        self._restarting = False
+        argv = sys.argv[1:] if sys.argv else []
+        data_files = [self._ndtfilename(), self._datafilename()]
+        if "--clean" in argv:
+            for f in data_files:
+                if os.path.exists(f):
+                    os.replace(f, f + ".old")
+        elif "--restart" in argv:
+            self._restarting = True
+        else:
+            for f in data_files:
+                if os.path.exists(f):
+                    raise NmagUserError(
+                        "found old file %s -- cannot proceed: run with --clean to move old "
+                        "data files aside or with --restart to continue the run" % f)
 
     # ------------------------------------------------------------- set-up
     def load_mesh(self, filename, region_names_and_mag_mats, unit_length=SI(1e-9, "m"),
@@ -236,7 +276,9 @@ class Simulation:
             self.reinitialise()
         g = self.magnet
         steps_before = g.stat("ts_steps")
+        wall = time.perf_counter()
         t_reached = g.advance(target_su, max_it, exact_tstop)
+        self.clock.real_time += time.perf_counter() - wall
         delta_step = int(g.stat("ts_steps") - steps_before)
         delta_time = t_reached - self.clock.time_reached_su
         time_to_si = su_factor(SI(1, "s").dims)
@@ -260,34 +302,292 @@ class Simulation:
         self.clock.convergence = c
         return c
 
-    def relax(self, H_applied=None, save=None, do=None, convergence_check=every("step", 5)):
-        """Integrate until max|dm/dt| < stopping_dm_dt on two consecutive checks,
-        checking every convergence_check.value steps (hysteresis.py:207-549)."""
-        if H_applied is not None:
-            self.set_H_ext(H_applied)
-        steps = convergence_check.value if convergence_check.what == "step" else 5
-        self.max_dm_dt = None
-        self.convergence = Convergence()
-        self.clock.stage_step = 0
-        max_t = self.max_time_reached
-        while True:
-            if self.is_converged():
-                break
-            t = self.advance_time(max_t, max_it=steps)
-            if t.value > 0.99 * max_t.value:
-                raise NmagUserError("Simulation time reached %s: are you starting from a "
-                                    "zero torque configuration?" % max_t)
-        self.clock.stage += 1
+    def relax(self, H_applied=None, save=[("averages", "fields", at("stage_end"))], do=[],
+              convergence_check=every("step", 5)):
+        """Integrate until max|dm/dt| < stopping_dm_dt on two consecutive checks
+        (one stage of hysteresis; interface/nmag/hysteresis.py:207-242).
+        Returns the time reached in the stage (SI seconds)."""
+        self.hysteresis([H_applied], save=save, do=do, convergence_check=convergence_check)
         return self.clock.time_reached_si
 
-    def hysteresis(self, H_ext_list, save=None, do=None, convergence_check=every("step", 5)):
-        """Relax at every applied field of H_ext_list in turn; returns the
-        average m (SI list) at the end of each stage."""
+    def hysteresis(self, H_ext_list, save=[("averages", "fields", at("stage_end"))], do=[],
+                   convergence_check=every("step", 5)):
+        """Relax at every applied field of H_ext_list in turn (None keeps the
+        current field), running the `do` and `save` actions whenever their
+        schedules fall due (interface/nmag/hysteresis.py:245-549).  Each entry
+        of `save`/`do` is (thing, ..., when): a thing is a function of the
+        Simulation or one of the names 'averages', 'fields', 'field_m',
+        'restart' (save) and 'next_stage', 'exit' (do).  Returns the average
+        m (SI) at the end of each stage."""
+        actions = self._join_actions(do, "do") + self._join_actions(save, "save")
+        planned = {}
+        for fn, _ in actions:
+            if id(fn) in planned:
+                raise NmagUserError("'save'/'do' name the same thing twice: join the "
+                                    "schedules with | instead")
+            planned[id(fn)] = None
+        # the first stage keeps the clock's stage number; a restarted run
+        # continues from the stage it was saved in (hysteresis.py:432-451).
+        # Later calls on the same object start a new stage (the reference would
+        # skip fields here, by indexing the list with the old stage number).
+        skip = 0
+        if self._restarting:
+            self.load_restart_file()
+            self._restarting = False
+            skip = max(0, self.clock.stage - 1)
+            stage = self.clock.stage
+        else:
+            stage = self.clock.stage if self._stages_run == 0 else self.clock.stage + 1
         out = []
-        for H in H_ext_list:
-            self.relax(H, convergence_check=convergence_check)
-            out.append(self.get_subfield_average_siv("m", self._region_name))
+        c = self.clock
+        for H in H_ext_list[skip:]:
+            self.do_next_stage(stage=stage)
+            stage = None
+            self._stages_run += 1
+            c.stage_end = False
+            if H is not None and not (isinstance(H, (list, tuple)) and len(H) == 0):
+                self.set_H_ext(H)
+            self.reinitialise(initial_time=0)
+            for fn, when in actions:
+                planned[id(fn)] = next_occurrence(when, c, _TIME_TOLS)
+            while True:
+                c.stage_end = self.is_converged()
+                deltas = next_deltas(convergence_check, c, tols=_TIME_TOLS)
+                for fn, when in actions:
+                    due = next_occurrence(when, c, _TIME_TOLS)
+                    # run when the event is now, or when the planned occurrence
+                    # has been passed (the next one moved on)
+                    if when.match_time(c) or due != planned[id(fn)]:
+                        fn(self)
+                    planned[id(fn)] = due
+                    deltas = next_deltas(when, c, suggest=deltas, tols=_TIME_TOLS)
+                if c.exit_hysteresis:
+                    c.exit_hysteresis = False
+                    return out
+                if c.stage_end:
+                    break
+                d_step, d_time, _ = deltas
+                if d_time is None:
+                    target = self.max_time_reached
+                else:
+                    target = SI(c.stage_time + d_time, "s")
+                t = self.advance_time(target, max_it=-1 if d_step is None else int(d_step))
+                if t.value > 0.99 * self.max_time_reached.value:
+                    raise NmagUserError("Simulation time reached %s s: are you starting from "
+                                        "a zero torque configuration?"
+                                        % self.max_time_reached.value)
+            out.append(self.get_subfield_average_siv("m", self._mat_name()))
         return out
+
+    # ------------------------------------------------- stages and actions
+    def do_next_stage(self, stage=None):
+        """Start a stage: number `stage`, or the next one (clock.py inc_stage)."""
+        self.max_dm_dt = None
+        self.convergence = Convergence()
+        c = self.clock
+        c.stage = c.stage + 1 if stage is None else stage
+        c.stage_step = 0
+        c.stage_time = 0.0
+        c.convergence = False
+        c.zero_stage_step = c.step
+        c.zero_stage_time = c.time
+
+    @staticmethod
+    def hysteresis_next_stage(sim):
+        """A 'do' action: end the current stage (simulation_core.py:256-262)."""
+        sim.clock.stage_end = True
+
+    @staticmethod
+    def hysteresis_exit(sim):
+        """A 'do' action: leave relax/hysteresis (simulation_core.py:264-271)."""
+        sim.clock.exit_hysteresis = True
+        sim.clock.stage_end = True
+
+    def _add_abbrev(self, prefix, abbreviation, function):
+        name = abbreviation if abbreviation.startswith(prefix + "_") else prefix + "_" + abbreviation
+        self.action_abbreviations[name] = function
+
+    def add_save_abbrev(self, abbreviation, function):
+        """Name a function for the 'save' list: add_save_abbrev('funky', f)
+        allows save=[('funky', every('step', 10))]."""
+        self._add_abbrev("save", abbreviation, function)
+
+    def add_do_abbrev(self, abbreviation, function):
+        """Name a function for the 'do' list."""
+        self._add_abbrev("do", abbreviation, function)
+
+    def _join_actions(self, items, prefix):
+        """[(function, when)] from a save/do list of (thing, ..., when)
+        (hysteresis.py:55-126)."""
+        out = []
+        try:
+            for item in items:
+                item = list(item)
+                things, when = item[:-1], item[-1]
+                for thing in things:
+                    if callable(thing):
+                        out.append((thing, when))
+                        continue
+                    if isinstance(thing, str):
+                        norm = re.sub(r"[ \t_]+", "_", thing.lower())
+                        fn = self.action_abbreviations.get(prefix + "_" + norm,
+                                                           self.action_abbreviations.get(norm))
+                        if fn is not None:
+                            out.append((fn, when))
+                            continue
+                    raise NmagUserError(
+                        "'%s' list: cannot %s '%s'; give a function or one of: %s"
+                        % (prefix, prefix, thing, ", ".join(sorted(self.action_abbreviations))))
+        except TypeError:
+            raise NmagUserError("'%s' must be a list of tuples (thing, ..., when)" % prefix)
+        return out
+
+    # ------------------------------------------------------- data files
+    def _ndtfilename(self):
+        return self.name + "_dat.ndt"
+
+    def _datafilename(self):
+        """Field store: the counterpart of the reference's _dat.h5."""
+        return self.name + "_dat.npz"
+
+    def get_restart_file_name(self):
+        return self.name + "_restart.npz"
+
+    def _mat_name(self):
+        return self._mat.name if self._mat is not None else self._region_name
+
+    def _subfield_names(self, fields):
+        mat = self._mat_name()
+        return ["%s_%s" % (f, mat) if f in PER_MATERIAL else f for f in fields]
+
+    def _all_fields_su(self):
+        """Every field of FIELD_UNITS at the current m, from one evaluation."""
+        g = self.magnet
+        g.execute("update_dmdt")
+        out = {f: g.get_field(f) for f in ("m", "H_exch", "H_demag", "H_anis", "H_total",
+                                           "H_ext", "dmdt", "phi", "rho")}
+        out["M"] = self._mat.su_Ms * out["m"]
+        out.update(self._energies_su(out))
+        return out
+
+    def _energies_su(self, f):
+        m = f["m"]
+        Ms, mu0 = self._mat.su_Ms, self._mat.su_mu0
+        # energy densities (nmag_lam.py template_E)
+        e = {"E_demag": -0.5 * mu0 * Ms * (m * f["H_demag"]).sum(1),
+             "E_exch": -0.5 * mu0 * Ms * (m * f["H_exch"]).sum(1),
+             "E_ext": -1.0 * mu0 * Ms * (m * f["H_ext"]).sum(1)}
+        an = self._mat.su_anisotropy()
+        if an is not None:
+            k1, k2, axis = an
+            u = m @ axis
+            e["E_anis"] = -k1 * u ** 2 - k2 * u ** 4
+        else:
+            e["E_anis"] = np.zeros(len(m))
+        e["E_total"] = e["E_demag"] + e["E_exch"] + e["E_ext"] + e["E_anis"]
+        return e
+
+    def _edges(self):
+        if getattr(self, "_edge_list", None) is None:
+            self._edge_list = datafile.mesh_edges(self._tets)
+        return self._edge_list
+
+    def _compute_averages(self):
+        """(names, values, units) of one ndt row: the clock, the volume average
+        of every field in SI, the largest angle of m across a mesh edge, local
+        and unix time (interface/nmag/main.py:2313-2479)."""
+        c = self.clock
+        names = [n for n, _ in NDT_CLOCK]
+        units = [u for _, u in NDT_CLOCK]
+        values = [float(c.time), int(c.id), int(c.step), float(c.last_step_dt_si),
+                  float(c.stage_time), int(c.stage_step), int(c.stage)]
+        f = self._all_fields_su()
+        V = self._volumes
+        for field, sub in zip(FIELD_UNITS, self._subfield_names(FIELD_UNITS)):
+            si = FIELD_UNITS[field]
+            vals = f[field] * su_factor(si.dims)
+            avg = (vals * (V[:, None] if vals.ndim > 1 else V)).sum(0) / V.sum()
+            names.append(sub)
+            units.append(si)
+            values.append([float(a) for a in avg] if vals.ndim > 1 else float(avg))
+        names.append("maxangle_m_%s" % self._mat_name())
+        units.append("<deg>")
+        values.append(datafile.max_angle_deg(f["m"], self._edges()))
+        now = time.localtime()
+        names += ["localtime", "unixtime"]
+        units += ["<>", SI(1, "s")]
+        values += ["%04d/%02d/%02d-%02d:%02d:%02d" % now[:6], time.time()]
+        return names, values, units, f
+
+    def save_data(self, fields=None, avoid_same_step=False):
+        """Append the averages to <name>_dat.ndt and, with `fields` ('all', a
+        field name or a list of names), the fields to <name>_dat.npz
+        (interface/nmag/main.py:2481-2535).  With avoid_same_step, a step
+        whose averages were saved already is not saved again."""
+        f = None
+        if not (avoid_same_step and self.clock.step in self._saved_steps):
+            self.clock.id += 1
+            names, values, units, f = self._compute_averages()
+            path = self._ndtfilename()
+            header = not os.path.exists(path) or os.path.getsize(path) == 0
+            datafile.write_ndt_row(path, names, values, units, header)
+            self._saved_steps.add(self.clock.step)
+        if fields:
+            if isinstance(fields, str):
+                fields = [] if fields.lower() == "all" else [fields]
+            self._save_fields(list(fields), f)
+
+    def _save_fields(self, fieldnames, f=None):
+        for name in fieldnames:
+            if name not in FIELD_UNITS:
+                raise NmagUserError("cannot save unknown field '%s'" % name)
+        names = fieldnames or list(FIELD_UNITS)
+        if f is None:
+            f = self._all_fields_su()
+        arrays = {sub: f[n] * su_factor(FIELD_UNITS[n].dims)
+                  for n, sub in zip(names, self._subfield_names(names))}
+        c = self.clock
+        meta = {"id": c.id, "step": c.step, "stage": c.stage, "stage_step": c.stage_step,
+                "time": c.time, "stage_time": c.stage_time}
+        path = self._datafilename()
+        row = len(datafile.read_meta(path)) if os.path.exists(path) else 0
+        datafile.append_fields(path, row, meta, arrays, self._coords_su * 1e-9, self._tets)
+
+    def save_restart_file(self, filename=None, fieldnames=["m"], all=False):
+        """Write the fields (default m) and the clock, from which a run
+        continues with --restart (interface/nmag/main.py:2764-2857)."""
+        names = list(FIELD_UNITS) if all else list(fieldnames)
+        for n in names:
+            if n not in FIELD_UNITS:
+                raise NmagUserError("cannot save unknown field '%s'" % n)
+        f = self._all_fields_su() if any(n != "m" for n in names) else \
+            {"m": self.magnet.get_field("m")}
+        arrays = {sub: f[n] * su_factor(FIELD_UNITS[n].dims)
+                  for n, sub in zip(names, self._subfield_names(names))}
+        datafile.write_restart(filename or self.get_restart_file_name(), arrays,
+                               dict(self.clock), self._coords_su * 1e-9)
+
+    def load_restart_file(self, file_name=None):
+        """Restore m and the clock from a restart file; a file saved at
+        convergence continues with the next stage (main.py:2859-2885)."""
+        path = file_name or self.get_restart_file_name()
+        fields, clock, pos = datafile.read_restart(path)
+        if pos.shape != self._coords_su.shape:
+            raise NmagUserError("restart file %s holds %d nodes, this mesh %d"
+                                % (path, len(pos), len(self._coords_su)))
+        self.load_m_from_h5file(path)
+        self.clock.update(clock)
+        if self.clock.convergence:
+            self.clock.stage += 1
+
+    def load_m_from_h5file(self, filename, id=None, row=None):
+        """Set m from a field store or restart file written by this package
+        (the first saved m unless id/row say otherwise; main.py:1876-1901).
+        Files are .npz archives: no HDF5 library is available here."""
+        if not (os.path.exists(filename) and zipfile.is_zipfile(filename)):
+            raise NmagUserError("%s is not a data file of this package (.npz); HDF5 files "
+                                "cannot be read without an HDF5 library" % filename)
+        self.set_m(datafile.read_subfield(filename, "m_%s" % self._mat_name(), id=id, row=row))
 
     # -------------------------------------------------------- observables
     def _field_su(self, field):
@@ -299,23 +599,10 @@ class Simulation:
             return g.get_field(field)
         if field == "M":
             return self._mat.su_Ms * g.get_field("m")
-        if field.startswith("E_"):
+        if field.startswith("E_") and field in FIELD_UNITS:
             g.execute("update_dmdt")
-            m = g.get_field("m")
-            Ms, mu0 = self._mat.su_Ms, self._mat.su_mu0
-            # energy densities (nmag_lam.py template_E)
-            e = {"E_demag": -0.5 * mu0 * Ms * (m * g.get_field("H_demag")).sum(1),
-                 "E_exch": -0.5 * mu0 * Ms * (m * g.get_field("H_exch")).sum(1),
-                 "E_ext": -1.0 * mu0 * Ms * (m * g.get_field("H_ext")).sum(1)}
-            an = self._mat.su_anisotropy()
-            if an is not None:
-                k1, k2, axis = an
-                u = m @ axis
-                e["E_anis"] = -k1 * u ** 2 - k2 * u ** 4
-            else:
-                e["E_anis"] = np.zeros(len(m))
-            e["E_total"] = e["E_demag"] + e["E_exch"] + e["E_ext"] + e["E_anis"]
-            return e[field]
+            f = {k: g.get_field(k) for k in ("m", "H_demag", "H_exch", "H_ext")}
+            return self._energies_su(f)[field]
         raise NmagUserError("unknown field '%s'" % field)
 
     def _split(self, subfieldname):
@@ -350,3 +637,16 @@ class Simulation:
         g = self.magnet
         return {k: g.timer(k) for k in ("exch_div", "cg_nbc", "bem", "cg_dbc", "demag_llg",
                                         "jtimes")}
+
+
+def get_subfield_from_h5file(filename, subfieldname, id=None, row=None):
+    """SI array of a subfield saved by save_data(fields=...) or
+    save_restart_file: the save with clock id `id`, else the row-th save
+    holding it (default the first); interface/nmag/main.py:2915-2956."""
+    return datafile.read_subfield(filename, subfieldname, id=id, row=row)
+
+
+def get_subfield_positions_from_h5file(filename, subfieldname=None):
+    """Node positions (m) of the mesh the file's fields live on
+    (interface/nmag/main.py:2958-2985)."""
+    return datafile.read_positions(filename)

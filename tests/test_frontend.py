@@ -65,7 +65,9 @@ def test_unsupported_options_fail_loudly():
     s = Simulation()
     with pytest.raises(NmagUserError):
         s.magnet
-    assert every("step", 5).value == 5
+    assert every("step", 5).delta == 5
+    with pytest.raises(ValueError):
+        every("step", 0)
 
 
 def test_element_to_node_projection_reduces_to_uniform_and_conserves_energy_weights():

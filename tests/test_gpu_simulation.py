@@ -7,9 +7,15 @@ import pytest
 
 import oracle
 from helpers import PERMALLOY, orc_params, rel_err
-from nmag import SI, MagMaterial, Simulation, every, kuhn_cuboid, uniaxial_anisotropy
+from nmag import SI, MagMaterial, Simulation, at, every, kuhn_cuboid, uniaxial_anisotropy
 
 pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(autouse=True)
+def _in_tmp(tmp_path, monkeypatch):
+    # relax saves <name>_dat.ndt / _dat.npz in the working directory
+    monkeypatch.chdir(tmp_path)
 
 
 def _sim(nx=6, ny=5, nz=3, h=2.5, alpha=0.5, **mat_kw):
@@ -63,3 +69,42 @@ def test_uniaxial_anisotropy_through_the_front_end():
     u = m[:, 2]
     expect = (2 * 4e4 / (4e-7 * math.pi * 8.6e5)) * u[:, None] * (np.array([0, 0, 1.0]) - u[:, None] * m)
     assert np.allclose(Ha, expect, rtol=1e-10, atol=1e-6)
+
+
+def test_saved_averages_fields_and_restart_on_the_device():
+    """relax saving averages every 20 steps and at the end of the stage, and
+    the fields at the end, then a restart round trip; the saved averages are
+    checked against the device fields."""
+    from nmag import datafile, get_subfield_from_h5file
+    sim, _, _, tets = _sim()
+    sim.set_m([1.0, 1.0, 0.3])
+    sim.relax(save=[("averages", every("step", 20) | at("stage_end")),
+                    ("fields", at("stage_end"))])
+    names, units, rows = datafile.read_ndt(sim.name + "_dat.ndt")
+    col = {n: i for i, n in enumerate(names)}
+    last = rows[-1]
+    assert int(last[col["step"]]) == sim.clock.step
+    assert [int(r[col["step"]]) % 20 for r in rows[:-1]] == [0] * (len(rows) - 1)
+    m_avg = sim.get_subfield_average_siv("m", "Py")
+    Hd_avg = sim.get_subfield_average_siv("H_demag")
+    for k in range(3):
+        assert last[col["m_Py_%d" % k]] == pytest.approx(m_avg[k], rel=1e-12, abs=1e-14)
+        assert last[col["H_demag_%d" % k]] == pytest.approx(Hd_avg[k], rel=1e-9, abs=1e-6)
+    m = sim.get_subfield("m_Py")
+    edges = datafile.mesh_edges(tets)
+    assert last[col["maxangle_m_Py"]] == pytest.approx(datafile.max_angle_deg(m, edges))
+    assert np.allclose(get_subfield_from_h5file(sim.name + "_dat.npz", "m_Py"), m)
+    sim.save_restart_file()
+    sim2 = Simulation(name="second", ksp_tolerances={"NBC.rtol": 1e-10, "DBC.rtol": 1e-10})
+    coords, _ = kuhn_cuboid(6, 5, 3, 2.5)
+    sim2.load_mesh((coords, tets), [("Py", MagMaterial("Py", llg_damping=0.5, **PERMALLOY))],
+                   unit_length=SI(1e-9, "m"))
+    sim2.load_restart_file(sim.get_restart_file_name())
+    # loading goes through set_m, which renormalises (the integrated m is
+    # unit length only to the integrator's tolerance)
+    m_unit = m / np.linalg.norm(m, axis=1, keepdims=True)
+    assert np.abs(np.linalg.norm(m, axis=1) - 1).max() < 1e-5
+    assert np.allclose(sim2.get_subfield("m_Py"), m_unit, rtol=1e-14, atol=1e-15)
+    assert sim2.clock.step == sim.clock.step
+    sim.set_m(m)
+    assert rel_err(sim2.get_subfield("H_demag"), sim.get_subfield("H_demag")) < 1e-9
