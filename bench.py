@@ -395,7 +395,7 @@ def run_precision(slab, window_ps=1000.0, checkpoint_ps=10.0, seed=1, rtol=1e-10
     ref, avg = {}, {}
     # "fp64_tight": FP64 integrated at rtol = atol = 1e-8, the size of the
     # integration error itself at the working tolerance 1e-6
-    for prec in ("fp64", "mixed", "bf16", "fp64_tight"):
+    for prec in ("fp64", "mixed", "mixed24", "bf16", "fp64_tight"):
         g = graphs(NativeMagnet(coords, tets, mat.su_coeffs(),
                                 precision="fp64" if prec == "fp64_tight" else prec,
                                 nbc_rtol=rtol, dbc_rtol=rtol))
@@ -416,7 +416,7 @@ def run_precision(slab, window_ps=1000.0, checkpoint_ps=10.0, seed=1, rtol=1e-10
             series.append((m * V[:, None]).sum(0) / V.sum())
         avg[prec] = np.array(series)
         g.close()
-    for prec in ("mixed", "bf16"):
+    for prec in ("mixed", "mixed24", "bf16"):
         out[prec]["mean_m_max_dev_vs_fp64"] = float(np.abs(avg[prec] - avg["fp64"]).max())
     out["fp64_rtol_1e-8_mean_m_max_dev_vs_fp64"] = float(
         np.abs(avg["fp64_tight"] - avg["fp64"]).max())
@@ -424,7 +424,7 @@ def run_precision(slab, window_ps=1000.0, checkpoint_ps=10.0, seed=1, rtol=1e-10
     coords, tets, mat, Hext, m = slab
     sref = {}
     out["slab"] = {}
-    for prec in ("fp64", "mixed", "bf16"):
+    for prec in ("fp64", "mixed", "mixed24", "bf16"):
         g = graphs(NativeMagnet(coords, tets, mat.su_coeffs(), precision=prec))
         g.set_field("H_ext", Hext)
         g.set_field("m", m)
@@ -590,7 +590,8 @@ def main():
     for route in ("dense", "iterative"):
         if route not in args.routes.split(","):
             continue
-        for prec in ("fp64", "mixed"):
+        # the 24-bit mode differs from FP32-mixed only in the dense operator
+        for prec in ("fp64", "mixed") + (("mixed24",) if route == "dense" else ()):
             key = prec + "/" + route
             log("timing %s ..." % key)
             results[key], outs[key] = run_gpu(prec, route, coords, tets, mat, Hext, states,
@@ -624,6 +625,11 @@ def main():
                                    pot_bytes(8), fp64["kernel_ms"]["potential"],
                                    MEASURED_TRAFFIC.get((8, n))),
     }
+    m24 = results.get("mixed24/dense")
+    if m24 is not None:
+        rooflines["potential_f24"] = roofline(
+            "k_dense_gemv<F24> potential operator (N=%d)" % n, pot_bytes(3),
+            m24["kernel_ms"]["potential"], MEASURED_TRAFFIC.get((3, n)))
     roof = rooflines.get(dominant)
     pot64_gbs = pot_bytes(8) / (fp64["kernel_ms"]["potential"] * 1e-3) / 1e9
     # iterative route (the reference's structure): per-iteration algorithmic
@@ -649,7 +655,8 @@ def main():
         for route in args.dynamics_routes.split(","):
             dyn[route] = {}
             avgs = {}
-            for prec in ("fp64", "mixed"):
+            precs = ("fp64", "mixed") + (("mixed24",) if route == "dense" else ())
+            for prec in precs:
                 wall, stats, avgs[prec] = run_dynamics(prec, route, coords, tets, mat, Hext, m0,
                                                        args.sim_window, 5.0, args.rtol)
                 dyn[route][prec] = {
@@ -657,6 +664,9 @@ def main():
                     "wall_s": round(wall, 3), **{k: int(v) for k, v in stats.items()}}
             dyn[route]["mean_m_max_dev_mixed_vs_fp64"] = float(
                 np.abs(avgs["mixed"] - avgs["fp64"]).max())
+            if "mixed24" in avgs:
+                dyn[route]["mean_m_max_dev_mixed24_vs_fp64"] = float(
+                    np.abs(avgs["mixed24"] - avgs["fp64"]).max())
         dyn["window_ps"] = args.sim_window
         dyn["integrator"] = "BDF order<=2, Newton-GMRES, rtol=atol=1e-6, from the seeded S-state"
 
@@ -723,6 +733,16 @@ def main():
         "mixed_kernel_ms": kern,
         "mixed_vs_fp64_speedup": round(mixed["evals_per_s"] / fp64["evals_per_s"], 3),
         "mixed_dmdt_max_rel_err": err,
+        # the potential operator in 24 bits (FP32 rounded to 15 significand
+        # bits, 3 bytes per entry), everything else as FP32-mixed
+        "mixed24": None if m24 is None else {
+            "value": round(m24["evals_per_s"], 2),
+            "ms_per_step": round(m24["ms_per_eval"], 4),
+            "vs_fp64_speedup": round(m24["evals_per_s"] / fp64["evals_per_s"], 3),
+            "dmdt_max_rel_err_vs_fp64": rel(outs["mixed24/dense"], outs["fp64/dense"]),
+            "kernel_ms": m24["kernel_ms"],
+            "potential_bytes": m24["potential_bytes"],
+            "autotuned": m24["autotuned"]},
         "dense_vs_iterative_fp64_dmdt_rel_diff": (
             rel(outs["fp64/dense"], outs["fp64/iterative"]) if "fp64/iterative" in outs
             else None),

@@ -43,13 +43,17 @@ typedef struct {
 enum {
   NSIM_FP64 = 0,       /* everything in FP64                                */
   NSIM_MIXED_FP32 = 1, /* FP32 SpMV and BEM, FP64 dots/state/norms          */
-  NSIM_MIXED_BF16 = 2  /* as MIXED_FP32 but the BEM matrix stored in BF16   */
+  NSIM_MIXED_BF16 = 2, /* as MIXED_FP32 but the BEM matrix stored in BF16   */
+  NSIM_MIXED_F24 = 3   /* as MIXED_FP32 but the dense potential operator
+                        * stored in 24 bits: FP32 rounded to 15 stored
+                        * significand bits (unit roundoff 2^-16), 3/4 of
+                        * FP32's bytes                                      */
 };
 
 /* Solver options; the KSP tolerances mirror Simulation(ksp_tolerances=...)
  * (interface/nmag/main.py:819-843, nmag_lam.py:614-633). */
 typedef struct {
-  int precision;   /* NSIM_FP64 / NSIM_MIXED_FP32 / NSIM_MIXED_BF16        */
+  int precision;   /* NSIM_FP64 / _MIXED_FP32 / _MIXED_BF16 / _MIXED_F24    */
   int do_demag;    /* 0: exchange/anisotropy/applied field only            */
   double nbc_rtol; /* Neumann solve relative tolerance                     */
   double dbc_rtol; /* Dirichlet solve relative tolerance                   */

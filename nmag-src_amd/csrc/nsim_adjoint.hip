@@ -148,11 +148,12 @@ __global__ __launch_bounds__(256) void k_adj_boundary(int nb, const int* __restr
 
 template <typename TB>
 struct TVec;
+// load(row, j, ld, o): entries j .. j+V-1 of a row of length ld
 template <>
 struct TVec<double> {
   static constexpr int V = 2;
-  __device__ static void load(const double* p, double* o) {
-    const double2 v = *reinterpret_cast<const double2*>(p);
+  __device__ static void load(const double* row, int j, int, double* o) {
+    const double2 v = *reinterpret_cast<const double2*>(row + j);
     o[0] = v.x;
     o[1] = v.y;
   }
@@ -160,8 +161,8 @@ struct TVec<double> {
 template <>
 struct TVec<float> {
   static constexpr int V = 4;
-  __device__ static void load(const float* p, double* o) {
-    const float4 v = *reinterpret_cast<const float4*>(p);
+  __device__ static void load(const float* row, int j, int, double* o) {
+    const float4 v = *reinterpret_cast<const float4*>(row + j);
     o[0] = v.x;
     o[1] = v.y;
     o[2] = v.z;
@@ -169,10 +170,17 @@ struct TVec<float> {
   }
 };
 template <>
+struct TVec<F24> {
+  static constexpr int V = 8;
+  __device__ static void load(const F24* row, int j, int ld, double* o) {
+    f24_load8(row, j, ld, o);
+  }
+};
+template <>
 struct TVec<uint16_t> {
   static constexpr int V = 8;
-  __device__ static void load(const uint16_t* p, double* o) {
-    const uint4 v = *reinterpret_cast<const uint4*>(p);
+  __device__ static void load(const uint16_t* row, int j, int, double* o) {
+    const uint4 v = *reinterpret_cast<const uint4*>(row + j);
     const uint32_t w[4] = {v.x, v.y, v.z, v.w};
     for (int k = 0; k < 4; k++) {
       o[2 * k] = __uint_as_float(w[k] << 16);
@@ -198,7 +206,7 @@ __global__ __launch_bounds__(256) void k_bem_gemv_t(int nb, int ldb, int rows_pe
   for (int v = 0; v < V; v++) acc[v] = 0.0;
   for (int i = r0; i < r1; i++) {
     double bv[V];
-    TVec<TB>::load(B + (size_t)i * ldb + j0, bv);
+    TVec<TB>::load(B + (size_t)i * ldb, j0, ldb, bv);
     const double ei = e[i];
     for (int v = 0; v < V; v++) acc[v] += bv[v] * ei;
   }
@@ -316,6 +324,7 @@ hipError_t launch_dense_gemv_t(const DenseOp& op, const double* e, double* part,
   if (op.h_n.size() != 1) return hipErrorInvalidConfiguration;
   if (op.type == kBemF64) return gemv_t_typed<double>(op, e, part, chi, st);
   if (op.type == kBemF32) return gemv_t_typed<float>(op, e, part, chi, st);
+  if (op.type == kBemF24) return gemv_t_typed<F24>(op, e, part, chi, st);
   return gemv_t_typed<uint16_t>(op, e, part, chi, st);
 }
 

@@ -136,6 +136,12 @@ struct BemVec<uint16_t> {
   using X = float;
   using A = float;
 };
+template <>
+struct BemVec<F24> {
+  static constexpr int V = 8;  // one 16-byte and one 8-byte load per lane
+  using X = double;
+  using A = double;
+};
 
 // 16-byte row chunks of a dense operator.  The operator is streamed once per
 // launch with no reuse, so the loads are non-temporal: on MI355X this lifts a
@@ -171,6 +177,16 @@ __device__ inline void load_row_chunk<uint16_t, float>(const uint16_t* p, float*
   }
 }
 
+// Entries c .. c+V-1 of a row of length ldb.
+template <typename TB, typename X>
+__device__ inline void load_row(const TB* row, int c, int ldb, X* out) {
+  load_row_chunk<TB, X>(row + c, out);
+}
+template <>
+__device__ inline void load_row<F24, double>(const F24* row, int c, int ldb, double* out) {
+  f24_load8(row, c, ldb, out);
+}
+
 // xs[j] = x[j] for j < nb, 0 up to ldb, by T threads: eight loads in flight
 // per thread before any LDS store (a plain loop issues one load at a time
 // and waits on L2 latency before every LDS store).
@@ -193,8 +209,9 @@ __device__ inline void stage_x(X* xs, const double* __restrict__ x, int nb, int 
 }
 
 // NR consecutive rows rb.. of a row-major segment (stride ldb) against the
-// staged x: y[map[rb + k]] = row . x.  Every lane loads 16 bytes of each row
-// per step; the partial sums are reduced across the wavefront.
+// staged x: y[map[rb + k]] = row . x.  Every lane loads V entries of each
+// row per step (16 bytes; 24 for 24-bit storage); the partial sums are
+// reduced across the wavefront.
 template <typename TB, int NR, typename X>
 __device__ inline void gemv_rows(const TB* __restrict__ B, int ldb, int rb, const X* xs,
                                  int lane, const int* __restrict__ map, double* __restrict__ y) {
@@ -215,7 +232,7 @@ __device__ inline void gemv_rows(const TB* __restrict__ B, int ldb, int rb, cons
 #pragma unroll
     for (int k = 0; k < NR; k++) {
       X bv[V];
-      load_row_chunk<TB, X>(rp[k] + c, bv);
+      load_row<TB, X>(rp[k], c, ldb, bv);
 #pragma unroll
       for (int e = 0; e < V; e++) acc[k] += (A)bv[e] * (A)xv[e];
     }
@@ -477,6 +494,7 @@ hipError_t launch_dense_gemv(const DenseOp& op, const double* x, double* y, hipS
   if (!op.A || op.grid == 0) return hipSuccess;
   if (op.type == kBemF64) return gemv_typed<double>(op, x, y, st);
   if (op.type == kBemF32) return gemv_typed<float>(op, x, y, st);
+  if (op.type == kBemF24) return gemv_typed<F24>(op, x, y, st);
   return gemv_typed<uint16_t>(op, x, y, st);
 }
 

@@ -414,7 +414,7 @@ int create_impl(const char* who, int ninst, const nsim_mesh* meshes,
     h->opt.nbc_maxits = h->opt.dbc_maxits = 10000;
     h->opt.potential = 0;
   }
-  if (h->opt.precision < NSIM_FP64 || h->opt.precision > NSIM_MIXED_BF16)
+  if (h->opt.precision < NSIM_FP64 || h->opt.precision > NSIM_MIXED_F24)
     return cleanup(fail(me + ": unknown precision"));
   if (h->opt.nbc_maxits <= 0) h->opt.nbc_maxits = 10000;
   if (h->opt.dbc_maxits <= 0) h->opt.dbc_maxits = 10000;
@@ -439,9 +439,12 @@ int create_impl(const char* who, int ninst, const nsim_mesh* meshes,
   d.ninst = ninst;
   for (int sl = 0; sl < o.nslices; sl++)
     d.max_width = std::max(d.max_width, (o.sell_off[sl + 1] - o.sell_off[sl]) / kSlice);
-  const int stype = h->opt.precision == NSIM_FP64         ? kBemF64
-                    : h->opt.precision == NSIM_MIXED_FP32 ? kBemF32
-                                                          : kBemBF16;
+  // storage of the BEM matrix, and of the dense potential operator (24-bit
+  // storage is offered for the potential operator only)
+  const int stype = h->opt.precision == NSIM_FP64        ? kBemF64
+                    : h->opt.precision == NSIM_MIXED_BF16 ? kBemBF16
+                                                          : kBemF32;
+  const int ptype = h->opt.precision == NSIM_MIXED_F24 ? kBemF24 : stype;
   d.bem.type = stype;
   // BEM segments: the boundary nodes are in ascending node order, so each
   // magnet's boundary nodes form one contiguous run
@@ -478,12 +481,11 @@ int create_impl(const char* who, int ninst, const nsim_mesh* meshes,
       pot_elems += (long long)nk * (((nk + 63) / 64) * 64);
       max_n = std::max(max_n, nk);
     }
-    const size_t esz = stype == kBemF64 ? 8 : stype == kBemF32 ? 4 : 2;
-    const bool fits = max_n <= dense_gemv_max_order(stype) &&
-                      (double)pot_elems * esz <= kPotentialBudgetBytes;
+    const bool fits = max_n <= dense_gemv_max_order(ptype) &&
+                      (double)dense_bytes(ptype, pot_elems) <= kPotentialBudgetBytes;
     if (h->opt.potential > 0 && !fits)
       return cleanup(fail(me + ": the dense potential operator needs every magnet to have at "
-                               "most " + std::to_string(dense_gemv_max_order(stype)) +
+                               "most " + std::to_string(dense_gemv_max_order(ptype)) +
                           " nodes and at most 32 GiB in all"));
     h->use_potential = h->opt.do_demag && o.nb > 0 && h->opt.potential >= 0 && fits;
   }
@@ -551,8 +553,7 @@ int create_impl(const char* who, int ninst, const nsim_mesh* meshes,
     UP(d.bem.aoff, d.bem.h_aoff);
     d.bem.map = d.bnode;
     if (setup_gemv(h, d.bem, 2, 2, 4) != 0) return cleanup(-1);
-    const size_t esz = stype == kBemF64 ? 8 : stype == kBemF32 ? 4 : 2;
-    e = hipMalloc(&d.bem.A, (size_t)d.bem.elems * esz);
+    e = hipMalloc(&d.bem.A, dense_bytes(stype, d.bem.elems));
     if (e != hipSuccess) return cleanup(fail(std::string("BEM alloc: ") + hipGetErrorString(e)));
     h->track(d.bem.A);
     e = launch_bem_assemble(d, h->stream);
@@ -890,8 +891,7 @@ namespace {
 // n_k x n_k row-major without padding.
 int copy_dense(nsim_handle* h, const DenseOp& op, double* out) {
   if (!op.A) return 0;
-  const size_t esz = op.type == kBemF64 ? 8 : op.type == kBemF32 ? 4 : 2;
-  std::vector<unsigned char> buf((size_t)op.elems * esz);
+  std::vector<unsigned char> buf(dense_bytes(op.type, op.elems));
   HIPCHK(hipStreamSynchronize(h->stream));
   HIPCHK(hipMemcpy(buf.data(), op.A, buf.size(), hipMemcpyDeviceToHost));
   size_t o = 0;
@@ -903,7 +903,18 @@ int copy_dense(nsim_handle* h, const DenseOp& op, double* out) {
         double v;
         if (op.type == kBemF64) v = reinterpret_cast<double*>(buf.data())[k];
         else if (op.type == kBemF32) v = reinterpret_cast<float*>(buf.data())[k];
-        else {
+        else if (op.type == kBemF24) {
+          // row i of the segment starts at byte 3 (base + i ld); within it
+          // entry j sits in a block of up to 512: high 16 bits, then low 8
+          const unsigned char* row = buf.data() + 3 * (base + i * ld);
+          const size_t j0 = j & ~(size_t)511, nbk = std::min<size_t>(512, ld - j0);
+          uint16_t hi;
+          std::memcpy(&hi, row + 3 * j0 + 2 * (j - j0), 2);
+          const uint32_t u = ((uint32_t)hi << 16) | ((uint32_t)row[3 * j0 + 2 * nbk + (j - j0)] << 8);
+          float f;
+          std::memcpy(&f, &u, 4);
+          v = f;
+        } else {
           uint32_t u = (uint32_t)reinterpret_cast<uint16_t*>(buf.data())[k] << 16;
           float f;
           std::memcpy(&f, &u, 4);
@@ -1024,8 +1035,8 @@ int nsim_get_stat(nsim_handle* h, const char* name, double* out) {
         {"pot_waves", p.waves}, {"pot_grid", p.grid}, {"n_magnets", h->d.ninst},
         {"potential", p.A ? 1.0 : 0.0}, {"tuned_bem_us", h->tuned_bem_us},
         {"tuned_cg_us", h->tuned_cg_us}, {"tuned_pot_us", h->tuned_pot_us},
-        {"potential_bytes", (double)p.elems * (p.type == kBemF64 ? 8 : p.type == kBemF32 ? 4 : 2)},
-        {"bem_bytes", (double)b.elems * (b.type == kBemF64 ? 8 : b.type == kBemF32 ? 4 : 2)}};
+        {"potential_bytes", (double)dense_bytes(p.type, p.elems)},
+        {"bem_bytes", (double)dense_bytes(b.type, b.elems)}};
     const auto it = simple.find(s);
     if (it != simple.end()) {
       *out = it->second;

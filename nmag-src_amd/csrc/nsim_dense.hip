@@ -90,6 +90,33 @@ __global__ __launch_bounds__(256) void k_store_rowmajor(int n, int ld,
   }
 }
 
+// The same transposing store into 24-bit rows (F24): entry (i, j) is rounded
+// and its high 16 and low 8 bits go to the two parts of its row block.
+__global__ __launch_bounds__(256) void k_store_rowmajor_f24(int n, int ld,
+                                                            const double* __restrict__ A,
+                                                            long long lda,
+                                                            unsigned char* __restrict__ out) {
+  __shared__ double tile[32][33];
+  const int j0 = blockIdx.x * 32, i0 = blockIdx.y * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  for (int k = ty; k < 32; k += 8) {
+    const int i = i0 + tx, j = j0 + k;
+    tile[k][tx] = (i < n && j < n) ? A[i + (long long)j * lda] : 0.0;
+  }
+  __syncthreads();
+  for (int k = ty; k < 32; k += 8) {
+    const int i = i0 + k, j = j0 + tx;
+    if (i < n && j < ld) {
+      const unsigned u = f24_round(tile[tx][k]);
+      size_t hi, lo;
+      f24_offsets(j, ld, &hi, &lo);
+      unsigned char* row = out + (size_t)i * ld * 3;
+      *reinterpret_cast<uint16_t*>(row + hi) = (uint16_t)(u >> 8);
+      row[lo] = (unsigned char)(u & 0xffu);
+    }
+  }
+}
+
 inline int blocks(long long n) { return (int)((n + 255) / 256); }
 
 }  // namespace
@@ -136,6 +163,8 @@ hipError_t dense_store_rowmajor(int type, int n, int ld, const double* A, long l
     k_store_rowmajor<double><<<grid, 256, 0, st>>>(n, ld, A, lda, (double*)out);
   else if (type == kBemF32)
     k_store_rowmajor<float><<<grid, 256, 0, st>>>(n, ld, A, lda, (float*)out);
+  else if (type == kBemF24)
+    k_store_rowmajor_f24<<<grid, 256, 0, st>>>(n, ld, A, lda, (unsigned char*)out);
   else
     k_store_rowmajor<uint16_t><<<grid, 256, 0, st>>>(n, ld, A, lda, (uint16_t*)out);
   return hipGetLastError();

@@ -157,6 +157,54 @@ __device__ inline void store_b<uint16_t>(uint16_t* p, double v) {
   *p = f32_to_bf16((float)v);
 }
 
+// 24-bit storage of a row-major dense operator: each entry is the FP32 value
+// with its low 8 bits rounded off to nearest (ties to even), keeping FP32's
+// sign and exponent and 15 stored significand bits (16-bit precision, unit
+// roundoff 2^-16).  A row of ld entries (ld a
+// multiple of 64) takes 3 ld bytes and is split into blocks of up to 512
+// entries; a block of nb entries holds the entries' high 16 bits (2 nb
+// bytes) followed by their low 8 bits (nb bytes), so that eight consecutive
+// entries are one 16-byte and one 8-byte aligned load.
+struct F24 {
+  unsigned char b[3];
+};
+constexpr int kF24Block = 512;
+
+typedef unsigned int f24_u4 __attribute__((ext_vector_type(4)));
+typedef unsigned int f24_u2 __attribute__((ext_vector_type(2)));
+
+__device__ inline unsigned f24_round(double v) {
+  const unsigned u = __float_as_uint((float)v);
+  return (u + 0x7fu + ((u >> 8) & 1u)) >> 8;
+}
+
+// Entry j of a 24-bit row: byte offsets of its high and low parts.
+__device__ inline void f24_offsets(int j, int ld, size_t* hi, size_t* lo) {
+  const int j0 = j & ~(kF24Block - 1);
+  const int nb = min(kF24Block, ld - j0);
+  *hi = 3 * (size_t)j0 + 2 * (size_t)(j - j0);
+  *lo = 3 * (size_t)j0 + 2 * (size_t)nb + (size_t)(j - j0);
+}
+
+// Entries j .. j+7 (j a multiple of 8) of a 24-bit row of length ld, read
+// once (non-temporal) and widened to T.
+template <typename T>
+__device__ inline void f24_load8(const F24* row, int j, int ld, T* out) {
+  size_t hi, lo;
+  f24_offsets(j, ld, &hi, &lo);
+  const unsigned char* base = reinterpret_cast<const unsigned char*>(row);
+  const f24_u4 h = __builtin_nontemporal_load(reinterpret_cast<const f24_u4*>(base + hi));
+  const f24_u2 l = __builtin_nontemporal_load(reinterpret_cast<const f24_u2*>(base + lo));
+  const unsigned hw[4] = {h.x, h.y, h.z, h.w};
+  const unsigned lw[2] = {l.x, l.y};
+#pragma unroll
+  for (int e = 0; e < 8; e++) {
+    const unsigned hi16 = (hw[e >> 1] >> (16 * (e & 1))) & 0xffffu;
+    const unsigned lo8 = (lw[e >> 2] >> (8 * (e & 3))) & 0xffu;
+    out[e] = (T)__uint_as_float((hi16 << 16) | (lo8 << 8));
+  }
+}
+
 // Wait until every memory operation this wave issued has completed.  A
 // workgroup barrier alone does not wait for stores on gfx950.
 __device__ inline void wait_all_memory() { __builtin_amdgcn_s_waitcnt(0); }

@@ -8,7 +8,13 @@
 
 namespace nsim {
 
-enum BemType { kBemF64 = 0, kBemF32 = 1, kBemBF16 = 2 };
+enum BemType { kBemF64 = 0, kBemF32 = 1, kBemBF16 = 2, kBemF24 = 3 };
+
+// Bytes of `elems` entries of a dense operator in storage type `type`.
+inline size_t dense_bytes(int type, long long elems) {
+  const size_t per = type == kBemF64 ? 8 : type == kBemF32 ? 4 : type == kBemF24 ? 3 : 2;
+  return (size_t)elems * per;
+}
 
 // Material and LLG coefficients of one magnet (simulation units).
 struct Coeffs {
@@ -22,7 +28,7 @@ struct Coeffs {
 // demag potential operator (x and y on all nodes).
 struct DenseOp {
   void* A = nullptr;
-  int type = kBemF64;                 // storage: FP64, FP32 or BF16
+  int type = kBemF64;                 // storage: FP64, FP32, BF16 or 24-bit
   int kind = 0;                       // 0: BEM matrix, 1: potential operator
   std::vector<int> h_n, h_ld, h_off;  // per segment: order, row stride (x64), x/map offset
   std::vector<long long> h_aoff;      // per segment: first element in A

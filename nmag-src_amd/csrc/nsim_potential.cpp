@@ -101,9 +101,8 @@ int build_potential(nsim_handle* h) {
   const HostOperators& o = h->host;
   hipStream_t st = h->stream;
   DenseOp& p = d.pot;
-  p.type = d.bem.type;
+  p.type = h->opt.precision == NSIM_MIXED_F24 ? kBemF24 : d.bem.type;
   p.kind = 1;
-  const size_t esz = p.type == kBemF64 ? 8 : p.type == kBemF32 ? 4 : 2;
   const int ninst = d.ninst;
   p.h_n.clear();
   p.h_ld.clear();
@@ -120,7 +119,7 @@ int build_potential(nsim_handle* h) {
     elems += (long long)nk * ld;
   }
   p.elems = elems;
-  HCHK(hipMalloc(&p.A, (size_t)elems * esz), "potential operator alloc");
+  HCHK(hipMalloc(&p.A, dense_bytes(p.type, elems)), "potential operator alloc");
   h->track(p.A);
   for (auto* v : {&p.h_n, &p.h_ld, &p.h_off}) {
     int* dv = nullptr;
@@ -227,7 +226,7 @@ int build_potential(nsim_handle* h) {
 
     // store row-major in the storage type
     HCHK(dense_store_rowmajor(p.type, N, p.h_ld[k], P, N,
-                              (char*)p.A + (size_t)p.h_aoff[k] * esz, st),
+                              (char*)p.A + dense_bytes(p.type, p.h_aoff[k]), st),
          "store");
     HCHK(hipStreamSynchronize(st), "sync");
     s.release(P);

@@ -15,8 +15,9 @@ _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
 _vp = ctypes.c_void_p
 
-FP64, MIXED_FP32, MIXED_BF16 = 0, 1, 2
-PRECISIONS = {"fp64": FP64, "mixed": MIXED_FP32, "fp32": MIXED_FP32, "bf16": MIXED_BF16}
+FP64, MIXED_FP32, MIXED_BF16, MIXED_F24 = 0, 1, 2, 3
+PRECISIONS = {"fp64": FP64, "mixed": MIXED_FP32, "fp32": MIXED_FP32, "bf16": MIXED_BF16,
+              "mixed24": MIXED_F24}
 
 
 class NsimMesh(ctypes.Structure):

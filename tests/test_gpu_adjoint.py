@@ -116,18 +116,21 @@ def test_rhs_vjp_against_oracle_jacobian():
     assert abs(r["y"] @ u.reshape(-1) - ref) <= 1e-6 * abs(ref)
 
 
-def test_rhs_vjp_mixed_precision_close_to_fp64():
+@pytest.mark.parametrize("prec,tol", [("mixed", 1e-5), ("mixed24", 1e-4)])
+def test_rhs_vjp_mixed_precision_close_to_fp64(prec, tol):
+    """The vector-Jacobian product with FP32 (24-bit) operator storage, whose
+    demag part applies the transposed potential operator, against FP64."""
     out = {}
-    for prec in ("fp64", "mixed"):
-        _, _, _, g = _magnet(precision=prec, rtol=1e-12)
+    for p in ("fp64", prec):
+        _, _, _, g = _magnet(precision=p, rtol=1e-12)
         m, H, k1, ax = _inputs(g.n, 3)
         _setup(g, H, k1, ax)
         lam = np.random.default_rng(4).standard_normal(3 * g.n)
-        out[prec] = g.rhs_vjp(m.reshape(-1), lam)
+        out[p] = g.rhs_vjp(m.reshape(-1), lam)
     for key in ("y", "K1", "H_ext"):
-        a, b = out["mixed"][key], out["fp64"][key]
-        assert np.max(np.abs(a - b)) <= 1e-5 * np.max(np.abs(b)), key
-    assert abs(out["mixed"]["Ms"] - out["fp64"]["Ms"]) <= 1e-5 * abs(out["fp64"]["Ms"])
+        a, b = out[prec][key], out["fp64"][key]
+        assert np.max(np.abs(a - b)) <= tol * np.max(np.abs(b)), key
+    assert abs(out[prec]["Ms"] - out["fp64"]["Ms"]) <= tol * abs(out["fp64"]["Ms"])
 
 
 def _problem(precision="fp64", checkpoint_every=10, n_steps=100, n_pulses=4, seed=7,

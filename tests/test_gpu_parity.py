@@ -88,10 +88,13 @@ def test_rhs_host_pointers_and_jtimes():
 
 @pytest.mark.parametrize("route", ["iterative", "dense"])
 @pytest.mark.parametrize("precision,tol_exch,tol_demag", [("fp32", 1e-7, 1e-6),
-                                                           ("bf16", 1e-7, 2e-2)])
+                                                           ("bf16", 1e-7, 2e-2),
+                                                           ("mixed24", 1e-7, 5e-5)])
 def test_mixed_precision_field_error(precision, tol_exch, tol_demag, route):
-    """FP32 (BF16) operator storage: the per-field error against the FP64 oracle
-    is that of rounding the stored operators; solves run to 1e-10."""
+    """FP32 (BF16, 24-bit) operator storage: the per-field error against the
+    FP64 oracle is that of rounding the stored operators; solves run to 1e-10.
+    (The 24-bit mode stores only the dense potential operator in 24 bits; on
+    the iterative route it is the FP32 mode.)"""
     _, _, orc, gpu, mat = make_pair(12, 12, 4, 2.5, precision=precision, rtol=1e-10,
                                     potential=route)
     m = random_m(orc.n, 1)
@@ -171,7 +174,31 @@ def test_dense_potential_operator_properties():
         gpu.get_field("phi1")
 
 
+def test_24bit_potential_operator_is_rounded_fp64_operator():
+    """The 24-bit potential operator holds, entry by entry, the FP64 operator
+    rounded to FP32 and then to nearest (ties to even) at 15 stored
+    significand bits.  585 nodes: rows of 640 entries span a full 512-entry
+    block and a partial one, so both block layouts are read back.  (The bit
+    formula below equals frexp-based rounding of the significand to 16 bits
+    with ties to even.)"""
+    M64 = make_pair(12, 8, 4, 2.5, potential="dense", build_bem=False)[3].potential()
+    M24 = make_pair(12, 8, 4, 2.5, precision="mixed24", potential="dense",
+                    build_bem=False)[3].potential()
+    assert M24.shape == M64.shape == (585, 585)
+    u = M64.astype(np.float32).view(np.uint32).astype(np.uint64)
+    u24 = (u + 0x7F + ((u >> 8) & 1)) >> 8
+    expect = (u24 << 8).astype(np.uint32).view(np.float32).astype(np.float64)
+    # the two handles build M with the same FP64 kernels
+    assert np.mean(M24 != expect) < 1e-3
+    nz = M64 != 0
+    rel = np.abs(M24[nz] - M64[nz]) / np.abs(M64[nz])
+    assert rel.max() <= 2.0 ** -16 * (1 + 2.0 ** -6)
+    # 15 significand bits kept, not 23: the rounding error reaches 2^-17
+    assert rel.max() > 2.0 ** -17
+
+
 @pytest.mark.parametrize("route,precision", [("dense", "fp64"), ("dense", "mixed"),
+                                             ("dense", "mixed24"),
                                              ("iterative", "fp64"), ("iterative", "bf16")])
 def test_dynamic_row_gemv_matches_static_plan(route, precision):
     """Rows claimed from a device counter (rows = 0) go through the same
