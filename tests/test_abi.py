@@ -62,3 +62,24 @@ def test_batch_create_rejects_invalid_meshes_before_touching_the_device():
     assert rc < 0 and not h.value
     assert b"magnet 1" in L.nsim_last_error() and b"out of range" in L.nsim_last_error()
     assert L.nsim_create_batch(0, meshes, mats, None, ctypes.byref(h)) < 0
+
+
+def test_create_rejects_unknown_precision_before_touching_the_device():
+    """The precision code is checked on the host: 0-3 are defined (FP64,
+    FP32-mixed, BF16, 24-bit potential operator), anything else is an error."""
+    import numpy as np
+    from nmag import kuhn_cuboid
+    L = _native.lib()
+    c, t = kuhn_cuboid(1, 1, 1, 2.5)
+    c = np.ascontiguousarray(c, dtype=np.float64)
+    t = np.ascontiguousarray(t, dtype=np.int32)
+    mesh = _native.NsimMesh(len(c), len(t), _native._dptr(c), t.ctypes.data_as(_native._ip))
+    mat = _native.NsimMaterial(0.86, -24.0, -0.2, -0.1, 0.1, 1.2566)
+    assert _native.PRECISIONS["mixed24"] == 3
+    for bad in (-1, 4):
+        opt = _native.NsimOptions(bad, 1, 1e-6, 1e-6, 100, 100, 0)
+        h = ctypes.c_void_p()
+        rc = L.nsim_create(ctypes.byref(mesh), ctypes.byref(mat), ctypes.byref(opt),
+                           ctypes.byref(h))
+        assert rc < 0 and not h.value
+        assert b"unknown precision" in L.nsim_last_error()

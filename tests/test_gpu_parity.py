@@ -197,6 +197,37 @@ def test_24bit_potential_operator_is_rounded_fp64_operator():
     assert rel.max() > 2.0 ** -17
 
 
+def test_route_selection_at_the_dense_size_limit():
+    """The dense potential route stages rho in 160 KiB of LDS: at most 20,480
+    FP64 entries.  A 20,000-node magnet takes it by default; a 20,500-node one
+    falls back to the iterative route, and asking for the dense route there is
+    an error that names the limit.  The fallback's field of a uniform state
+    has Aharoni's demagnetising factor."""
+    from nmag import kuhn_cuboid
+    from nmag._native import NativeMagnet
+    from test_oracle import aharoni_nz
+    mat = material()
+    c0, t0 = kuhn_cuboid(39, 24, 19, 2.5)     # 40 x 25 x 20 = 20,000 nodes
+    c1, t1 = kuhn_cuboid(40, 24, 19, 2.5)     # 41 x 25 x 20 = 20,500 nodes
+    g0 = NativeMagnet(c0, t0, mat.su_coeffs(), potential="auto")
+    assert g0.n == 20000 and g0.stat("potential") == 1
+    g0.close()
+    with pytest.raises(Exception) as err:
+        NativeMagnet(c1, t1, mat.su_coeffs(), potential="dense")
+    assert "20480" in str(err.value)
+    g1 = NativeMagnet(c1, t1, mat.su_coeffs(), potential="auto", nbc_rtol=1e-10,
+                      dbc_rtol=1e-10)
+    assert g1.n == 20500 and g1.stat("potential") == 0
+    g1.set_field("m", np.tile([0.0, 0.0, 1.0], (g1.n, 1)))
+    g1.execute()
+    hz = g1.get_field("H_demag")[:, 2]
+    V = g1.get_field("volumes")
+    g1.close()
+    # 40 x 24 x 19 cubes of 2.5 nm: a 100 x 60 x 47.5 nm cuboid
+    Nz = -(hz * V).sum() / V.sum() / mat.su_Ms
+    assert abs(Nz - aharoni_nz(100.0, 60.0, 47.5)) < 0.02
+
+
 @pytest.mark.parametrize("route,precision", [("dense", "fp64"), ("dense", "mixed"),
                                              ("dense", "mixed24"),
                                              ("iterative", "fp64"), ("iterative", "bf16")])
