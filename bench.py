@@ -565,7 +565,7 @@ def main():
         print("--routes must start with dense (the headline route)", file=sys.stderr)
         sys.exit(2)
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path "
+        print("bench.py measures one GPU: the library has no multi-GPU path "
               "(each magnet runs on a single MI355X); use --gpus 1", file=sys.stderr)
         sys.exit(2)
 
