@@ -10,6 +10,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cfloat>
 #include <cmath>
 
 namespace nsim {
@@ -75,6 +76,9 @@ __host__ __device__ inline double ts_factorial(int k) {
   for (int i = 2; i <= k; i++) f *= i;
   return f;
 }
+
+// Finite (neither NaN nor +-inf), usable from host and device code alike.
+__host__ __device__ inline bool ts_finite(double v) { return fabs(v) <= DBL_MAX; }
 
 __host__ __device__ inline bool ts_finished(const TsState& s) {
   if (s.max_steps > 0 && s.steps_call >= s.max_steps) return true;
@@ -188,7 +192,7 @@ __host__ __device__ inline void ts_after_newton(TsState& s, double d, double err
   s.newton++;
   s.linear += its;
   bool converged = false, failed = false;
-  if (!isfinite(d)) {
+  if (!ts_finite(d)) {
     failed = true;
   } else {
     if (s.k > 0) s.crate_cur = fmax(0.3 * s.crate_cur, d / s.del_old);
@@ -212,7 +216,7 @@ __host__ __device__ inline void ts_after_newton(TsState& s, double d, double err
     return;
   }
   const double err = s.C * sqrt(errsq / n3);
-  if (!isfinite(err)) {
+  if (!ts_finite(err)) {
     s.conv_fails++;
     s.hnext = 0.25 * s.h;
     ts_reject(s);
