@@ -194,7 +194,8 @@ int nsim_psolve(nsim_handle* h, double t, const double* y, const double* fy, con
 /* Replaces lam_ts_init (src/nsimexec.ml:4113-4128 -> src/nsim.ml:2485-2536):
  * (re)initialises the device BDF integrator from the handle's m at t0 with
  * scalar tolerances (CVodeReInit semantics: history is discarded).
- * max_order <= 0 keeps 2, krylov_max <= 0 keeps 300 (main.py:838-839). */
+ * max_order <= 0 keeps 2, krylov_max <= 0 keeps 300 (main.py:838-839).
+ * max_order above 2 and krylov_max above 500 are errors (status < 0). */
 int nsim_ts_init(nsim_handle* h, double t0, double rtol, double atol, int max_order,
                  int krylov_max);
 /* Replaces lam_ts_advance (src/nsimexec.ml:4130-4146 -> src/nsim.ml:2537-2580):

@@ -1079,6 +1079,14 @@ int nsim_ts_init(nsim_handle* h, double t0, double rtol, double atol, int max_or
                  int krylov_max) {
   if (!h) return fail("nsim_ts_init: null handle");
   if (!(rtol >= 0.0) || !(atol > 0.0)) return fail("nsim_ts_init: need rtol >= 0, atol > 0");
+  // BDF orders 1 and 2 (the reference's default maximum, main.py:838) are
+  // built; Krylov bases up to 500 vectors
+  if (max_order > 2)
+    return fail("nsim_ts_init: max_order " + std::to_string(max_order) +
+                " not available (BDF orders 1 and 2 are)");
+  if (krylov_max > 500)
+    return fail("nsim_ts_init: krylov_max " + std::to_string(krylov_max) +
+                " above the largest Krylov basis (500)");
   TsOptions o = h->ts_opt;
   o.rtol = rtol;
   o.atol = atol;

@@ -81,6 +81,12 @@ class Simulation:
                  do_anisotropy_jacobian=False, temperature=None, thermal_delta_t=None,
                  timestepper_max_order=2, timestepper_krylov_max=300, ksp_tolerances={},
                  adjust_tolerances=False, precision="fp64"):
+        if not 1 <= int(timestepper_max_order) <= 2:
+            raise NmagUserError("timestepper_max_order %s: BDF orders 1 and 2 are available"
+                                % timestepper_max_order)
+        if not 1 <= int(timestepper_krylov_max) <= 500:
+            raise NmagUserError("timestepper_krylov_max %s: Krylov bases of 1 to 500 vectors "
+                                "are available" % timestepper_krylov_max)
         if phi_BEM is not None:
             raise NmagUserError("H-matrix BEM compression (phi_BEM) is not available on this path")
         if periodic_bc is not None:

@@ -65,6 +65,10 @@ def test_unsupported_options_fail_loudly():
     s = Simulation()
     with pytest.raises(NmagUserError):
         s.magnet
+    with pytest.raises(NmagUserError):
+        Simulation(timestepper_max_order=5)
+    with pytest.raises(NmagUserError):
+        Simulation(timestepper_krylov_max=600)
     assert every("step", 5).delta == 5
     with pytest.raises(ValueError):
         every("step", 0)

@@ -134,6 +134,11 @@ def test_stop_semantics_and_statistics():
     assert g.stat("ts_max_dm") > 0
     for k in ("ts_rhs", "ts_newton", "ts_linear"):
         assert g.stat(k) > 0
+    # orders above 2 and Krylov bases above 500 are refused, not clamped
+    with pytest.raises(Exception, match="max_order"):
+        g.ts_init(0.0, rtol=1e-6, atol=1e-6, max_order=3)
+    with pytest.raises(Exception, match="krylov_max"):
+        g.ts_init(0.0, rtol=1e-6, atol=1e-6, krylov_max=501)
 
 
 def test_relaxation_decreases_energy_and_converges():
