@@ -8,6 +8,7 @@
 
 #include "nsim_dev_common.h"
 #include "nsim_device.h"
+#include "nsim_tsctl.h"
 
 namespace nsim {
 
@@ -54,6 +55,10 @@ struct NewtonSystem {
   int sweeps = 0;       // local Jacobi sweeps over each workgroup's own nodes
   const double* gamma_p = nullptr;  // device word holding gamma (overrides gamma)
   const int* halt = nullptr;        // nonzero device word: the launch does nothing
+  // step controller record: when set, the launch's last act is the
+  // controller's step on this Newton iteration's results (ts_control), so no
+  // separate controller launch is needed
+  TsState* ts = nullptr;
 };
 
 // Solves (I - gamma J) x = avec + gamma f - ycur by right-preconditioned

@@ -78,6 +78,11 @@ struct GmresArgs {
   unsigned long long epoch;
   const double* gamma_p;  // if set, gamma is read here (set by the device controller)
   const int* halt;        // nonzero word: every workgroup returns at once
+  // If set, workgroup 0 runs the step controller on the results at the end.
+  // Every workgroup reads gamma and halt (fields of this record) at its start,
+  // and workgroup 0 reaches the end only after the last exchange, to which
+  // every workgroup contributes; so no read races the controller's write.
+  TsState* ts;
 };
 
 struct GShared {
@@ -277,6 +282,13 @@ __device__ void local_dots(const GmresArgs<TV>& a, GShared& sm, const double* v,
     s = wave_sum(s);
     if (lane == 0) sm.loc[i] = s;
   }
+}
+
+// The step controller as a call rather than inlined code: inlined, its
+// registers would count against the whole kernel and lower its occupancy.
+__device__ __attribute__((noinline)) void run_controller(TsState* s, double its, double xw2,
+                                                         double bad, double errsq, int n3) {
+  ts_control(s, its, xw2, bad, errsq, n3);
 }
 
 template <typename TV>
@@ -496,10 +508,12 @@ __global__ __launch_bounds__(256) void k_gmres(GmresArgs<TV> a) {
     ok = exchange(a, sm, ++ph, 2, true);
   }
   if (blockIdx.x == 0 && tid == 0) {
+    const double xw2 = ok ? sm.tot[0] : NAN, errsq = ok ? sm.tot[1] : NAN;
     a.status[0] = (double)k;
-    a.status[1] = ok ? sm.tot[0] : NAN;
+    a.status[1] = xw2;
     a.status[2] = ok ? 0.0 : 1.0;
-    a.status[3] = ok ? sm.tot[1] : NAN;
+    a.status[3] = errsq;
+    if (a.ts) run_controller(a.ts, (double)k, xw2, ok ? 0.0 : 1.0, errsq, (int)a.ld);
   }
 }
 
@@ -516,7 +530,7 @@ hipError_t launch_typed(const DeviceOps& d, const TV* K, GmresWork& w, int grid,
                   s.ewt,   s.kappa,   s.Minv,   s.gamma,  s.tol,   s.avec,    s.f,
                   s.yp,    s.x,       w.V,      w.Z,      3 * (size_t)d.n,    w.part,
                   w.tag,   w.R,       w.y,      w.status, w.err,   ++w.epoch,
-                  s.gamma_p, s.halt};
+                  s.gamma_p, s.halt, s.ts};
   void* args[] = {&a};
   if (w.cooperative)
     return hipLaunchCooperativeKernel(kernel_ptr<TV>(), dim3(grid), dim3(256), args, 0, st);

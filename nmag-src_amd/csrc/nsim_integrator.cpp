@@ -275,8 +275,9 @@ int Integrator::gmres(const double* r, double* x, int* iters) {
 }
 
 // One Newton cycle of the device-driven run: start of attempt (when the
-// controller asks for one), field evaluation at ycur, the persistent GMRES
-// (residual, preconditioner, solve, ycur += delta, norms) and the controller.
+// controller asks for one), field evaluation at ycur, and the persistent GMRES
+// (residual, preconditioner, solve, ycur += delta, norms), whose last act is
+// the controller's step.
 int Integrator::enqueue_cycle() {
   HTRY(launch_ts_begin(s_, n3_, ds_, slots_dev_, fn_, ycur_, yp_, avec_, ewt_));
   TRY(device_rhs(h_, ycur_, f_));
@@ -295,10 +296,10 @@ int Integrator::enqueue_cycle() {
   sys.halt = &ds_->halt;
   sys.tol = kLinCoef * kNewtonCoef * std::sqrt((double)n3_);
   sys.sweeps = opt_.gmres_pc_sweeps;
+  sys.ts = ds_;
   gw_.cooperative = opt_.cooperative;
   HTRY(launch_newton_gmres(handle_ops(h_), handle_mixed(h_), gw_, ggrid_, gspw_,
                            opt_.krylov_max, sys, s_));
-  HTRY(launch_ts_ctrl(s_, ds_, gw_.status, n3_));
   return 0;
 }
 

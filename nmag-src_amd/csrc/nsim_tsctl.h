@@ -55,16 +55,21 @@ struct TsState {
   int q_last;
 };
 
-// l_j'(T[0]) of the Lagrange basis polynomial of node j among T[0..m].
+// l_j'(T[0]) of the Lagrange basis polynomial of node j among T[0..m]
+// (m <= 2).  The loops have fixed bounds so that, unrolled at a call with a
+// constant j, the small arrays stay in registers.
 __host__ __device__ inline double ts_lagrange_deriv0(const double* T, int m, int j) {
   if (j == 0) {
     double s = 0.0;
-    for (int i = 1; i <= m; i++) s += 1.0 / (T[0] - T[i]);
+#pragma unroll
+    for (int i = 1; i < 3; i++)
+      if (i <= m) s += 1.0 / (T[0] - T[i]);
     return s;
   }
   double num = 1.0, den = 1.0;
-  for (int i = 0; i <= m; i++) {
-    if (i == j) continue;
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    if (i > m || i == j) continue;
     den *= T[j] - T[i];
     if (i != 0) num *= T[0] - T[i];
   }
@@ -79,6 +84,15 @@ __host__ __device__ inline double ts_factorial(int k) {
 
 // Finite (neither NaN nor +-inf), usable from host and device code alike.
 __host__ __device__ inline bool ts_finite(double v) { return fabs(v) <= DBL_MAX; }
+
+// d^(q+1) and err^(-1/(q+1)) for the orders built (q = 1, 2): products,
+// sqrt and cbrt instead of the general power function.
+__host__ __device__ inline double ts_pow_q1(double d, int q) {
+  return q >= 2 ? d * d * d : d * d;
+}
+__host__ __device__ inline double ts_root_q1(double err, int q) {
+  return q >= 2 ? 1.0 / cbrt(err) : 1.0 / sqrt(err);
+}
 
 __host__ __device__ inline bool ts_finished(const TsState& s) {
   if (s.max_steps > 0 && s.steps_call >= s.max_steps) return true;
@@ -97,10 +111,14 @@ __host__ __device__ inline void ts_prepare(TsState& s, double h, int q) {
   s.del_old = 0.0;
   const double T[3] = {s.tn + h, s.tn, s.tnm1};
   double dl[3] = {0.0, 0.0, 0.0};
-  for (int j = 0; j <= q; j++) dl[j] = ts_lagrange_deriv0(T, q, j);
+#pragma unroll
+  for (int j = 0; j < 3; j++)
+    if (j <= q) dl[j] = ts_lagrange_deriv0(T, q, j);
   s.gamma = 1.0 / dl[0];
   double R = 0.0;
-  for (int j = 1; j <= q; j++) R += dl[j] * pow(T[j] - T[0], q + 1) / ts_factorial(q + 1);
+#pragma unroll
+  for (int j = 1; j < 3; j++)
+    if (j <= q) R += dl[j] * ts_pow_q1(T[j] - T[0], q) / ts_factorial(q + 1);
   const double E_c = -R / dl[0];
   double E_p;
   if (s.nhist == 0) {  // explicit Euler from f(y_n)
@@ -114,10 +132,13 @@ __host__ __device__ inline void ts_prepare(TsState& s, double h, int q) {
     const double t1 = s.tn + h;
     const double P[3] = {s.tn, s.tnm1, s.tnm2};
     double L[3] = {0.0, 0.0, 0.0}, ep = 1.0;
-    for (int j = 0; j <= q; j++) {
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      if (j > q) continue;
       double w = 1.0;
-      for (int i = 0; i <= q; i++)
-        if (i != j) w *= (t1 - P[i]) / (P[j] - P[i]);
+#pragma unroll
+      for (int i = 0; i < 3; i++)
+        if (i <= q && i != j) w *= (t1 - P[i]) / (P[j] - P[i]);
       L[j] = w;
       ep *= t1 - P[j];
     }
@@ -167,7 +188,7 @@ __host__ __device__ inline void ts_accept(TsState& s, double err) {
   s.h_last = s.h;
   s.q_last = s.q;
   const double etamax = s.steps <= 10 ? 100.0 : 10.0;
-  double eta = err > 0.0 ? 0.9 * pow(err, -1.0 / (s.q + 1)) : etamax;
+  double eta = err > 0.0 ? 0.9 * ts_root_q1(err, s.q) : etamax;
   eta = fmin(etamax, fmax(0.2, eta));
   if (eta > 1.0 && eta < 1.5) eta = 1.0;
   s.hnext = s.h * eta;
@@ -224,11 +245,29 @@ __host__ __device__ inline void ts_after_newton(TsState& s, double d, double err
   }
   if (err > 1.0) {
     s.err_fails++;
-    s.hnext = s.h * fmax(0.2, 0.9 * pow(err, -1.0 / (s.q + 1)));
+    s.hnext = s.h * fmax(0.2, 0.9 * ts_root_q1(err, s.q));
     ts_reject(s);
     return;
   }
   ts_accept(s, err);
+}
+
+// The controller's step after a Newton iteration whose results are its
+// Krylov iterations, the correction's weighted sum of squares xw2, an error
+// flag and errsq (as ts_after_newton).  One thread, on a register copy of the
+// state written back whole.  Nothing to do once the run has stopped.
+__host__ __device__ inline void ts_control(TsState* s, double its, double xw2, double bad,
+                                           double errsq, int n3) {
+  TsState st = *s;
+  if (st.mode == kTsDone || st.mode == kTsError || st.halt) return;
+  if (bad != 0.0) {
+    st.mode = kTsError;
+    st.err_code = 1;
+    st.halt = 1;
+  } else {
+    ts_after_newton(st, sqrt(xw2 / n3), errsq, (int)its, n3);
+  }
+  *s = st;
 }
 
 // Launchers (nsim_tsctl.hip).

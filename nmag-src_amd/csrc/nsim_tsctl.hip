@@ -50,17 +50,7 @@ __global__ __launch_bounds__(256) void k_ts_begin(int n3, const TsState* __restr
 // whole, so the loads are in flight together rather than one field at a time).
 __global__ void k_ts_ctrl(TsState* s, const double* __restrict__ status, int n3) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  TsState st = *s;
-  if (st.mode == kTsDone || st.mode == kTsError || st.halt) return;
-  const double its = status[0], xw2 = status[1], bad = status[2], errsq = status[3];
-  if (bad != 0.0) {
-    st.mode = kTsError;
-    st.err_code = 1;
-    st.halt = 1;
-  } else {
-    ts_after_newton(st, sqrt(xw2 / n3), errsq, (int)its, n3);
-  }
-  *s = st;
+  ts_control(s, status[0], status[1], status[2], status[3], n3);
 }
 
 }  // namespace
