@@ -270,7 +270,7 @@ def run_batch(steps, warmup, rtol, window_ps, torch):
             mag.set_field(key, f[key])
 
     # batched field evaluations
-    for prec in ("fp64", "mixed"):
+    for prec in ("fp64", "mixed", "mixed24"):
         t0 = time.perf_counter()
         b = graphs(NativeBatch(meshes, mats, precision=prec, nbc_rtol=rtol, dbc_rtol=rtol))
         t_setup = time.perf_counter() - t0
@@ -297,6 +297,10 @@ def run_batch(steps, warmup, rtol, window_ps, torch):
         log("batched %s: %.1f batch evals/s (setup %.1f s)" % (prec, steps / dt, t_setup))
         if prec == "fp64":
             ref_out = YD[warmup:].cpu().numpy()
+        else:
+            got = YD[warmup:].cpu().numpy()
+            out["batched_" + prec]["dmdt_max_rel_err_vs_fp64"] = float(
+                np.max(np.abs(got - ref_out)) / np.max(np.abs(ref_out)))
         del Y, YD
         b.close()
 
@@ -331,9 +335,10 @@ def run_batch(steps, warmup, rtol, window_ps, torch):
                               "setup_s": round(t_setup, 2)}
     out["batched_vs_sequential_fp64"] = round(
         out["batched_fp64"]["batch_evals_per_s"] / out["sequential_fp64"]["batch_evals_per_s"], 3)
-    out["batched_mixed_vs_sequential_fp64"] = round(
-        out["batched_mixed"]["batch_evals_per_s"] /
-        out["sequential_fp64"]["batch_evals_per_s"], 3)
+    for prec in ("mixed", "mixed24"):
+        out["batched_%s_vs_sequential_fp64" % prec] = round(
+            out["batched_" + prec]["batch_evals_per_s"] /
+            out["sequential_fp64"]["batch_evals_per_s"], 3)
     out["batched_vs_sequential_max_rel_diff"] = float(
         np.max(np.abs(ref_out - seq)) / np.max(np.abs(seq)))
     log("sequential fp64: %.1f batch evals/s" % (steps / dt))

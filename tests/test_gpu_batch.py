@@ -123,18 +123,22 @@ def test_batch_jtimes_and_preconditioner_match_single_magnets():
         assert rel_err(z[k], zr) < 1e-9, k
 
 
-def test_batch_mixed_precision_error_bounds():
+@pytest.mark.parametrize("prec,tol_demag,tol_dmdt", [("mixed", 1e-6, 1e-6),
+                                                     ("mixed24", 5e-5, 1e-5)])
+def test_batch_mixed_precision_error_bounds(prec, tol_demag, tol_dmdt):
+    """FP32 (24-bit potential operator) storage in a batch of uneven magnets:
+    per-magnet bounds from the storage rounding, 2^-24 (2^-16) per entry."""
     meshes, mats, fields = _batch_inputs()
     out = {}
-    for prec in ("fp64", "mixed"):
-        b = _batch(meshes, mats, prec, rtol=1e-12)
+    for p in ("fp64", prec):
+        b = _batch(meshes, mats, p, rtol=1e-12)
         _set_all(b, fields)
         b.execute("update_dmdt")
-        out[prec] = {k: b.split(b.get_field(k)) for k in ("H_exch", "H_demag", "dmdt")}
+        out[p] = {k: b.split(b.get_field(k)) for k in ("H_exch", "H_demag", "dmdt")}
     for k in range(len(MAGNETS)):
-        assert rel_err(out["mixed"]["H_exch"][k], out["fp64"]["H_exch"][k]) < 1e-6, k
-        assert rel_err(out["mixed"]["H_demag"][k], out["fp64"]["H_demag"][k]) < 1e-6, k
-        assert rel_err(out["mixed"]["dmdt"][k], out["fp64"]["dmdt"][k]) < 1e-6, k
+        assert rel_err(out[prec]["H_exch"][k], out["fp64"]["H_exch"][k]) < 1e-6, k
+        assert rel_err(out[prec]["H_demag"][k], out["fp64"]["H_demag"][k]) < tol_demag, k
+        assert rel_err(out[prec]["dmdt"][k], out["fp64"]["dmdt"][k]) < tol_dmdt, k
 
 
 def test_batch_integration_matches_single_magnets():
