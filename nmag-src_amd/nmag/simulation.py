@@ -346,6 +346,8 @@ class Simulation:
             stage = self.clock.stage if self._stages_run == 0 else self.clock.stage + 1
         out = []
         c = self.clock
+        # a clock restored from a file saved by an exiting loop carries the flag
+        c.exit_hysteresis = False
         for H in H_ext_list[skip:]:
             self.do_next_stage(stage=stage)
             stage = None

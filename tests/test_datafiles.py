@@ -236,6 +236,35 @@ def test_do_next_stage_and_exit(standin):
         sim.relax(save=[("nonsense", at("stage_end"))])
 
 
+def test_hysteresis_restart_continues_with_the_next_stage(standin, monkeypatch):
+    """A loop that saved a restart file at the end of each stage and was cut
+    after stage 2 continues at stage 3 under --restart (hysteresis.py:432-451,
+    main.py:2876-2885): the stages already done are not run again."""
+    H = [[SI(v, "A/m"), 0, 0] for v in (1e4, 2e4, 3e4, 4e4)]
+    sim = _sim()
+    seen = []
+
+    def cut_after_two(s):
+        seen.append(s.clock.stage)
+        if s.clock.stage == 2:
+            Simulation.hysteresis_exit(s)
+
+    sim.hysteresis(H, save=[("restart", at("stage_end"))],
+                   do=[(cut_after_two, at("stage_end"))])
+    assert seen == [1, 2]
+    steps_done = sim.clock.step
+    monkeypatch.setattr(simmod.sys, "argv", ["run.py", "--restart"])
+    sim2 = _sim()
+    seen2 = []
+    avgs = sim2.hysteresis(H, save=[(lambda s: seen2.append((s.clock.stage,
+                                                              s.get_subfield_average_siv(
+                                                                  "H_ext")[0])),
+                                     at("stage_end"))])
+    assert [st for st, _ in seen2] == [3, 4]
+    assert [h for _, h in seen2] == pytest.approx([3e4, 4e4])
+    assert len(avgs) == 2 and sim2.clock.step > steps_done
+
+
 def test_restart_round_trip(standin, monkeypatch):
     sim = _sim()
     sim.advance_time(SI(3e-12, "s"))
