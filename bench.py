@@ -699,14 +699,30 @@ def main():
 
     cpu = None
     if not args.no_cpu:
+        import oracle
         log("cpu baseline (oracle, %.0f s budget) ..." % args.cpu_budget)
+        threads = oracle.set_threads(int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))
         rate, done, setup, _ = run_cpu(coords, tets, mat, Hext, states[args.warmup:],
                                        args.rtol, args.cpu_budget)
-        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        # the same on one core, a smaller bounded sample
+        oracle.set_threads(1)
+        rate1, done1, _, _ = run_cpu(coords, tets, mat, Hext, states[args.warmup:],
+                                     args.rtol, args.cpu_budget / 2)
+        oracle.set_threads(threads)
+        model = None
+        try:
+            with open("/proc/cpuinfo") as f:
+                model = next((ln.split(":", 1)[1].strip() for ln in f
+                              if ln.startswith("model name")), None)
+        except OSError:
+            pass
         cpu = {"value": round(rate, 3), "unit": "field evals/s", "cores": threads,
                "kind": "port",
                "sample": "%d FP64 field evals of the same trajectory states, warm-started "
-                         "Jacobi-PCG rtol %g (setup %.1f s excluded)" % (done, args.rtol, setup)}
+                         "Jacobi-PCG rtol %g (setup %.1f s excluded)" % (done, args.rtol, setup),
+               "one_core": {"value": round(rate1, 3), "unit": "field evals/s",
+                            "sample": "%d evals of the same states" % done1},
+               "cpu_model": model}
 
     line = {
         "metric": "LLG field evals/s & sim-ns/hour, 20k-node mesh, mixed vs FP64 at "

@@ -38,6 +38,8 @@ def lib():
         L.orc_setup.restype = ctypes.c_void_p
         L.orc_setup.argtypes = [ctypes.c_int, _dp, ctypes.c_int, _ip, ctypes.c_int]
         L.orc_free.argtypes = [ctypes.c_void_p]
+        L.orc_set_threads.argtypes = [ctypes.c_int]
+        L.orc_set_threads.restype = ctypes.c_int
         L.orc_sizes.argtypes = [ctypes.c_void_p, _ip]
         L.orc_get_volumes.argtypes = [ctypes.c_void_p, _dp]
         L.orc_get_stiffness.argtypes = [ctypes.c_void_p, _ip, _ip, _dp]
@@ -66,6 +68,12 @@ def _p(a, dtype=np.float64):
         return None
     assert a.dtype == dtype and a.flags.c_contiguous
     return a.ctypes.data_as(_dp if dtype == np.float64 else _ip)
+
+
+def set_threads(n):
+    """OpenMP threads of the oracle from now on (n <= 0: all processors);
+    returns the count in effect."""
+    return int(lib().orc_set_threads(int(n)))
 
 
 def kuhn_cuboid(nx, ny, nz, h):

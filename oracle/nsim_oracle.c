@@ -384,6 +384,13 @@ void orc_free(orc_mesh* m) {
   free(m);
 }
 
+/* OpenMP threads of the parallel loops (n <= 0: all processors); returns
+ * the count in effect, so a CPU-baseline timing can state its cores. */
+int orc_set_threads(int n) {
+  omp_set_num_threads(n > 0 ? n : omp_get_num_procs());
+  return omp_get_max_threads();
+}
+
 void orc_sizes(const orc_mesh* m, int* s) {
   s[0] = m->n; s[1] = m->rp[m->n]; s[2] = m->nb; s[3] = m->ntri; s[4] = m->T;
 }

@@ -50,6 +50,9 @@ int orc_kuhn_cuboid(int nx, int ny, int nz, double h, double* coords, int* tets)
 orc_mesh* orc_setup(int n_nodes, const double* coords, int n_tets, const int* tets,
                     int build_bem);
 void orc_free(orc_mesh* m);
+/* OpenMP threads used from now on (n <= 0: all processors); returns the
+ * count in effect. */
+int orc_set_threads(int n);
 
 /* sizes[0..4] = n_nodes, nnz(K), n_boundary_nodes, n_boundary_triangles, n_tets */
 void orc_sizes(const orc_mesh* m, int* sizes);
