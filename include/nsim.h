@@ -47,7 +47,8 @@ enum {
   NSIM_MIXED_F24 = 3   /* as MIXED_FP32 but the dense potential operator
                         * stored in 24 bits: FP32 rounded to 15 stored
                         * significand bits (unit roundoff 2^-16), 3/4 of
-                        * FP32's bytes                                      */
+                        * FP32's bytes.  No counterpart in the reference
+                        * (whose path is FP64 throughout).                  */
 };
 
 /* Solver options; the KSP tolerances mirror Simulation(ksp_tolerances=...)
