@@ -29,12 +29,13 @@ hipError_t launch_adj_exch_div(const DeviceOps& d, bool mixed, const double* x, 
 // interior solve of psi).
 hipError_t launch_adj_boundary(const DeviceOps& d, const double* psi, const double* z,
                                double* e, hipStream_t st);
-// chi[map[j]] += (A^T e)_j for a single-segment DenseOp (the BEM: e on the
-// boundary, map = boundary nodes; the potential operator: e and chi on all
-// nodes); part: scratch of dense_t_chunks(op) * ld doubles.
+// chi[map[j]] += (A^T e)_j (accumulate) or = (A^T e)_j for a single-segment
+// DenseOp (the BEM: e on the boundary, map = boundary nodes; the potential
+// operator: e and chi on all nodes); part: scratch of dense_t_chunks(op) * ld
+// doubles.
 int dense_t_chunks(const DenseOp& op);
 hipError_t launch_dense_gemv_t(const DenseOp& op, const double* e, double* part, double* chi,
-                               hipStream_t st);
+                               bool accumulate, hipStream_t st);
 // x -= mean(x) (one workgroup)
 hipError_t launch_mean_project(int n, double* x, hipStream_t st);
 // a -= Ms V_i (G phi)_i (the adjoint of the divergence load: D^T = -V G)

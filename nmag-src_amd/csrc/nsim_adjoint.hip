@@ -216,12 +216,13 @@ __global__ __launch_bounds__(256) void k_bem_gemv_t(int nb, int ldb, int rows_pe
 __global__ __launch_bounds__(256) void k_bem_t_reduce(int nb, int ldb, int chunks,
                                                       const double* __restrict__ part,
                                                       const int* __restrict__ bnode,
-                                                      double* __restrict__ chi) {
+                                                      double* __restrict__ chi, int accumulate) {
   const int j = blockIdx.x * 256 + threadIdx.x;
   if (j >= nb) return;
   double s = 0.0;
   for (int c = 0; c < chunks; c++) s += part[(size_t)c * ldb + j];
-  chi[bnode[j]] += s;
+  if (accumulate) s += chi[bnode[j]];
+  chi[bnode[j]] = s;
 }
 
 __global__ __launch_bounds__(1024) void k_mean_project(int n, double* __restrict__ x) {
@@ -268,14 +269,15 @@ __global__ __launch_bounds__(256) void k_adj_grad(int n, const int* __restrict__
 
 template <typename TB>
 hipError_t gemv_t_typed(const DenseOp& op, const double* e, double* part, double* chi,
-                        hipStream_t st) {
+                        bool accumulate, hipStream_t st) {
   constexpr int V = TVec<TB>::V;
   const int ldb = op.h_ld[0], nb = op.h_n[0];
   const int chunks = dense_t_chunks(op);
   const int rpc = (nb + chunks - 1) / chunks;
   const int cols = (ldb / V + 255) / 256;
   k_bem_gemv_t<TB><<<dim3(cols, chunks), 256, 0, st>>>(nb, ldb, rpc, (const TB*)op.A, e, part);
-  k_bem_t_reduce<<<blocks_for(nb), 256, 0, st>>>(nb, ldb, chunks, part, op.map, chi);
+  k_bem_t_reduce<<<blocks_for(nb), 256, 0, st>>>(nb, ldb, chunks, part, op.map, chi,
+                                                 accumulate ? 1 : 0);
   return hipGetLastError();
 }
 
@@ -319,13 +321,13 @@ int dense_t_chunks(const DenseOp& op) {
 }
 
 hipError_t launch_dense_gemv_t(const DenseOp& op, const double* e, double* part, double* chi,
-                               hipStream_t st) {
+                               bool accumulate, hipStream_t st) {
   if (!op.A || op.h_n.empty() || op.h_n[0] == 0) return hipSuccess;
   if (op.h_n.size() != 1) return hipErrorInvalidConfiguration;
-  if (op.type == kBemF64) return gemv_t_typed<double>(op, e, part, chi, st);
-  if (op.type == kBemF32) return gemv_t_typed<float>(op, e, part, chi, st);
-  if (op.type == kBemF24) return gemv_t_typed<F24>(op, e, part, chi, st);
-  return gemv_t_typed<uint16_t>(op, e, part, chi, st);
+  if (op.type == kBemF64) return gemv_t_typed<double>(op, e, part, chi, accumulate, st);
+  if (op.type == kBemF32) return gemv_t_typed<float>(op, e, part, chi, accumulate, st);
+  if (op.type == kBemF24) return gemv_t_typed<F24>(op, e, part, chi, accumulate, st);
+  return gemv_t_typed<uint16_t>(op, e, part, chi, accumulate, st);
 }
 
 hipError_t launch_mean_project(int n, double* x, hipStream_t st) {

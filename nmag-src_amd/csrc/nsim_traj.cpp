@@ -121,9 +121,9 @@ int vjp(nsim_handle* h, const double* y, const double* lam, double* out, const A
   HIPCHK(launch_adj_local(d, field_args_of(h, y), b, lam, A->g, A->x, out, acc, st));
   HIPCHK(launch_adj_exch_div(d, h->mixed, A->x, out, A->psi, st));
   if (demag && d.pot.A) {
-    // dense route phi = M rho: the adjoint potential is M^T psi
-    HIPCHK(hipMemsetAsync(A->phia, 0, (size_t)d.n * sizeof(double), st));
-    HIPCHK(launch_dense_gemv_t(d.pot, A->psi, A->part, A->phia, st));
+    // dense route phi = M rho: the adjoint potential is M^T psi (written
+    // whole: the operator's map covers every node)
+    HIPCHK(launch_dense_gemv_t(d.pot, A->psi, A->part, A->phia, false, st));
     HIPCHK(launch_adj_grad(d, h->mixed, A->phia, out, st));
   } else if (demag) {
     CgWork& w = h->cgw;
@@ -135,7 +135,7 @@ int vjp(nsim_handle* h, const double* y, const double* lam, double* out, const A
     HIPCHK(launch_adj_boundary(d, A->psi, A->z, A->e, st));
     // chi = psi + R^T B^T e, then K^+ chi (the pseudo-inverse projects out the
     // constants first)
-    HIPCHK(launch_dense_gemv_t(d.bem, A->e, A->part, A->psi, st));
+    HIPCHK(launch_dense_gemv_t(d.bem, A->e, A->part, A->psi, true, st));
     HIPCHK(launch_mean_project(d.n, A->psi, st));
     w.sync = h->sync_nbc;
     HIPCHK(launch_cg(h->cg, w, false, h->cg_coarse, A->psi, A->phia, h->opt.nbc_rtol,
@@ -362,22 +362,17 @@ int nsim_traj_adjoint(nsim_handle* h, const nsim_traj* tr, const double* amps,
       AdjAccum acc{A->k1bar, A->msbar, w >= 0 ? A->hbar + (size_t)w * n : nullptr,
                    {tr->pulse_dir[0], tr->pulse_dir[1], tr->pulse_dir[2]}};
       const double* Y = A->stages + (size_t)(s - s0) * 4 * n3;
-      HIPCHK(vec_lincomb(st, n3, dt / 6.0, A->ybar, 0, nullptr, 0, nullptr, 0, nullptr, kb[0]));
-      HIPCHK(vec_lincomb(st, n3, dt / 3.0, A->ybar, 0, nullptr, 0, nullptr, 0, nullptr, kb[1]));
-      HIPCHK(vec_lincomb(st, n3, dt / 3.0, A->ybar, 0, nullptr, 0, nullptr, 0, nullptr, kb[2]));
-      HIPCHK(vec_lincomb(st, n3, dt / 6.0, A->ybar, 0, nullptr, 0, nullptr, 0, nullptr, kb[3]));
+      HIPCHK(vec_scale4(st, n3, A->ybar, dt / 6.0, kb[0], dt / 3.0, kb[1], dt / 3.0, kb[2],
+                        dt / 6.0, kb[3]));
       // stage 4: Y4 = y + dt k3
       TRY(vjp(h, Y + 3 * (size_t)n3, kb[3], A->out, acc));
-      HIPCHK(vec_lincomb(st, n3, 1.0, A->ybar, 1.0, A->out, 0, nullptr, 0, nullptr, A->ybar));
-      HIPCHK(vec_lincomb(st, n3, 1.0, kb[2], dt, A->out, 0, nullptr, 0, nullptr, kb[2]));
+      HIPCHK(vec_update2(st, n3, A->out, 1.0, A->ybar, dt, kb[2]));
       // stage 3: Y3 = y + dt/2 k2
       TRY(vjp(h, Y + 2 * (size_t)n3, kb[2], A->out, acc));
-      HIPCHK(vec_lincomb(st, n3, 1.0, A->ybar, 1.0, A->out, 0, nullptr, 0, nullptr, A->ybar));
-      HIPCHK(vec_lincomb(st, n3, 1.0, kb[1], 0.5 * dt, A->out, 0, nullptr, 0, nullptr, kb[1]));
+      HIPCHK(vec_update2(st, n3, A->out, 1.0, A->ybar, 0.5 * dt, kb[1]));
       // stage 2: Y2 = y + dt/2 k1
       TRY(vjp(h, Y + (size_t)n3, kb[1], A->out, acc));
-      HIPCHK(vec_lincomb(st, n3, 1.0, A->ybar, 1.0, A->out, 0, nullptr, 0, nullptr, A->ybar));
-      HIPCHK(vec_lincomb(st, n3, 1.0, kb[0], 0.5 * dt, A->out, 0, nullptr, 0, nullptr, kb[0]));
+      HIPCHK(vec_update2(st, n3, A->out, 1.0, A->ybar, 0.5 * dt, kb[0]));
       // stage 1: Y1 = y
       TRY(vjp(h, Y, kb[0], A->out, acc));
       HIPCHK(vec_lincomb(st, n3, 1.0, A->ybar, 1.0, A->out, 0, nullptr, 0, nullptr, A->ybar));

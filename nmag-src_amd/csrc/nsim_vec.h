@@ -16,12 +16,12 @@ hipError_t vec_lincomb(hipStream_t st, int n, double c0, const double* x0, doubl
                        const double* x3, double* y);
 hipError_t vec_mul(hipStream_t st, int n, const double* x, const double* w, bool inverse,
                    double* y);
-// ewt = 1/(rtol |yn| + atol); yp = ycur = L0 yn + L1 p1 + L2 p2; avec = a1 yn + a2 a2v
-// (null vectors drop their terms).
-hipError_t vec_step_begin(hipStream_t st, int n, const double* yn, double L0, const double* p1,
-                          double L1, const double* p2, double L2, double a1, const double* a2v,
-                          double a2, double rtol, double atol, double* ewt, double* yp,
-                          double* ycur, double* avec);
+// y_k = c_k x, k = 0..3, in one launch.
+hipError_t vec_scale4(hipStream_t st, int n, const double* x, double c0, double* y0, double c1,
+                      double* y1, double c2, double* y2, double c3, double* y3);
+// y1 += a1 x; y2 += a2 x, in one launch.
+hipError_t vec_update2(hipStream_t st, int n, const double* x, double a1, double* y1, double a2,
+                       double* y2);
 hipError_t vec_ewt(hipStream_t st, int n, const double* y, double rtol, double atol,
                    double* ewt);
 hipError_t vec_wsq_partials(hipStream_t st, int n, const double* x, const double* w,

@@ -35,26 +35,26 @@ __global__ __launch_bounds__(256) void k_lincomb(int n, double c0, const double*
   }
 }
 
-// Start of a BDF step attempt in one pass: ewt = 1/(rtol |yn| + atol), the
-// predictor yp = L0 yn + L1 p1 + L2 p2, ycur = yp, and avec = a1 yn + a2 a2v
-// (null p1, p2, a2v drop their terms).
-__global__ __launch_bounds__(256) void k_step_begin(int n, const double* yn, double L0,
-                                                    const double* p1, double L1,
-                                                    const double* p2, double L2, double a1,
-                                                    const double* a2v, double a2, double rtol,
-                                                    double atol, double* ewt, double* yp,
-                                                    double* ycur, double* avec) {
+// y_k = c_k x for four outputs in one pass.
+__global__ __launch_bounds__(256) void k_scale4(int n, const double* x, double c0, double* y0,
+                                                double c1, double* y1, double c2, double* y2,
+                                                double c3, double* y3) {
   for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
-    const double y = yn[i];
-    ewt[i] = 1.0 / (rtol * fabs(y) + atol);
-    double v = L0 * y;
-    if (p1) v += L1 * p1[i];
-    if (p2) v += L2 * p2[i];
-    yp[i] = v;
-    ycur[i] = v;
-    double a = a1 * y;
-    if (a2v) a += a2 * a2v[i];
-    avec[i] = a;
+    const double v = x[i];
+    y0[i] = c0 * v;
+    y1[i] = c1 * v;
+    y2[i] = c2 * v;
+    y3[i] = c3 * v;
+  }
+}
+
+// y1 += a1 x and y2 += a2 x in one pass.
+__global__ __launch_bounds__(256) void k_update2(int n, const double* x, double a1, double* y1,
+                                                 double a2, double* y2) {
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+    const double v = x[i];
+    y1[i] += a1 * v;
+    y2[i] += a2 * v;
   }
 }
 
@@ -193,12 +193,15 @@ hipError_t vec_lincomb(hipStream_t st, int n, double c0, const double* x0, doubl
   return hipGetLastError();
 }
 
-hipError_t vec_step_begin(hipStream_t st, int n, const double* yn, double L0, const double* p1,
-                          double L1, const double* p2, double L2, double a1, const double* a2v,
-                          double a2, double rtol, double atol, double* ewt, double* yp,
-                          double* ycur, double* avec) {
-  k_step_begin<<<grid_for(n), 256, 0, st>>>(n, yn, L0, p1, L1, p2, L2, a1, a2v, a2, rtol, atol,
-                                            ewt, yp, ycur, avec);
+hipError_t vec_scale4(hipStream_t st, int n, const double* x, double c0, double* y0, double c1,
+                      double* y1, double c2, double* y2, double c3, double* y3) {
+  k_scale4<<<grid_for(n), 256, 0, st>>>(n, x, c0, y0, c1, y1, c2, y2, c3, y3);
+  return hipGetLastError();
+}
+
+hipError_t vec_update2(hipStream_t st, int n, const double* x, double a1, double* y1, double a2,
+                       double* y2) {
+  k_update2<<<grid_for(n), 256, 0, st>>>(n, x, a1, y1, a2, y2);
   return hipGetLastError();
 }
 
