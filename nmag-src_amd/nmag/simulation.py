@@ -121,7 +121,9 @@ class Simulation:
         self._ts_initialised = False
         self._stages_run = 0
         self._saved_steps = set()
-        self.max_time_reached = SI(1e-6, "s")
+        # the stand-in for "no time limit" of relax/hysteresis
+        # (simulation_core.py:170-180)
+        self.max_time_reached = SI(1.0, "s")
         # relax/hysteresis 'save' and 'do' names (simulation_core.py:185-197)
         self.action_abbreviations = {}
         self.add_save_abbrev("save_averages", lambda s: s.save_data(avoid_same_step=True))
