@@ -45,6 +45,9 @@ constexpr int kSplit = 4;                  // wavefronts per slice in the J.z ga
 constexpr unsigned kSpinLimit = 1u << 20;  // polls before an exchange gives up
 constexpr int kMaxGrid = 1024;
 constexpr int kPcNodes = 256;              // largest node block of the local sweeps
+// Krylov iterations whose R every workgroup keeps in LDS (what fits beside
+// the rest while three workgroups share a CU's 160 KiB)
+constexpr int kRtCap = 15;
 
 template <typename TV>
 struct GmresArgs {
@@ -96,6 +99,10 @@ struct GShared {
   double pz[2][3 * kPcNodes];  //                 sweep iterates
   double red[8];
   double hn2, vn2;  // |v|^2 from |v'|^2 - |h2|^2, and |v'|^2
+  // the first kRtCap columns of R, upper triangle packed column by column
+  // (R[i][j] at j (j + 1) / 2 + i): every workgroup holds the same R, so up
+  // to kRtCap iterations each solves R y = g itself, with no broadcast
+  double Rt[kRtCap * (kRtCap + 1) / 2];
 };
 
 // z = M^-1 t on one node (3x3 block).
@@ -446,6 +453,8 @@ __global__ __launch_bounds__(256) void k_gmres(GmresArgs<TV> a) {
       c[j] = rr;
       sm.g[j + 1] = -sm.sn[j] * sm.g[j];
       sm.g[j] = sm.cs[j] * sm.g[j];
+      if (j < kRtCap)
+        for (int i = 0; i <= j; i++) sm.Rt[j * (j + 1) / 2 + i] = c[i];
       if (blockIdx.x == 0)
         for (int i = 0; i <= j; i++) a.R[(size_t)i * a.maxl + j] = c[i];
     }
@@ -453,8 +462,20 @@ __global__ __launch_bounds__(256) void k_gmres(GmresArgs<TV> a) {
     k = j + 1;
     if (fabs(sm.g[j + 1]) <= tol || !(hn > 0.0) || !isfinite(hn)) break;
   }
-  // y = R^-1 g (workgroup 0), broadcast into sm.tot as y_i zs_i
-  if (ok && k > 0) {
+  // y = R^-1 g into sm.tot as y_i zs_i: in every workgroup from its own copy
+  // of R when k <= kRtCap (the same k everywhere, so all take this branch),
+  // else by workgroup 0 and broadcast
+  if (ok && k > 0 && k <= kRtCap) {
+    if (tid == 0)
+      for (int i = k - 1; i >= 0; i--) {
+        double s = sm.g[i];
+        for (int c = i + 1; c < k; c++) s -= sm.Rt[c * (c + 1) / 2 + i] * sm.tot[c];
+        sm.tot[i] = s / sm.Rt[i * (i + 1) / 2 + i];
+      }
+    __syncthreads();
+    for (int i = tid; i < k; i += 256) sm.tot[i] *= sm.zs[i];
+    __syncthreads();
+  } else if (ok && k > 0) {
     ++ph;
     if (blockIdx.x == 0) {
       if (tid == 0) {
