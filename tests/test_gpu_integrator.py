@@ -190,6 +190,33 @@ def test_psetup_psolve_blocks_match_the_jacobian_diagonal():
         g.psolve(r.reshape(-1), gamma, lr=2)
 
 
+@pytest.mark.parametrize("precision", ["mixed", "mixed24"])
+def test_reduced_storage_trajectory_close_to_fp64(precision):
+    """<m>(t) with the potential operator in FP32 (24 bits) against FP64 over
+    50 ps of precession and relaxation at the working tolerance 1e-6: within
+    the 1e-5 of the precision target."""
+    mat = material(alpha=0.1)
+    coords, tets = kuhn_cuboid(10, 6, 3, 2.5)
+    n = len(coords)
+    m0 = random_m(n, 12) * 0.3 + np.array([1.0, 0.5, 0.0])
+    m0 /= np.linalg.norm(m0, axis=1, keepdims=True)
+    avg = {}
+    for p in ("fp64", precision):
+        g = NativeMagnet(coords, tets, mat.su_coeffs(), precision=p, potential="dense",
+                         nbc_rtol=1e-10, dbc_rtol=1e-10)
+        g.set_field("H_ext", np.tile([0.0, 0.02, 0.01], (n, 1)))
+        g.set_field("m", m0)
+        g.ts_init(0.0, rtol=1e-6, atol=1e-6)
+        V = g.get_field("volumes")
+        series = []
+        for t in np.linspace(5.0, 50.0, 10):
+            assert g.advance(t) == pytest.approx(t)
+            series.append((g.get_field("m") * V[:, None]).sum(0) / V.sum())
+        avg[p] = np.array(series)
+        g.close()
+    assert np.abs(avg[precision] - avg["fp64"]).max() < 1e-5
+
+
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_persistent_gmres_matches_host_driven_gmres(precision):
     """The one-launch GMRES and the host-driven Arnoldi loop solve the same
