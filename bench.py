@@ -17,7 +17,6 @@ Prints ONE JSON line.  Progress goes to stderr.
 """
 import argparse
 import json
-import math
 import os
 import sys
 import time
@@ -36,29 +35,9 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def s_state(coords, seed, noise_deg=1.0):
-    """In-plane S-state phi(u) = (pi/4) u^3, u = (x - xc)/(Lx/2), plus seeded
-    isotropic noise of noise_deg degrees, renormalised."""
-    x = coords[:, 0]
-    u = (x - 0.5 * (x.min() + x.max())) / (0.5 * (x.max() - x.min()))
-    phi = 0.25 * math.pi * u ** 3
-    m = np.stack([np.cos(phi), np.sin(phi), np.zeros_like(phi)], axis=1)
-    rng = np.random.default_rng(seed)
-    m += math.radians(noise_deg) * rng.standard_normal(m.shape)
-    return m / np.linalg.norm(m, axis=1, keepdims=True)
-
-
 def make_material(alpha):
-    from nmag import MagMaterial, SI
-    return MagMaterial("Py", Ms=SI(8.6e5, "A/m"), exchange_coupling=SI(1.3e-11, "J/m"),
-                       llg_gamma_G=SI(2.211e5, "m/A s"), llg_damping=alpha)
-
-
-def applied_field(n, mT=25.0, deg=170.0):
-    from nmag.units import to_su, SI, mu0
-    H = to_su(SI(mT * 1e-3, "T") / mu0, "A/m")
-    return np.tile([H * math.cos(math.radians(deg)), H * math.sin(math.radians(deg)), 0.0],
-                   (n, 1))
+    from nmag.workloads import permalloy
+    return permalloy(alpha)
 
 
 def trajectory_states(magnet, m0, count, dt):
@@ -176,11 +155,14 @@ def run_dynamics(prec, route, coords, tets, mat, Hext, m0, window_ps, checkpoint
     """advance_time over window_ps from the S-state (BDF, rtol = atol = 1e-6);
     returns wall time, integrator statistics and <m>(t) at the checkpoints."""
     from nmag._native import NativeMagnet
+    t_set = time.perf_counter()
     g = graphs(NativeMagnet(coords, tets, mat.su_coeffs(), precision=prec, nbc_rtol=ksp_rtol,
                             dbc_rtol=ksp_rtol, potential=route))
     g.set_field("H_ext", Hext)
     g.set_field("m", m0)
     g.ts_init(0.0, 1e-6, 1e-6)
+    g.synchronize()
+    setup_s = time.perf_counter() - t_set
     V = g.get_field("volumes")
     avgs, t = [], 0.0
     t0 = time.perf_counter()
@@ -194,6 +176,7 @@ def run_dynamics(prec, route, coords, tets, mat, Hext, m0, window_ps, checkpoint
     wall = time.perf_counter() - t0
     stats = {k: g.stat("ts_" + k) for k in ("steps", "rhs", "newton", "linear", "err_fails",
                                             "conv_fails")}
+    stats["setup_s"] = setup_s
     g.close()
     return wall, stats, np.array(avgs)
 
@@ -220,43 +203,39 @@ def run_cpu(coords, tets, mat, Hext, states, rtol, budget_s):
     return done / t_used, done, setup, outs
 
 
-def batch_workload(count=64, seed=4, h=2.5):
-    """64 permalloy cuboids: sides U[20,120] x U[20,80] x U[5,15] nm rounded to
-    2.5 nm cubes, Ms U[6e5,1e6] A/m, uniaxial K U[0,5e4] J/m^3 along a random
-    axis, applied field a random unit vector times U[0,50] mT, alpha 0.5; m0
-    normalised i.i.d. Gaussian per node."""
-    from nmag import MagMaterial, SI, kuhn_cuboid, to_su
-    rng = np.random.default_rng(seed)
-    mags = []
-    for k in range(count):
-        L = rng.uniform([20.0, 20.0, 5.0], [120.0, 80.0, 15.0])
-        nx, ny, nz = [max(1, int(round(v / h))) for v in L]
-        Ms = rng.uniform(6e5, 1e6)
-        K1 = rng.uniform(0.0, 5e4)
-        ax = rng.standard_normal(3)
-        ax /= np.linalg.norm(ax)
-        hd = rng.standard_normal(3)
-        hd /= np.linalg.norm(hd)
-        H = hd * rng.uniform(0.0, 50.0) * 1e-3 / (4e-7 * math.pi)  # A/m
-        coords, tets = kuhn_cuboid(nx, ny, nz, h)
-        n = len(coords)
-        mat = MagMaterial("Py%d" % k, Ms=SI(Ms, "A/m"), exchange_coupling=SI(1.3e-11, "J/m"),
-                          llg_gamma_G=SI(2.211e5, "m/A s"), llg_damping=0.5)
-        m0 = rng.standard_normal((n, 3))
-        m0 /= np.linalg.norm(m0, axis=1, keepdims=True)
-        fields = {"m": m0,
-                  "H_ext": np.tile(float(to_su(SI(1.0, "A/m"), "A/m")) * H, (n, 1)),
-                  "K1": np.full(n, float(to_su(SI(K1, "J/m^3"), "J/m^3"))),
-                  "axis": np.tile(ax, (n, 1))}
-        mags.append(dict(dims=(nx, ny, nz), coords=coords, tets=tets, mat=mat, fields=fields))
-    return mags
+def run_cpu_leg(coords, tets, mat, Hext, states, rtol, budget_s):
+    """The oracle on the host cores (all threads, then one core) on a bounded
+    sample of the same trajectory states."""
+    import oracle
+    log("cpu baseline (oracle, %.0f s budget) ..." % budget_s)
+    threads = oracle.set_threads(int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))
+    rate, done, setup, _ = run_cpu(coords, tets, mat, Hext, states, rtol, budget_s)
+    # the same on one core, a smaller bounded sample
+    oracle.set_threads(1)
+    rate1, done1, _, _ = run_cpu(coords, tets, mat, Hext, states, rtol, budget_s / 2)
+    oracle.set_threads(threads)
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((ln.split(":", 1)[1].strip() for ln in f
+                          if ln.startswith("model name")), None)
+    except OSError:
+        pass
+    return {"value": round(rate, 3), "unit": "field evals/s", "cores": threads,
+            "kind": "port",
+            "sample": "%d FP64 field evals of the same trajectory states, warm-started "
+                      "Jacobi-PCG rtol %g (setup %.1f s excluded)" % (done, rtol, setup),
+            "one_core": {"value": round(rate1, 3), "unit": "field evals/s",
+                         "sample": "%d evals of the same states" % done1},
+            "cpu_model": model}
 
 
 def run_batch(steps, warmup, rtol, window_ps, torch):
     """Batching form: one handle for all magnets (grouped SpMV/BEM, one
     persistent CG) against one FP64 handle per magnet run one after another."""
     from nmag._native import NativeBatch, NativeMagnet
-    mags = batch_workload()
+    from nmag.workloads import cuboid_batch
+    mags = cuboid_batch()
     meshes = [(g["coords"], g["tets"]) for g in mags]
     mats = [g["mat"].su_coeffs() for g in mags]
     sizes = np.array([len(g["coords"]) for g in mags])
@@ -460,20 +439,11 @@ def run_adjoint(t_ns, rtol, fd_checks=True):
     (40 x 20 x 2 cubes, 2583 nodes), alpha 0.02, per-element K ~ N(2e4, 5e3)
     J/m^3 along z, 16 piecewise-constant x-field pulses over t_ns ~ N(0, 5) mT,
     loss <m_x>(t_ns); fixed-step RK4 at 0.2 ps, checkpoints every 50 steps."""
-    from nmag import MagMaterial, SI, kuhn_cuboid, to_su
     from nmag._native import NativeMagnet
     from nmag.adjoint import TrajectoryProblem
-    coords, tets = kuhn_cuboid(40, 20, 2, 2.5)
-    mat = MagMaterial("Py", Ms=SI(8.6e5, "A/m"), exchange_coupling=SI(1.3e-11, "J/m"),
-                      llg_gamma_G=SI(2.211e5, "m/A s"), llg_damping=0.02)
-    rng = np.random.default_rng(55)
+    from nmag.workloads import differentiation_slab
+    coords, tets, mat, K, amps, m0 = differentiation_slab()
     n = len(coords)
-    K = rng.normal(2e4, 5e3, len(tets)) * float(to_su(SI(1.0, "J/m^3"), "J/m^3"))
-    mt = float(to_su(SI(1e-3 / (4e-7 * math.pi), "A/m"), "A/m"))
-    amps = rng.normal(0.0, 5.0, 16) * mt
-    m0 = np.tile([1.0, 1.0, 0.0], (n, 1)) / math.sqrt(2.0)
-    m0 += math.radians(1.0) * rng.standard_normal((n, 3))
-    m0 /= np.linalg.norm(m0, axis=1, keepdims=True)
     dt = 0.2
     n_steps = int(round(t_ns * 1000.0 / dt))
     out = {"workload": "Py 100x50x5 nm, 2.5 nm Kuhn mesh, %d nodes, %d tets, %d pulses, "
@@ -591,6 +561,7 @@ def main():
 
     nx, ny, nz, h = [float(v) for v in args.dims.split(",")]
     coords, tets = kuhn_cuboid(int(nx), int(ny), int(nz), h)
+    from nmag.workloads import applied_field, s_state
     mat = make_material(alpha=0.02)
     Hext = applied_field(len(coords))
     m0 = s_state(coords, seed=2024)
@@ -602,6 +573,20 @@ def main():
     states = trajectory_states(gen, m0, total, args.dt)
     gen.close()
 
+    # Every leg runs guarded: a failing leg is recorded in the line's
+    # "errors" and the run exits non-zero, but what was measured is printed.
+    errors = {}
+
+    def leg(name, fn, *a, **kw):
+        try:
+            return fn(*a, **kw)
+        except Exception as e:  # noqa: BLE001 (recorded, then the run fails)
+            import traceback
+            errors[name] = "%s: %s" % (type(e).__name__, e)
+            log("leg %s FAILED: %s" % (name, errors[name]))
+            traceback.print_exc(file=sys.stderr)
+            return None
+
     results, outs = {}, {}
     for route in ("dense", "iterative"):
         if route not in args.routes.split(","):
@@ -610,8 +595,11 @@ def main():
         for prec in ("fp64", "mixed") + (("mixed24",) if route == "dense" else ()):
             key = prec + "/" + route
             log("timing %s ..." % key)
-            results[key], outs[key] = run_gpu(prec, route, coords, tets, mat, Hext, states,
-                                              args.warmup, args.steps, args.rtol, torch)
+            r = leg("field_evals:" + key, run_gpu, prec, route, coords, tets, mat, Hext, states,
+                    args.warmup, args.steps, args.rtol, torch)
+            if r is None:
+                continue
+            results[key], outs[key] = r
             log("%s: %.1f evals/s, kernels(ms) %s, cg iters %.1f/%.1f" % (
                 key, results[key]["evals_per_s"], results[key]["kernel_ms"],
                 results[key]["cg_iters_nbc"], results[key]["cg_iters_dbc"]))
@@ -619,35 +607,40 @@ def main():
     def rel(a, b):
         return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
 
-    # headline: the dense potential route
-    mixed, fp64 = results["mixed/dense"], results["fp64/dense"]
-    err = rel(outs["mixed/dense"], outs["fp64/dense"])
-    nb, n, nnz = mixed["nb"], mixed["n"], mixed["nnz"]
-    kern = mixed["kernel_ms"]
-    dominant = max(kern, key=kern.get)
-
     def roofline(kernel, alg_bytes, ms, traffic=None):
         ach = alg_bytes / (ms * 1e-3) / 1e9
         return {"bound": "hbm", "kernel": kernel, "achieved": round(ach, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
                 "traffic": traffic, "alg_bytes_per_launch": int(alg_bytes)}
 
+    # headline: the dense potential route
+    mixed, fp64 = results.get("mixed/dense"), results.get("fp64/dense")
+    m24 = results.get("mixed24/dense")
+    head = mixed or fp64 or m24
+    nb = head["nb"] if head else None
+    n = head["n"] if head else len(coords)
+    nnz = head["nnz"] if head else None
+    kern = mixed["kernel_ms"] if mixed else None
+    dominant = max(kern, key=kern.get) if kern else None
+    err = rel(outs["mixed/dense"], outs["fp64/dense"]) if mixed and fp64 else None
+
     # dense potential GEMV: the N x N operator once, rho in, phi out
     pot_bytes = lambda esz: esz * n * n + 16 * n
-    rooflines = {
-        "potential": roofline("k_dense_gemv<float> potential operator (N=%d)" % n,
-                              pot_bytes(4), kern["potential"], MEASURED_TRAFFIC.get((4, n))),
-        "potential_fp64": roofline("k_dense_gemv<double> potential operator (N=%d)" % n,
-                                   pot_bytes(8), fp64["kernel_ms"]["potential"],
-                                   MEASURED_TRAFFIC.get((8, n))),
-    }
-    m24 = results.get("mixed24/dense")
-    if m24 is not None:
+    rooflines = {}
+    if mixed:
+        rooflines["potential"] = roofline(
+            "k_dense_gemv<float> potential operator (N=%d)" % n, pot_bytes(4),
+            kern["potential"], MEASURED_TRAFFIC.get((4, n)))
+    if fp64:
+        rooflines["potential_fp64"] = roofline(
+            "k_dense_gemv<double> potential operator (N=%d)" % n, pot_bytes(8),
+            fp64["kernel_ms"]["potential"], MEASURED_TRAFFIC.get((8, n)))
+    if m24:
         rooflines["potential_f24"] = roofline(
             "k_dense_gemv<F24> potential operator (N=%d)" % n, pot_bytes(3),
             m24["kernel_ms"]["potential"], MEASURED_TRAFFIC.get((3, n)))
-    roof = rooflines.get(dominant)
-    pot64_gbs = pot_bytes(8) / (fp64["kernel_ms"]["potential"] * 1e-3) / 1e9
+    roof = rooflines.get(dominant) if dominant else None
+    pot64_gbs = (pot_bytes(8) / (fp64["kernel_ms"]["potential"] * 1e-3) / 1e9) if fp64 else None
     # iterative route (the reference's structure): per-iteration algorithmic
     # bytes of one PCG iteration (SpMV over the stiffness pattern with FP64
     # values + 4-byte columns, row pointers, about eight FP64 vector passes)
@@ -656,13 +649,13 @@ def main():
     if "mixed/iterative" in results:
         it_m = results["mixed/iterative"]
         kit = it_m["kernel_ms"]
-        cg_iter_bytes = 12 * nnz + 4 * (n + 1) + 64 * n
+        cg_iter_bytes = 12 * it_m["nnz"] + 4 * (n + 1) + 64 * n
         rooflines_iterative = {
             "cg_nbc": roofline("k_cg<double> Neumann solve",
                                cg_iter_bytes * max(it_m["cg_iters_nbc"], 1), kit["cg_nbc"]),
             "cg_dbc": roofline("k_cg<double> Dirichlet solve",
                                cg_iter_bytes * max(it_m["cg_iters_dbc"], 1), kit["cg_dbc"]),
-            "bem": roofline("k_dense_gemv<float> BEM", gemv_bytes(nb, 4), kit["bem"]),
+            "bem": roofline("k_dense_gemv<float> BEM", gemv_bytes(it_m["nb"], 4), kit["bem"]),
         }
 
     dyn = None
@@ -673,16 +666,30 @@ def main():
             avgs = {}
             precs = ("fp64", "mixed") + (("mixed24",) if route == "dense" else ())
             for prec in precs:
-                wall, stats, avgs[prec] = run_dynamics(prec, route, coords, tets, mat, Hext, m0,
-                                                       args.sim_window, 5.0, args.rtol)
+                r = leg("dynamics:%s/%s" % (prec, route), run_dynamics, prec, route, coords,
+                        tets, mat, Hext, m0, args.sim_window, 5.0, args.rtol)
+                if r is None:
+                    continue
+                wall, stats, avgs[prec] = r
+                setup_s = stats.pop("setup_s")
+                hours = args.sim_window * 1e-3
                 dyn[route][prec] = {
-                    "sim_ns_per_hour": round(args.sim_window * 1e-3 / (wall / 3600.0), 3),
-                    "wall_s": round(wall, 3), **{k: int(v) for k, v in stats.items()}}
-            dyn[route]["mean_m_max_dev_mixed_vs_fp64"] = float(
-                np.abs(avgs["mixed"] - avgs["fp64"]).max())
-            if "mixed24" in avgs:
+                    "sim_ns_per_hour": round(hours / (wall / 3600.0), 3),
+                    "wall_s": round(wall, 3),
+                    # end to end: handle creation (operator set-up) included
+                    "setup_s": round(setup_s, 3),
+                    "sim_ns_per_hour_incl_setup": round(hours / ((wall + setup_s) / 3600.0), 3),
+                    **{k: int(v) for k, v in stats.items()}}
+            if "mixed" in avgs and "fp64" in avgs:
+                dyn[route]["mean_m_max_dev_mixed_vs_fp64"] = float(
+                    np.abs(avgs["mixed"] - avgs["fp64"]).max())
+            if "mixed24" in avgs and "fp64" in avgs:
                 dyn[route]["mean_m_max_dev_mixed24_vs_fp64"] = float(
                     np.abs(avgs["mixed24"] - avgs["fp64"]).max())
+            if "mixed" in dyn[route] and "fp64" in dyn[route]:
+                dyn[route]["mixed_vs_fp64_sim_ns_per_hour"] = round(
+                    dyn[route]["mixed"]["sim_ns_per_hour"] /
+                    dyn[route]["fp64"]["sim_ns_per_hour"], 3)
         dyn["window_ps"] = args.sim_window
         dyn["integrator"] = "BDF order<=2, Newton-GMRES, rtol=atol=1e-6, from the seeded S-state"
 
@@ -698,72 +705,53 @@ def main():
 
     batch = None
     if not args.no_batch:
-        batch = run_batch(args.batch_steps, 3, args.rtol, args.batch_window, torch)
+        batch = leg("batch", run_batch, args.batch_steps, 3, args.rtol, args.batch_window, torch)
 
     precision = None
     if not args.no_precision:
-        precision = run_precision((coords, tets, mat, Hext, states[args.warmup]))
+        precision = leg("precision", run_precision, (coords, tets, mat, Hext, states[args.warmup]))
 
     adjoint = None
     if args.adjoint_ns > 0:
-        adjoint = run_adjoint(args.adjoint_ns, args.adjoint_rtol)
+        adjoint = leg("differentiation", run_adjoint, args.adjoint_ns, args.adjoint_rtol)
 
     cpu = None
     if not args.no_cpu:
-        import oracle
-        log("cpu baseline (oracle, %.0f s budget) ..." % args.cpu_budget)
-        threads = oracle.set_threads(int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))
-        rate, done, setup, _ = run_cpu(coords, tets, mat, Hext, states[args.warmup:],
-                                       args.rtol, args.cpu_budget)
-        # the same on one core, a smaller bounded sample
-        oracle.set_threads(1)
-        rate1, done1, _, _ = run_cpu(coords, tets, mat, Hext, states[args.warmup:],
-                                     args.rtol, args.cpu_budget / 2)
-        oracle.set_threads(threads)
-        model = None
-        try:
-            with open("/proc/cpuinfo") as f:
-                model = next((ln.split(":", 1)[1].strip() for ln in f
-                              if ln.startswith("model name")), None)
-        except OSError:
-            pass
-        cpu = {"value": round(rate, 3), "unit": "field evals/s", "cores": threads,
-               "kind": "port",
-               "sample": "%d FP64 field evals of the same trajectory states, warm-started "
-                         "Jacobi-PCG rtol %g (setup %.1f s excluded)" % (done, args.rtol, setup),
-               "one_core": {"value": round(rate1, 3), "unit": "field evals/s",
-                            "sample": "%d evals of the same states" % done1},
-               "cpu_model": model}
+        cpu = leg("cpu_baseline", run_cpu_leg, coords, tets, mat, Hext, states[args.warmup:],
+                  args.rtol, args.cpu_budget)
 
     line = {
         "metric": "LLG field evals/s & sim-ns/hour, 20k-node mesh, mixed vs FP64 at "
                   "⟨m⟩ err≤1e-5",
-        "value": round(mixed["evals_per_s"], 2),
+        "value": round(mixed["evals_per_s"], 2) if mixed else None,
         "unit": "field evals/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(mixed["ms_per_eval"], 4),
+        "ms_per_step": round(mixed["ms_per_eval"], 4) if mixed else None,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "fp32-mixed (FP32 potential operator and exchange/gradient SpMV, FP64 "
                  "accumulation, divergence, dots and state)",
         "data": "synthetic (seeded S-state trajectory, explicit-Euler states 0.1 ps apart)",
-        "config": {"workload": "Py slab 160x80x10 nm, 2 nm Kuhn mesh, 19926 nodes, "
-                               "Nb=%d, alpha=0.02, 25 mT @170deg" % nb,
-                   "demag": "dense potential operator M=(I+EBR)K^+ (%.2f GB FP32), built in "
-                            "%.1f s" % (mixed["potential_bytes"] / 1e9, mixed["setup_s"]),
+        "config": {"workload": "Py slab 160x80x10 nm, 2 nm Kuhn mesh, %d nodes, "
+                               "Nb=%s, alpha=0.02, 25 mT @170deg" % (n, nb),
+                   "demag": None if not mixed else
+                   "dense potential operator M=(I+EBR)K^+ (%.2f GB FP32), built in "
+                   "%.1f s" % (mixed["potential_bytes"] / 1e9, mixed["setup_s"]),
                    "ksp_rtol_iterative_route": args.rtol},
-        "fp64": {"value": round(fp64["evals_per_s"], 2),
-                 "ms_per_step": round(fp64["ms_per_eval"], 4),
-                 "kernel_ms": fp64["kernel_ms"],
-                 "potential_gemv_GBs": round(pot64_gbs, 1),
-                 "setup_s": fp64["setup_s"],
-                 "autotuned": fp64["autotuned"]},
-        "mixed_autotuned": mixed["autotuned"],
+        "fp64": None if not fp64 else {
+            "value": round(fp64["evals_per_s"], 2),
+            "ms_per_step": round(fp64["ms_per_eval"], 4),
+            "kernel_ms": fp64["kernel_ms"],
+            "potential_gemv_GBs": round(pot64_gbs, 1),
+            "setup_s": fp64["setup_s"],
+            "autotuned": fp64["autotuned"]},
+        "mixed_autotuned": mixed["autotuned"] if mixed else None,
         "mixed_kernel_ms": kern,
-        "mixed_vs_fp64_speedup": round(mixed["evals_per_s"] / fp64["evals_per_s"], 3),
+        "mixed_vs_fp64_speedup": (round(mixed["evals_per_s"] / fp64["evals_per_s"], 3)
+                                  if mixed and fp64 else None),
         "mixed_dmdt_max_rel_err": err,
         # host arrays in and out per evaluation (the CVODE-callback form),
         # PCIe copies included; never the headline value
@@ -775,14 +763,16 @@ def main():
         "mixed24": None if m24 is None else {
             "value": round(m24["evals_per_s"], 2),
             "ms_per_step": round(m24["ms_per_eval"], 4),
-            "vs_fp64_speedup": round(m24["evals_per_s"] / fp64["evals_per_s"], 3),
-            "dmdt_max_rel_err_vs_fp64": rel(outs["mixed24/dense"], outs["fp64/dense"]),
+            "vs_fp64_speedup": (round(m24["evals_per_s"] / fp64["evals_per_s"], 3)
+                                if fp64 else None),
+            "dmdt_max_rel_err_vs_fp64": (rel(outs["mixed24/dense"], outs["fp64/dense"])
+                                         if fp64 else None),
             "kernel_ms": m24["kernel_ms"],
             "potential_bytes": m24["potential_bytes"],
             "autotuned": m24["autotuned"]},
         "dense_vs_iterative_fp64_dmdt_rel_diff": (
-            rel(outs["fp64/dense"], outs["fp64/iterative"]) if "fp64/iterative" in outs
-            else None),
+            rel(outs["fp64/dense"], outs["fp64/iterative"])
+            if "fp64/iterative" in outs and "fp64/dense" in outs else None),
         "iterative_route": {
             key.split("/")[0]: {"value": round(results[key]["evals_per_s"], 2),
                                 "kernel_ms": results[key]["kernel_ms"],
@@ -799,8 +789,11 @@ def main():
         "precision": precision,
         "batch": batch,
         "differentiation": adjoint,
+        "errors": errors or None,
     }
-    print(json.dumps(line))
+    print(json.dumps(line), flush=True)
+    if errors:
+        sys.exit(1)
 
 
 if __name__ == "__main__":

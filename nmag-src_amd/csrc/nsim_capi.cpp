@@ -126,7 +126,8 @@ int setup_cg(nsim_handle* h, int rpt) {
   std::string err = build_cg_operator(h->host, 256 * rpt, op);
   if (!err.empty()) return fail(err);
   const int maxw = cg_slot_variant(op.max_width, rpt);
-  if (op.parts > 1024 || op.parts > cg_max_workgroups(rpt, maxw, h->device))
+  const int resident = cg_max_workgroups(rpt, maxw, h->device);
+  if (op.parts > 1024 || op.parts > resident)
     return fail("CG parts do not fit one co-resident grid at " + std::to_string(rpt) +
                 " rows per thread");
   for (void* p : h->cg_allocs) (void)hipFree(p);
@@ -140,6 +141,7 @@ int setup_cg(nsim_handle* h, int rpt) {
   c.naggs = op.naggs;
   c.maxw = maxw;
   c.max_width = op.max_width;
+  c.resident = resident;
   auto up = [&](auto** dst, const auto& vec) {
     hipError_t e = upload(dst, vec.data(), vec.size());
     if (*dst) h->cg_allocs.push_back(*dst);

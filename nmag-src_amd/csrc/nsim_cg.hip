@@ -72,6 +72,9 @@ struct Shared {
   double cin[kMaxAggs];
   double zc[kMaxAggs];
 };
+// three workgroups per CU, with 1 KiB for the compiler's own LDS use
+static_assert(3 * lds_granules((int)sizeof(Shared) + 1024) * kLdsGranule <= kLdsPerCu,
+              "CG workgroup LDS too large for three workgroups per CU");
 
 constexpr unsigned kSpinLimit = 1u << 20;
 
@@ -403,14 +406,30 @@ int cg_slot_variant(int width, int rpt) {
   return 0;
 }
 
+int resident_per_cu(const void* kernel, int threads, int device) {
+  int occ = 0, lds_cu = 0;
+  hipFuncAttributes fa;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kernel, threads, 0) != hipSuccess ||
+      hipFuncGetAttributes(&fa, kernel) != hipSuccess ||
+      hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor,
+                            device) != hipSuccess)
+    return 0;
+  if (lds_cu <= 0 || lds_cu > kLdsPerCu) lds_cu = kLdsPerCu;
+  if (fa.sharedSizeBytes > 0) {
+    const int per = lds_granules((int)fa.sharedSizeBytes) * kLdsGranule;
+    const int by_lds = lds_cu / per;
+    if (by_lds < occ) occ = by_lds;
+  }
+  return occ > 0 ? occ : 0;
+}
+
 int cg_max_workgroups(int rpt, int maxw, int device) {
   const void* k = select_kernel<double>(rpt, maxw);
   if (!k) return 0;
-  int per_cu = 0, cus = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, 256, 0) != hipSuccess) return 0;
+  int cus = 0;
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
     return 0;
-  return per_cu * cus;
+  return resident_per_cu(k, 256, device) * cus;
 }
 
 hipError_t launch_cg(const CgDevice& c, CgWork& w, bool dirichlet, bool coarse,
@@ -418,7 +437,10 @@ hipError_t launch_cg(const CgDevice& c, CgWork& w, bool dirichlet, bool coarse,
                      double* xsum, double* restrict_out, const int* blocal,
                      hipStream_t st) {
   const int grid = c.parts;
-  if (grid <= 0 || grid > w.gmax || grid > kMaxParts || maxit >= (1 << 23) ||
+  // a plain launch above the resident count would leave workgroups waiting
+  // for ones that cannot start
+  if (grid <= 0 || grid > w.gmax || grid > kMaxParts || grid > c.resident ||
+      maxit >= (1 << 23) ||
       (c.maxw > 0 && c.max_width > c.maxw) || c.naggs > kMaxAggs ||
       2LL * c.npad * 8 >= 0x7fffffffLL || (256 << 0) * c.rpt != (1 << c.cap_shift))
     return hipErrorInvalidConfiguration;

@@ -64,6 +64,17 @@ __device__ inline unsigned ld_coh(const unsigned* p) {
   return __hip_atomic_load(const_cast<unsigned*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// LDS is given to workgroups in whole allocation granules.  gfx950 has
+// 160 KiB of LDS per CU; its granule is taken as 2 KiB, the largest of the
+// granules in use on the CDNA parts, so a residency count made with it never
+// exceeds what the CU holds.  The persistent kernels size their grids from
+// this count: a workgroup that cannot start would leave the others waiting.
+constexpr int kLdsPerCu = 160 * 1024;
+constexpr int kLdsGranule = 2048;
+__host__ __device__ constexpr int lds_granules(int bytes) {
+  return (bytes + kLdsGranule - 1) / kLdsGranule;
+}
+
 // Phase tags of the grid-wide exchanges sit one per 128-byte line: every
 // workgroup polls all of them, and packed tags put all pollers on a few
 // lines (measured on MI355X: about 1 us more per exchange at 312 workgroups).

@@ -131,6 +131,7 @@ struct CgDevice {
   int* p2o = nullptr;
   double *dinv_nbc = nullptr, *dinv_dbc = nullptr;
   double *ainv_nbc = nullptr, *ainv_dbc = nullptr;
+  int resident = 0;  // workgroups of the kernel the device holds at once
 };
 // Persistent pipelined two-level PCG on the stiffness matrix; see the kernel.
 // b, x, xadd, xsum are in the original node order.
@@ -162,5 +163,10 @@ hipError_t launch_jtimes(const DeviceOps& d, bool mixed, const double* m, const 
 int cg_slot_variant(int width, int rpt);
 // Cooperative-launch capacity of the CG kernel (workgroups of 256 threads).
 int cg_max_workgroups(int rpt, int maxw, int device);
+// Workgroups of `kernel` (`threads` threads each) that one CU holds at once:
+// the runtime's occupancy figure, capped by kLdsPerCu over the kernel's LDS
+// rounded up to whole allocation granules (the runtime's figure divides by
+// the unrounded size).  0 if the device cannot be queried.
+int resident_per_cu(const void* kernel, int threads, int device);
 
 }  // namespace nsim

@@ -12,7 +12,10 @@
 
 namespace nsim {
 
-constexpr int kGmresMaxl = 512;  // largest Krylov dimension the kernel holds
+// Largest Krylov dimension the kernel holds (the reference's default maxl is
+// 300).  Larger bases run the host-driven GMRES.  The per-workgroup arrays of
+// this size are what keep three workgroups' LDS within one CU.
+constexpr int kGmresMaxl = 320;
 constexpr int kGmresTagStride = dev::kTagStride;  // tag words per workgroup and parity
 
 // Device buffers of the solver (owned by the caller).
@@ -26,6 +29,7 @@ struct GmresWork {
   double* status = nullptr;   // [0] iterations [1] sum (x ewt)^2 [2] error flag
   unsigned* err = nullptr;    // exchange error flag
   int gmax = 0, pstride = 0;
+  int resident = 0;              // workgroups the device holds at once (gmres_grid)
   unsigned long long epoch = 0;  // launch counter; makes every phase tag unique
   // The grid never exceeds what the device holds at once (gmres_grid), and
   // the launches before and after it on the stream wait for it, so its
@@ -37,8 +41,10 @@ struct GmresWork {
 
 // Grid of the kernel for a handle with `nslices` SELL slices: workgroups
 // (all co-resident, at most gmax) and slices per workgroup (at least
-// min_spw).  Returns false if no grid fits.
-bool gmres_grid(int nslices, int maxl, int device, int gmax, int min_spw, int* grid, int* spw);
+// min_spw); `resident` receives the number of workgroups the device holds at
+// once.  Returns false if no grid fits or maxl exceeds kGmresMaxl.
+bool gmres_grid(int nslices, int maxl, int device, int gmax, int min_spw, int* grid, int* spw,
+                int* resident);
 
 // One Newton iteration of a BDF step, y - gamma f(y) - avec = 0.
 struct NewtonSystem {
@@ -66,6 +72,7 @@ struct NewtonSystem {
 // to |residual| <= tol (scaled Euclidean) or maxl iterations, then sets
 // ycur += x.  Writes w.status: [0] iterations, [1] sum (x ewt)^2,
 // [2] != 0 if an exchange timed out, [3] sum ((ycur - yp) ewt)^2.
+// Refuses a grid above w.resident.
 hipError_t launch_newton_gmres(const DeviceOps& d, bool mixed, GmresWork& w, int grid, int spw,
                                int maxl, const NewtonSystem& s, hipStream_t st);
 

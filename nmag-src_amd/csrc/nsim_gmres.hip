@@ -47,7 +47,7 @@ constexpr int kMaxGrid = 1024;
 constexpr int kPcNodes = 256;              // largest node block of the local sweeps
 // Krylov iterations whose R every workgroup keeps in LDS (what fits beside
 // the rest while three workgroups share a CU's 160 KiB)
-constexpr int kRtCap = 15;
+constexpr int kRtCap = 32;
 
 template <typename TV>
 struct GmresArgs {
@@ -104,6 +104,12 @@ struct GShared {
   // to kRtCap iterations each solves R y = g itself, with no broadcast
   double Rt[kRtCap * (kRtCap + 1) / 2];
 };
+// Three workgroups per CU (the VGPR budget allows three), each rounded up to
+// whole LDS granules, with 1 KiB for the compiler's own LDS use.  A record
+// that rounded past a third of the CU would let only two start per CU while
+// the occupancy query still reports three.
+static_assert(3 * lds_granules((int)sizeof(GShared) + 1024) * kLdsGranule <= kLdsPerCu,
+              "GMRES workgroup LDS too large for three workgroups per CU");
 
 // z = M^-1 t on one node (3x3 block).
 __device__ inline void pc_node(const double* __restrict__ Minv, int i, const double* t,
@@ -561,17 +567,19 @@ hipError_t launch_typed(const DeviceOps& d, const TV* K, GmresWork& w, int grid,
 }  // namespace
 
 bool gmres_grid(int nslices, int maxl, int device, int gmax, int min_spw, int* grid,
-                int* spw) {
+                int* spw, int* resident) {
   if (nslices <= 0 || maxl < 1 || maxl > kGmresMaxl) return false;
-  int cus = 0, pf = 0, pd = 0;
+  int cus = 0;
   // the FP32 and FP64 variants differ only in one load; take the smaller
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&pf, kernel_ptr<float>(), 256, 0) !=
-          hipSuccess ||
-      hipOccupancyMaxActiveBlocksPerMultiprocessor(&pd, kernel_ptr<double>(), 256, 0) !=
-          hipSuccess ||
-      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
+  const int pf = resident_per_cu(kernel_ptr<float>(), 256, device);
+  const int pd = resident_per_cu(kernel_ptr<double>(), 256, device);
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
     return false;
   int cap = (pf < pd ? pf : pd) * cus;
+  *resident = cap;
+  // one slot in sixteen is left free, so that a few workgroups of another
+  // queue's kernel on the device cannot keep part of the grid from starting
+  cap -= cap / 16;
   if (cap > kMaxGrid) cap = kMaxGrid;
   if (cap > gmax) cap = gmax;
   if (cap < 1) return false;
@@ -584,7 +592,8 @@ bool gmres_grid(int nslices, int maxl, int device, int gmax, int min_spw, int* g
 
 hipError_t launch_newton_gmres(const DeviceOps& d, bool mixed, GmresWork& w, int grid, int spw,
                                int maxl, const NewtonSystem& s, hipStream_t st) {
-  if (grid < 1 || grid > w.gmax || maxl < 1 || maxl > kGmresMaxl || w.pstride < maxl + 2 ||
+  if (grid < 1 || grid > w.gmax || grid > w.resident || maxl < 1 || maxl > kGmresMaxl ||
+      w.pstride < maxl + 2 ||
       (long long)grid * spw < d.nslices || (long long)(grid - 1) * spw >= d.nslices ||
       3LL * d.n * 8 >= (1LL << 32) || (long long)w.gmax * w.pstride * 8 >= (1LL << 32))
     return hipErrorInvalidConfiguration;

@@ -80,11 +80,12 @@ Integrator::Integrator(nsim_handle* h, const TsOptions& o) : h_(h), opt_(o) {
   }
   // persistent GMRES: one co-resident grid over the SELL slices
   const DeviceOps& d = handle_ops(h);
-  int grid = 0, spw = 0;
+  int grid = 0, spw = 0, resident = 0;
   if (ok_ && nsim::gmres_grid(d.nslices, opt_.krylov_max, handle_device(h), 1 << 20,
-                              std::max(1, opt_.gmres_spw), &grid, &spw)) {
+                              std::max(1, opt_.gmres_spw), &grid, &spw, &resident)) {
     gw_.V = V_;
     gw_.gmax = grid;
+    gw_.resident = resident;
     gw_.pstride = (opt_.krylov_max + 2 + 7) & ~7;
     const size_t maxl = opt_.krylov_max;
     bool g = alloc(&gw_.Z, n * maxl) && alloc(&gw_.part, 2 * (size_t)grid * gw_.pstride) &&
