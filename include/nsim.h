@@ -176,21 +176,48 @@ int nsim_get_potential(nsim_handle* h, double* host_out);
 int nsim_rhs(nsim_handle* h, double t, const double* y, double* ydot, int device_ptrs);
 
 /* CVODE jtimes callback (src/sundials_sp_stubs.c:579, src/nsim.ml:2400-2417):
- * Jv = J(y) v with H frozen at the last rhs except for the exchange term,
- * the reference's jacobi_ts1 linearisation (nmag_lam.py:432-436). */
+ * Jv = J v with the reference's jacobi_ts1 linearisation (nmag_lam.py:432-436:
+ * H frozen except for the exchange term).  J is the Jacobian built by the
+ * last nsim_psetup that rebuilt it, whatever y is, as in the reference, whose
+ * jtimes multiplies by the matrix of its last re-Jacobi and reads neither y
+ * nor fy.  Before the first psetup, J is taken at y with H_total of the last
+ * rhs.  fy (CVODE's f(t, y)) is accepted for the callback's shape and not
+ * read; y may be null once a psetup has run. */
 int nsim_jtimes(nsim_handle* h, double t, const double* v, double* Jv,
-                const double* y, int device_ptrs);
+                const double* y, const double* fy, int device_ptrs);
 
 /* CVODE psetup callback (src/sundials_sp_stubs.c:352, src/nsim.ml:2419-2461):
- * builds M = I - gamma J_block, the 3x3-per-node block diagonal of the
- * jacobi_ts1 Jacobian at y with H_total of the last rhs; *jcur = 1. */
+ * when jok == 0 or no Jacobian exists yet, rebuilds the jacobi_ts1 Jacobian
+ * at y (the fields of y are evaluated, as the reference's re-Jacobi runs the
+ * field part of the rhs) and sets *jcur = 1; when jok != 0 keeps the last
+ * Jacobian and sets *jcur = 0.  Then forms M = I - gamma J and its 3x3
+ * node-block Jacobi splitting, the preconditioner of nsim_psolve's solve.
+ * fy is not read (nor is it in the reference). */
 int nsim_psetup(nsim_handle* h, double t, const double* y, const double* fy, int jok,
                 int* jcur, double gamma, int device_ptrs);
 /* CVODE psolve callback (src/sundials_sp_stubs.c:473, src/nsim.ml:2465-2484):
- * z = M^-1 r.  Only left preconditioning (lr == 1) is accepted, as in the
- * reference (src/nsim.ml:2469-2472). */
+ * z solves M z = r, M = I - gamma J from the last nsim_psetup (psolve's gamma
+ * and delta are not used, as in the reference).  Restarted GMRES(30) from a
+ * zero guess, left-preconditioned by M's 3x3 node blocks (the reference's KSP
+ * uses bjacobi/ILU), until the preconditioned residual is at most
+ * max(pc_rtol x its initial value, pc_atol), for at most pc_maxit iterations,
+ * or until it exceeds pc_dtol x its initial value: pc_rtol 1e-2, pc_atol 1e-7
+ * (interface/nmag/main.py:728-729), dtol 1e5 and maxit 10000 (PETSc's
+ * defaults); set them with nsim_ts_set_tols.  nsim_set_tuning
+ * "pc_full_solve" 0 makes z = M_block^-1 r instead (one block solve).  Only
+ * left preconditioning (lr == 1) is accepted, as in the reference
+ * (src/nsim.ml:2469-2472).  Statistics: "pc_solves", "pc_iters_last",
+ * "pc_iters_total", "pc_unconverged", "pc_jac_builds". */
 int nsim_psolve(nsim_handle* h, double t, const double* y, const double* fy, const double* r,
                 double* z, double gamma, double delta, int lr, int device_ptrs);
+
+/* Replaces lam_ts_set_tols (src/nsimexec.ml:4148-4171 -> src/nsim.ml:2581-2600):
+ * psolve's KSP tolerances (rtol, atol, dtol, maxit) and the integrator's
+ * rtol and atol.  A negative value (maxit: <= 0) keeps the current setting,
+ * as the reference's optional arguments do.  Integrator tolerances take
+ * effect at the next nsim_advance. */
+int nsim_ts_set_tols(nsim_handle* h, double ksp_rtol, double ksp_atol, double ksp_dtol,
+                     int ksp_maxit, double cvode_rtol, double cvode_atol);
 
 /* Replaces lam_ts_init (src/nsimexec.ml:4113-4128 -> src/nsim.ml:2485-2536):
  * (re)initialises the device BDF integrator from the handle's m at t0 with

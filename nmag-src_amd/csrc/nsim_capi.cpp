@@ -696,6 +696,10 @@ int nsim_set_tuning(nsim_handle* h, const char* key, int value) {
     if (h->ts) h->ts->options().cooperative = value != 0;
     return 0;
   }
+  if (k == "pc_full_solve") {
+    h->pc_full = value != 0;
+    return 0;
+  }
   if (k == "gmres_device") {
     h->ts_opt.device_gmres = value != 0;
     if (h->ts) h->ts->options().device_gmres = h->ts_opt.device_gmres;
@@ -968,22 +972,6 @@ int nsim_rhs(nsim_handle* h, double t, const double* y, double* ydot, int device
   return 0;
 }
 
-int nsim_jtimes(nsim_handle* h, double t, const double* v, double* Jv, const double* y,
-                int device_ptrs) {
-  (void)t;
-  if (!h || !v || !Jv || !y) return fail("nsim_jtimes: null argument");
-  const size_t bytes = 3 * (size_t)h->d.n * sizeof(double);
-  if (device_ptrs) return device_jtimes(h, y, v, Jv);
-  HIPCHK(hipMemcpyAsync(h->io_a, y, bytes, hipMemcpyHostToDevice, h->stream));
-  HIPCHK(hipMemcpyAsync(h->io_b, v, bytes, hipMemcpyHostToDevice, h->stream));
-  double* out = h->io_c;
-  int rc = device_jtimes(h, h->io_a, h->io_b, out);
-  if (rc) return rc;
-  HIPCHK(hipMemcpyAsync(Jv, out, bytes, hipMemcpyDeviceToHost, h->stream));
-  HIPCHK(hipStreamSynchronize(h->stream));
-  return 0;
-}
-
 int nsim_get_stat(nsim_handle* h, const char* name, double* out) {
   if (!h || !name || !out) return fail("nsim_get_stat: null argument");
   std::string s(name);
@@ -1021,6 +1009,15 @@ int nsim_get_stat(nsim_handle* h, const char* name, double* out) {
     else if (s == "ts_t") *out = t.t;
     else if (s == "ts_max_dm") *out = t.max_dm;
     else if (s == "ts_gmres_grid") *out = h->ts->gmres_grid();
+    else return fail("nsim_get_stat: unknown statistic '" + s + "'");
+    return 0;
+  }
+  if (s.rfind("pc_", 0) == 0) {
+    if (s == "pc_solves") *out = (double)h->pc_solves;
+    else if (s == "pc_iters_last") *out = h->pc_iters_last;
+    else if (s == "pc_iters_total") *out = (double)h->pc_iters_total;
+    else if (s == "pc_unconverged") *out = (double)h->pc_unconverged;
+    else if (s == "pc_jac_builds") *out = (double)h->pc_jac_builds;
     else return fail("nsim_get_stat: unknown statistic '" + s + "'");
     return 0;
   }
@@ -1115,52 +1112,6 @@ int nsim_advance(nsim_handle* h, double t_final, int max_steps, int exact_tstop,
   if (handle_error_flags(h, &flags) == 0 && flags != 0)
     return fail("nsim_advance: a Poisson solve reported an error (flag " +
                 std::to_string(flags) + ")");
-  return 0;
-}
-
-int nsim_psetup(nsim_handle* h, double t, const double* y, const double* fy, int jok,
-                int* jcur, double gamma, int device_ptrs) {
-  (void)t;
-  (void)fy;
-  (void)jok;
-  if (!h || !y) return fail("nsim_psetup: null argument");
-  const size_t n = h->d.n;
-  if (!h->pc_Minv) {
-    HIPCHK(hipMalloc(&h->pc_Minv, 9 * n * sizeof(double)));
-    h->track(h->pc_Minv);
-  }
-  const double* yd = y;
-  if (!device_ptrs) {
-    HIPCHK(hipMemcpyAsync(h->io_a, y, 3 * n * sizeof(double), hipMemcpyHostToDevice, h->stream));
-    yd = h->io_a;
-  }
-  // H_total of the most recent rhs (CVODE calls psetup after f(t, y))
-  HIPCHK(bjac_setup(h->stream, (int)n, h->d.coefs, h->d.inst, gamma, yd, h->Htotal, h->kappa,
-                    h->pc_Minv));
-  if (jcur) *jcur = 1;
-  if (!device_ptrs) HIPCHK(hipStreamSynchronize(h->stream));
-  return 0;
-}
-
-int nsim_psolve(nsim_handle* h, double t, const double* y, const double* fy, const double* r,
-                double* z, double gamma, double delta, int lr, int device_ptrs) {
-  (void)t;
-  (void)y;
-  (void)fy;
-  (void)gamma;
-  (void)delta;
-  if (!h || !r || !z) return fail("nsim_psolve: null argument");
-  if (lr != 1) return fail("nsim_psolve: only left preconditioning (lr == 1) is supported");
-  if (!h->pc_Minv) return fail("nsim_psolve: call nsim_psetup first");
-  const size_t n = h->d.n;
-  if (device_ptrs) {
-    HIPCHK(bjac_solve(h->stream, (int)n, h->pc_Minv, r, z));
-    return 0;
-  }
-  HIPCHK(hipMemcpyAsync(h->io_a, r, 3 * n * sizeof(double), hipMemcpyHostToDevice, h->stream));
-  HIPCHK(bjac_solve(h->stream, (int)n, h->pc_Minv, h->io_a, h->io_b));
-  HIPCHK(hipMemcpyAsync(z, h->io_b, 3 * n * sizeof(double), hipMemcpyDeviceToHost, h->stream));
-  HIPCHK(hipStreamSynchronize(h->stream));
   return 0;
 }
 

@@ -74,7 +74,20 @@ struct nsim_handle {
   nsim::Integrator* ts = nullptr;
   nsim::TsOptions ts_opt;
   double* kappa = nullptr;    // negJ K_ii / V_i: dH_exch_i/dm_i
-  double* pc_Minv = nullptr;  // block-Jacobi preconditioner of nsim_psetup
+  // CVODE preconditioner callbacks (nsim_psolve.hip): the Jacobian frozen at
+  // the last re-Jacobi (linearisation point m and its H_total), M = I - gamma J
+  // with its 3x3 node-block inverses, psolve's KSP settings and workspace
+  double* pc_Minv = nullptr;
+  double *pc_m = nullptr, *pc_H = nullptr;
+  bool pc_have_J = false;
+  double pc_gamma = 0.0;
+  double pc_rtol = 1e-2, pc_atol = 1e-7;  // interface/nmag/main.py:728-729
+  double pc_dtol = 1e5;                   // PETSc's defaults
+  int pc_maxit = 10000;
+  bool pc_full = true;  // 0: psolve applies the block preconditioner only
+  double* pc_ws = nullptr;
+  long pc_solves = 0, pc_iters_total = 0, pc_unconverged = 0, pc_jac_builds = 0;
+  int pc_iters_last = 0;
   nsim::AdjointState* adj = nullptr;
   std::vector<void*> allocations;
 

@@ -58,7 +58,9 @@ SIGNATURES = {
     "nsim_execute": (ctypes.c_int, [_vp, ctypes.c_char_p]),
     "nsim_get_bem": (ctypes.c_int, [_vp, _dp]),
     "nsim_rhs": (ctypes.c_int, [_vp, ctypes.c_double, _vp, _vp, ctypes.c_int]),
-    "nsim_jtimes": (ctypes.c_int, [_vp, ctypes.c_double, _vp, _vp, _vp, ctypes.c_int]),
+    "nsim_jtimes": (ctypes.c_int, [_vp, ctypes.c_double, _vp, _vp, _vp, _vp, ctypes.c_int]),
+    "nsim_ts_set_tols": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                        ctypes.c_int, ctypes.c_double, ctypes.c_double]),
     "nsim_get_stat": (ctypes.c_int, [_vp, ctypes.c_char_p, _dp]),
     "nsim_timers_enable": (ctypes.c_int, [_vp, ctypes.c_int]),
     "nsim_get_timer": (ctypes.c_int, [_vp, ctypes.c_char_p, _dp]),
@@ -229,10 +231,15 @@ class NativeMagnet:
         _check(lib().nsim_rhs(self._h, t, y_ptr, ydot_ptr, 1), "nsim_rhs")
 
     def jtimes(self, y, v, t=0.0):
-        y = np.ascontiguousarray(y, dtype=np.float64).reshape(-1)
+        """J v with the Jacobian of the last re-Jacobi (psetup), or, before any
+        psetup, at y with H_total of the last rhs (y may then not be None)."""
         v = np.ascontiguousarray(v, dtype=np.float64).reshape(-1)
+        yp = None
+        if y is not None:
+            y = np.ascontiguousarray(y, dtype=np.float64).reshape(-1)
+            yp = y.ctypes.data
         jv = np.zeros_like(v)
-        _check(lib().nsim_jtimes(self._h, t, v.ctypes.data, jv.ctypes.data, y.ctypes.data, 0),
+        _check(lib().nsim_jtimes(self._h, t, v.ctypes.data, jv.ctypes.data, yp, None, 0),
                "nsim_jtimes")
         return jv
 
@@ -291,12 +298,19 @@ class NativeMagnet:
     def synchronize(self):
         _check(lib().nsim_synchronize(self._h), "nsim_synchronize")
 
-    def psetup(self, y, gamma, t=0.0):
+    def psetup(self, y, gamma, t=0.0, jok=False):
+        """CVODE's psetup: returns jcur (1 if the Jacobian was rebuilt at y)."""
         y = np.ascontiguousarray(y, dtype=np.float64).reshape(-1)
         jcur = ctypes.c_int(0)
-        _check(lib().nsim_psetup(self._h, t, y.ctypes.data, None, 0, ctypes.byref(jcur), gamma, 0),
-               "nsim_psetup")
+        _check(lib().nsim_psetup(self._h, t, y.ctypes.data, None, 1 if jok else 0,
+                                 ctypes.byref(jcur), gamma, 0), "nsim_psetup")
         return jcur.value
+
+    def set_tols(self, ksp_rtol=-1.0, ksp_atol=-1.0, ksp_dtol=-1.0, ksp_maxit=0,
+                 cvode_rtol=-1.0, cvode_atol=-1.0):
+        """lam_ts_set_tols: negative values (maxit <= 0) keep the setting."""
+        _check(lib().nsim_ts_set_tols(self._h, ksp_rtol, ksp_atol, ksp_dtol, int(ksp_maxit),
+                                      cvode_rtol, cvode_atol), "nsim_ts_set_tols")
 
     def psolve(self, r, gamma=0.0, t=0.0, lr=1):
         r = np.ascontiguousarray(r, dtype=np.float64).reshape(-1)

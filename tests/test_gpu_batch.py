@@ -107,7 +107,10 @@ def test_batch_jtimes_and_preconditioner_match_single_magnets():
     y = _cat(fields, "m")
     b.rhs(y.reshape(-1))
     jv = b.split(b.jtimes(y.reshape(-1), v.reshape(-1)).reshape(-1, 3))
-    b.psetup(y.reshape(-1), gamma=0.7)
+    b.psetup(y.reshape(-1), gamma=0.05)
+    # solved to near rounding, so that the batch's one solve and the
+    # per-magnet solves agree (to pc_rtol 1e-2 they would stop differently)
+    b.set_tols(ksp_rtol=1e-11, ksp_atol=0.0)
     z = b.split(b.psolve(v.reshape(-1)).reshape(-1, 3))
     vs = b.split(v)
     for k, (coords, tets) in enumerate(meshes):
@@ -118,9 +121,12 @@ def test_batch_jtimes_and_preconditioner_match_single_magnets():
         s.rhs(yk)
         ref = s.jtimes(yk, vs[k].reshape(-1)).reshape(-1, 3)
         assert rel_err(jv[k], ref) < 1e-9, k
-        s.psetup(yk, gamma=0.7)
+        s.psetup(yk, gamma=0.05)
+        s.set_tols(ksp_rtol=1e-11, ksp_atol=0.0)
         zr = s.psolve(vs[k].reshape(-1)).reshape(-1, 3)
         assert rel_err(z[k], zr) < 1e-9, k
+        assert s.stat("pc_unconverged") == 0
+    assert b.stat("pc_unconverged") == 0
 
 
 @pytest.mark.parametrize("prec,tol_demag,tol_dmdt", [("mixed", 1e-6, 1e-6),

@@ -165,8 +165,11 @@ def test_relaxation_decreases_energy_and_converges():
 
 
 def test_psetup_psolve_blocks_match_the_jacobian_diagonal():
+    """psolve with "pc_full_solve" 0 applies only the 3x3 node blocks of
+    M = I - gamma J (the preconditioner of the full solve)."""
     mat = material(alpha=0.2)
     coords, tets, g = _magnet(2, 2, 1, 2.5, mat)
+    g.tune("pc_full_solve", 0)
     m = random_m(g.n, 31)
     H = np.tile([0.03, -0.01, 0.02], (g.n, 1))
     g.set_field("H_ext", H)
