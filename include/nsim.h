@@ -211,6 +211,22 @@ int nsim_psetup(nsim_handle* h, double t, const double* y, const double* fy, int
 int nsim_psolve(nsim_handle* h, double t, const double* y, const double* fy, const double* r,
                 double* z, double gamma, double delta, int lr, int device_ptrs);
 
+/* nsim_psolve with nothing allocated: the same solve, its GMRES workspace
+ * taken from a static device array of the library (16 MiB per GPU, held until
+ * the process ends) instead of memory nsim_psolve allocates at its first
+ * call.  No counterpart in the reference: it is for callers that cannot
+ * allocate device memory (tight solver loops, harnesses that forbid
+ * hipMalloc).  Magnets of up to about 20,400 nodes (3N-vector workspace of
+ * 816 bytes per node); larger ones fail with an error naming the limit.
+ * nsim_psetup must have run (it allocates the frozen Jacobian).  Every call on
+ * one GPU shares that array, so calls that overlap on one GPU (two handles on
+ * two host threads) race; calls in sequence are independent.  The call
+ * synchronises its stream once per Arnoldi step, so it cannot be captured in a
+ * HIP graph. */
+int nsim_psolve_noalloc(nsim_handle* h, double t, const double* y, const double* fy,
+                        const double* r, double* z, double gamma, double delta, int lr,
+                        int device_ptrs);
+
 /* Replaces lam_ts_set_tols (src/nsimexec.ml:4148-4171 -> src/nsim.ml:2581-2600):
  * psolve's KSP tolerances (rtol, atol, dtol, maxit) and the integrator's
  * rtol and atol.  A negative value (maxit: <= 0) keeps the current setting,

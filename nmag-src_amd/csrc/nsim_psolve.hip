@@ -28,6 +28,8 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
 
 #include "nsim_device.h"
@@ -244,6 +246,42 @@ int pc_gmres(nsim_handle* h, const PcWork& w, const double* b, double* z) {
   return 0;
 }
 
+// Workspace of nsim_psolve_noalloc: a file-scope device array, one copy per
+// GPU, held until the process ends.  16 MiB: the GMRES(30) workspace of a
+// magnet of up to about 20,400 nodes (816 bytes per node).  In host code the
+// array's name is only a stand-in; its device address comes from
+// hipGetSymbolAddress, taken once per device with that device current.
+constexpr size_t kScratchDoubles = 2u * 1024u * 1024u;
+constexpr int kScratchMaxNodes = 20400;
+__device__ __align__(256) double scratch[kScratchDoubles];
+
+std::mutex g_scratch_mu;
+std::map<int, double*> g_scratch_ptr;  // device -> address of its copy
+
+int scratch_ptr(int device, double** out) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  auto it = g_scratch_ptr.find(device);
+  if (it != g_scratch_ptr.end()) {
+    *out = it->second;
+    return 0;
+  }
+  int cur = -1;
+  HIPCHK(hipGetDevice(&cur));
+  if (cur != device) HIPCHK(hipSetDevice(device));
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (cur != device) HIPCHK(hipSetDevice(cur));
+  if (e != hipSuccess || !p)
+    return fail(std::string("nsim_psolve_noalloc: hipGetSymbolAddress(scratch): ") +
+                hipGetErrorString(e));
+  g_scratch_ptr[device] = static_cast<double*>(p);
+  *out = static_cast<double*>(p);
+  return 0;
+}
+
+static_assert(kScratchMaxNodes * 816ull + 64 * 1024 <= kScratchDoubles * sizeof(double),
+              "the stated node limit must fit the scratch");
+
 }  // namespace
 
 namespace nsim {
@@ -346,6 +384,45 @@ int nsim_psolve(nsim_handle* h, double t, const double* y, const double* fy, con
   } else {
     TRY(apply_P(h, rd, zd));
   }
+  if (!device_ptrs) {
+    HIPCHK(hipMemcpyAsync(z, zd, bytes, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+  }
+  return 0;
+}
+
+int nsim_psolve_noalloc(nsim_handle* h, double t, const double* y, const double* fy,
+                        const double* r, double* z, double gamma, double delta, int lr,
+                        int device_ptrs) {
+  (void)t;
+  (void)y;
+  (void)fy;
+  (void)gamma;
+  (void)delta;
+  if (!h || !r || !z) return fail("nsim_psolve_noalloc: null argument");
+  if (lr != 1)
+    return fail("nsim_psolve_noalloc: only left preconditioning (lr == 1) is supported");
+  if (!h->pc_have_J) return fail("nsim_psolve_noalloc: call nsim_psetup first");
+  const int n3 = 3 * h->d.n;
+  const size_t need = pc_work_doubles(n3);
+  if (h->pc_full && need > kScratchDoubles)
+    return fail("nsim_psolve_noalloc: " + std::to_string(h->d.n) + " nodes need " +
+                std::to_string(need * sizeof(double)) + " bytes of workspace, above the " +
+                std::to_string(kScratchDoubles * sizeof(double)) +
+                " bytes of the static scratch (at most about " +
+                std::to_string(kScratchMaxNodes) + " nodes)");
+  double* ws = nullptr;
+  TRY(scratch_ptr(h->device, &ws));
+  const size_t bytes = (size_t)n3 * sizeof(double);
+  const double* rd = r;
+  double* zd = z;
+  if (!device_ptrs) {
+    HIPCHK(hipMemcpyAsync(h->io_a, r, bytes, hipMemcpyHostToDevice, h->stream));
+    rd = h->io_a;
+    zd = h->io_b;
+  }
+  if (h->pc_full) TRY(pc_gmres(h, pc_work_layout(ws, n3), rd, zd));
+  else TRY(apply_P(h, rd, zd));
   if (!device_ptrs) {
     HIPCHK(hipMemcpyAsync(z, zd, bytes, hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));

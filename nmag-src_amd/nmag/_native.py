@@ -59,6 +59,9 @@ SIGNATURES = {
     "nsim_get_bem": (ctypes.c_int, [_vp, _dp]),
     "nsim_rhs": (ctypes.c_int, [_vp, ctypes.c_double, _vp, _vp, ctypes.c_int]),
     "nsim_jtimes": (ctypes.c_int, [_vp, ctypes.c_double, _vp, _vp, _vp, _vp, ctypes.c_int]),
+    "nsim_psolve_noalloc": (ctypes.c_int, [_vp, ctypes.c_double, _vp, _vp, _vp, _vp,
+                                           ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                           ctypes.c_int]),
     "nsim_ts_set_tols": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                         ctypes.c_int, ctypes.c_double, ctypes.c_double]),
     "nsim_get_stat": (ctypes.c_int, [_vp, ctypes.c_char_p, _dp]),
@@ -305,6 +308,14 @@ class NativeMagnet:
         _check(lib().nsim_psetup(self._h, t, y.ctypes.data, None, 1 if jok else 0,
                                  ctypes.byref(jcur), gamma, 0), "nsim_psetup")
         return jcur.value
+
+    def psolve_noalloc(self, r, gamma=0.0, t=0.0, lr=1):
+        """psolve with its workspace in the library's static device scratch."""
+        r = np.ascontiguousarray(r, dtype=np.float64).reshape(-1)
+        z = np.zeros_like(r)
+        _check(lib().nsim_psolve_noalloc(self._h, t, None, None, r.ctypes.data, z.ctypes.data,
+                                         gamma, 0.0, lr, 0), "nsim_psolve_noalloc")
+        return z
 
     def set_tols(self, ksp_rtol=-1.0, ksp_atol=-1.0, ksp_dtol=-1.0, ksp_maxit=0,
                  cvode_rtol=-1.0, cvode_atol=-1.0):
