@@ -76,6 +76,7 @@ def graphs(g):
 MEASURED_TRAFFIC = {(4, 19926): 777846 * 2048 + 179 * 1024,
                     (8, 19926): 1554955 * 2048 + 185 * 1024}
 KERNELS = {"dense": ("exch_div", "potential", "demag_llg"),
+           "hmatrix": ("exch_div", "potential", "demag_llg"),
            "iterative": ("exch_div", "cg_nbc", "bem", "cg_dbc", "demag_llg")}
 
 
@@ -522,9 +523,9 @@ def main():
     ap.add_argument("--sim-window", type=float, default=1000.0,
                     help="ps of LLG dynamics timed for sim-ns/hour (1000 = the full 1 ns)")
     ap.add_argument("--no-dynamics", action="store_true")
-    ap.add_argument("--dynamics-routes", default="dense",
+    ap.add_argument("--dynamics-routes", default="dense,hmatrix",
                     help="demag routes timed in the dynamics leg")
-    ap.add_argument("--routes", default="dense,iterative",
+    ap.add_argument("--routes", default="dense,iterative,hmatrix",
                     help="demag routes timed in the field-evaluation leg (dense first)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default tile shapes (short profiler runs)")
@@ -588,11 +589,15 @@ def main():
             return None
 
     results, outs = {}, {}
-    for route in ("dense", "iterative"):
+    # the precisions timed per route: the 24-bit mode differs from FP32-mixed
+    # only in the dense operator; the compressed operator is timed with FP32
+    # entries (in FP64 at 1e-12 per block it compresses too little to pay)
+    route_precs = {"dense": ("fp64", "mixed", "mixed24"), "iterative": ("fp64", "mixed"),
+                   "hmatrix": ("mixed",)}
+    for route in ("dense", "iterative", "hmatrix"):
         if route not in args.routes.split(","):
             continue
-        # the 24-bit mode differs from FP32-mixed only in the dense operator
-        for prec in ("fp64", "mixed") + (("mixed24",) if route == "dense" else ()):
+        for prec in route_precs[route]:
             key = prec + "/" + route
             log("timing %s ..." % key)
             r = leg("field_evals:" + key, run_gpu, prec, route, coords, tets, mat, Hext, states,
@@ -613,8 +618,12 @@ def main():
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
                 "traffic": traffic, "alg_bytes_per_launch": int(alg_bytes)}
 
-    # headline: the dense potential route
-    mixed, fp64 = results.get("mixed/dense"), results.get("fp64/dense")
+    # headline: FP32-mixed on the compressed (H-matrix) potential operator when
+    # it was timed, else on the dense one; the FP64 path is the dense operator
+    # (exact up to rounding)
+    head_key = "mixed/hmatrix" if "mixed/hmatrix" in results else "mixed/dense"
+    mixed, fp64 = results.get(head_key), results.get("fp64/dense")
+    mixed_dense = results.get("mixed/dense")
     m24 = results.get("mixed24/dense")
     head = mixed or fp64 or m24
     nb = head["nb"] if head else None
@@ -622,15 +631,22 @@ def main():
     nnz = head["nnz"] if head else None
     kern = mixed["kernel_ms"] if mixed else None
     dominant = max(kern, key=kern.get) if kern else None
-    err = rel(outs["mixed/dense"], outs["fp64/dense"]) if mixed and fp64 else None
+    err = rel(outs[head_key], outs["fp64/dense"]) if mixed and fp64 else None
 
     # dense potential GEMV: the N x N operator once, rho in, phi out
     pot_bytes = lambda esz: esz * n * n + 16 * n
     rooflines = {}
-    if mixed:
+    if mixed and head_key == "mixed/hmatrix":
+        # the compressed operator: its FP32 entries and index arrays once, rho
+        # in and phi out (two launches, timed together)
         rooflines["potential"] = roofline(
+            "k_panel_gemv<float> x2 H-matrix potential operator (N=%d, %.0f MB)" % (
+                n, mixed["potential_bytes"] / 1e6),
+            mixed["potential_bytes"] + 16 * n, kern["potential"], None)
+    if mixed_dense:
+        rooflines["potential_dense" if head_key == "mixed/hmatrix" else "potential"] = roofline(
             "k_dense_gemv<float> potential operator (N=%d)" % n, pot_bytes(4),
-            kern["potential"], MEASURED_TRAFFIC.get((4, n)))
+            mixed_dense["kernel_ms"]["potential"], MEASURED_TRAFFIC.get((4, n)))
     if fp64:
         rooflines["potential_fp64"] = roofline(
             "k_dense_gemv<double> potential operator (N=%d)" % n, pot_bytes(8),
@@ -661,16 +677,19 @@ def main():
     dyn = None
     if not args.no_dynamics:
         dyn = {}
+        all_avgs = {}
         for route in args.dynamics_routes.split(","):
             dyn[route] = {}
             avgs = {}
-            precs = ("fp64", "mixed") + (("mixed24",) if route == "dense" else ())
+            precs = {"dense": ("fp64", "mixed", "mixed24"), "hmatrix": ("mixed",)}.get(
+                route, ("fp64", "mixed"))
             for prec in precs:
                 r = leg("dynamics:%s/%s" % (prec, route), run_dynamics, prec, route, coords,
                         tets, mat, Hext, m0, args.sim_window, 5.0, args.rtol)
                 if r is None:
                     continue
                 wall, stats, avgs[prec] = r
+                all_avgs[(prec, route)] = avgs[prec]
                 setup_s = stats.pop("setup_s")
                 hours = args.sim_window * 1e-3
                 dyn[route][prec] = {
@@ -690,6 +709,15 @@ def main():
                 dyn[route]["mixed_vs_fp64_sim_ns_per_hour"] = round(
                     dyn[route]["mixed"]["sim_ns_per_hour"] /
                     dyn[route]["fp64"]["sim_ns_per_hour"], 3)
+            # against the dense FP64 run (the FP64 path) when this route has none
+            ref64 = all_avgs.get(("fp64", "dense"))
+            if route != "dense" and "mixed" in avgs and ref64 is not None:
+                dyn[route]["mean_m_max_dev_mixed_vs_fp64_dense"] = float(
+                    np.abs(avgs["mixed"] - ref64).max())
+                if "fp64" in dyn.get("dense", {}):
+                    dyn[route]["mixed_vs_fp64_dense_sim_ns_per_hour"] = round(
+                        dyn[route]["mixed"]["sim_ns_per_hour"] /
+                        dyn["dense"]["fp64"]["sim_ns_per_hour"], 3)
         dyn["window_ps"] = args.sim_window
         dyn["integrator"] = "BDF order<=2, Newton-GMRES, rtol=atol=1e-6, from the seeded S-state"
 
@@ -733,13 +761,17 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "fp32-mixed (FP32 potential operator and exchange/gradient SpMV, FP64 "
-                 "accumulation, divergence, dots and state)",
+                 "accumulation, divergence, dots and state)" + (
+                     "; potential operator compressed to 1e-7 per block, the order of FP32 "
+                     "rounding" if head_key == "mixed/hmatrix" else ""),
         "data": "synthetic (seeded S-state trajectory, explicit-Euler states 0.1 ps apart)",
         "config": {"workload": "Py slab 160x80x10 nm, 2 nm Kuhn mesh, %d nodes, "
                                "Nb=%s, alpha=0.02, 25 mT @170deg" % (n, nb),
                    "demag": None if not mixed else
-                   "dense potential operator M=(I+EBR)K^+ (%.2f GB FP32), built in "
-                   "%.1f s" % (mixed["potential_bytes"] / 1e9, mixed["setup_s"]),
+                   ("H-matrix potential operator M=(I+EBR)K^+ (%.3f GB: FP32 entries and "
+                    "indices), built in %.1f s" if head_key == "mixed/hmatrix" else
+                    "dense potential operator M=(I+EBR)K^+ (%.2f GB FP32), built in %.1f s") % (
+                        mixed["potential_bytes"] / 1e9, mixed["setup_s"]),
                    "ksp_rtol_iterative_route": args.rtol},
         "fp64": None if not fp64 else {
             "value": round(fp64["evals_per_s"], 2),
@@ -753,11 +785,24 @@ def main():
         "mixed_vs_fp64_speedup": (round(mixed["evals_per_s"] / fp64["evals_per_s"], 3)
                                   if mixed and fp64 else None),
         "mixed_dmdt_max_rel_err": err,
+        # FP32-mixed on the dense operator (the same arithmetic without the
+        # compression)
+        "mixed_dense": None if not mixed_dense else {
+            "value": round(mixed_dense["evals_per_s"], 2),
+            "ms_per_step": round(mixed_dense["ms_per_eval"], 4),
+            "vs_fp64_speedup": (round(mixed_dense["evals_per_s"] / fp64["evals_per_s"], 3)
+                                if fp64 else None),
+            "dmdt_max_rel_err_vs_fp64": (rel(outs["mixed/dense"], outs["fp64/dense"])
+                                         if fp64 else None),
+            "kernel_ms": mixed_dense["kernel_ms"],
+            "setup_s": mixed_dense["setup_s"]},
+        "headline_route": head_key,
         # host arrays in and out per evaluation (the CVODE-callback form),
         # PCIe copies included; never the headline value
         "host_pointer_evals_per_s": {
-            key.split("/")[0]: round(results[key]["host_pointer_evals_per_s"], 2)
-            for key in ("mixed/dense", "fp64/dense", "mixed24/dense") if key in results},
+            key: round(results[key]["host_pointer_evals_per_s"], 2)
+            for key in ("mixed/hmatrix", "mixed/dense", "fp64/dense", "mixed24/dense")
+            if key in results},
         # the potential operator in 24 bits (FP32 rounded to 15 significand
         # bits, 3 bytes per entry), everything else as FP32-mixed
         "mixed24": None if m24 is None else {

@@ -66,9 +66,18 @@ typedef struct {
    * create time (Cholesky factorisations), so that phi = M rho is one GEMV
    * streamed from HBM; exact up to rounding, the tolerances are unused.
    * 0: 1 when every magnet has at most 20480 nodes (40960 with BF16
-   * storage) and all operators fit 32 GiB, else -1; nsim_get_stat
-   * "potential" tells which route a handle took. */
+   * storage) and all operators fit 32 GiB, else -1.  2: the same operator M
+   * compressed as an H-matrix (dense near-field blocks, low-rank far-field
+   * blocks to a relative tolerance of compress_eps per block), applied in two
+   * launches; magnets above 20480 nodes allowed.  No counterpart in the
+   * reference (its hlib H-matrices compress the BEM matrix B only,
+   * src/bem3d.ml:555-621).  nsim_get_stat "potential" tells which route a
+   * handle took (1 dense, 2 H-matrix, 0 iterative). */
   int potential;
+  /* Relative Frobenius tolerance of each low-rank block of the H-matrix
+   * route; 0 selects the default, 1e-12 with FP64 storage and 1e-7 with FP32
+   * storage (the other precisions), an error of the order of their rounding. */
+  double compress_eps;
 } nsim_options;
 
 /* Replaces raw_make_lam (src/nsimexec.ml:3605 -> src/nsim.ml:2828): builds

@@ -174,10 +174,19 @@ namespace nsim {
 // Full field evaluation for magnetisation m (device) into dmdt (device).
 namespace {
 
+// phi = M rho on the potential route: one dense GEMV or the two H-matrix
+// phases.
+hipError_t launch_potential(nsim_handle* h, const double* rho, double* phi) {
+  // the H-matrix phases read rho in their cluster order, which k_exch_div
+  // wrote beside rho
+  if (h->use_hmatrix) return launch_hmatrix(h->d.hm, phi, h->stream);
+  return launch_dense_gemv(h->d.pot, rho, phi, h->stream);
+}
+
 // The dense route's three launches, as recorded into a graph.
 hipError_t dense_rhs_launches(nsim_handle* h, const double* m, double* dmdt, bool demag) {
   hipError_t e = launch_exch_div(h->d, h->mixed, m, h->Hexch, h->rho, demag, h->stream);
-  if (e == hipSuccess && demag) e = launch_dense_gemv(h->d.pot, h->rho, h->phi, h->stream);
+  if (e == hipSuccess && demag) e = launch_potential(h, h->rho, h->phi);
   if (e != hipSuccess) return e;
   FieldBufs b{h->Hexch, h->Hdemag, h->Hanis, h->Htotal, h->rho,
               h->phi1,  h->phi2,   h->phi,   h->phi1b};
@@ -259,7 +268,7 @@ void drop_rhs_graphs(nsim_handle* h) {
 
 int device_rhs(nsim_handle* h, const double* m, double* dmdt) {
   const bool demag = h->opt.do_demag != 0 && h->d.nb > 0;
-  if (h->rhs_graph && !h->timers_on && (!demag || h->d.pot.A)) {
+  if (h->rhs_graph && !h->timers_on && (!demag || h->use_potential)) {
     // the eager first evaluation inside graph_rhs counts as the same one
     const int rc = graph_rhs(h, m, dmdt, demag);
     if (rc == 0) h->rhs_evals++;
@@ -269,10 +278,10 @@ int device_rhs(nsim_handle* h, const double* m, double* dmdt) {
     Span s(h, "exch_div");
     HIPCHK(launch_exch_div(h->d, h->mixed, m, h->Hexch, h->rho, demag, h->stream));
   }
-  if (demag && h->d.pot.A) {
+  if (demag && h->use_potential) {
     // phi = M rho with the precomputed potential operator M = (I + E B R) K^+
     Span s(h, "potential");
-    HIPCHK(launch_dense_gemv(h->d.pot, h->rho, h->phi, h->stream));
+    HIPCHK(launch_potential(h, h->rho, h->phi));
   } else if (demag) {
     CgWork& w = h->cgw;
     {
@@ -337,6 +346,7 @@ void handle_set_halt(nsim_handle* h, const int* halt) {
   drop_rhs_graphs(h);  // their kernel arguments hold the old word
   h->d.halt = halt;
   h->d.pot.halt = halt;
+  h->d.hm.halt = halt;
   h->d.bem.halt = halt;
   h->cgw.halt = halt;
 }
@@ -415,7 +425,12 @@ int create_impl(const char* who, int ninst, const nsim_mesh* meshes,
     h->opt.nbc_rtol = h->opt.dbc_rtol = 1e-5;
     h->opt.nbc_maxits = h->opt.dbc_maxits = 10000;
     h->opt.potential = 0;
+    h->opt.compress_eps = 0.0;
   }
+  if (h->opt.potential > 2)
+    return cleanup(fail(me + ": unknown potential route " + std::to_string(h->opt.potential)));
+  if (!(h->opt.compress_eps >= 0.0) || h->opt.compress_eps >= 1.0)
+    return cleanup(fail(me + ": compress_eps must be in [0, 1)"));
   if (h->opt.precision < NSIM_FP64 || h->opt.precision > NSIM_MIXED_F24)
     return cleanup(fail(me + ": unknown precision"));
   if (h->opt.nbc_maxits <= 0) h->opt.nbc_maxits = 10000;
@@ -485,11 +500,13 @@ int create_impl(const char* who, int ninst, const nsim_mesh* meshes,
     }
     const bool fits = max_n <= dense_gemv_max_order(ptype) &&
                       (double)dense_bytes(ptype, pot_elems) <= kPotentialBudgetBytes;
-    if (h->opt.potential > 0 && !fits)
+    if (h->opt.potential == 1 && !fits)
       return cleanup(fail(me + ": the dense potential operator needs every magnet to have at "
                                "most " + std::to_string(dense_gemv_max_order(ptype)) +
                           " nodes and at most 32 GiB in all"));
-    h->use_potential = h->opt.do_demag && o.nb > 0 && h->opt.potential >= 0 && fits;
+    h->use_hmatrix = h->opt.do_demag && o.nb > 0 && h->opt.potential == 2;
+    h->use_potential =
+        h->opt.do_demag && o.nb > 0 && (h->opt.potential == 2 || (h->opt.potential >= 0 && fits));
   }
 
   hipError_t e;
@@ -567,7 +584,7 @@ int create_impl(const char* who, int ninst, const nsim_mesh* meshes,
       for (int i = 0; i < o.n; i++) iota[i] = i;
       UP(d.iota, iota);
       if (build_potential(h) != 0) return cleanup(-1);
-      if (setup_gemv(h, d.pot, 2, 1, 16) != 0) return cleanup(-1);
+      if (!h->use_hmatrix && setup_gemv(h, d.pot, 2, 1, 16) != 0) return cleanup(-1);
     }
   }
 
@@ -794,7 +811,7 @@ int nsim_autotune(nsim_handle* h, int what) {
     h->tuned_pot_us = tune_dense(h, h->d.pot, h->rho, h->phi2, a, b);
     if (h->tuned_pot_us < 0.0) return done(-1);
   }
-  if ((what & 2) && demag && !h->d.pot.A) {
+  if ((what & 2) && demag && !h->use_potential) {
     // CG layout: rows per thread (part size), timed on a cold Neumann solve
     // of the load of the current m
     if (launch_exch_div(h->d, h->mixed, h->m, h->Hexch, h->rho, true, h->stream) != hipSuccess)
@@ -867,7 +884,7 @@ int nsim_get_field(nsim_handle* h, const char* name, double* out) {
     for (int i = 0; i < h->d.nb; i++) out[i] = h->host.bnode[i];
     return 0;
   }
-  if ((f == "phi1" || f == "phi2") && h->d.pot.A)
+  if ((f == "phi1" || f == "phi2") && h->use_potential)
     return fail("nsim_get_field: '" + f + "' is not formed on the dense potential route "
                 "(phi = phi1 + phi2 is)");
   if (f == "m") src = h->m, cnt = 3 * n;
@@ -944,7 +961,10 @@ int nsim_get_bem(nsim_handle* h, double* out) {
 
 int nsim_get_potential(nsim_handle* h, double* out) {
   if (!h || !out) return fail("nsim_get_potential: null argument");
-  if (!h->d.pot.A) return fail("nsim_get_potential: this handle uses the iterative route");
+  if (!h->d.pot.A)
+    return fail(h->use_hmatrix
+                    ? "nsim_get_potential: this handle holds the operator compressed (H-matrix)"
+                    : "nsim_get_potential: this handle uses the iterative route");
   return copy_dense(h, h->d.pot, out);
 }
 
@@ -1028,13 +1048,26 @@ int nsim_get_stat(nsim_handle* h, const char* name, double* out) {
   {
     const DenseOp& b = h->d.bem;
     const DenseOp& p = h->d.pot;
+    const HMatrixOp& hm = h->d.hm;
+    const double pot_bytes = h->use_hmatrix ? (double)(hm.p1.bytes + hm.p2.bytes)
+                                            : (double)dense_bytes(p.type, p.elems);
     const std::map<std::string, double> simple = {
+        {"hm_eps", hm.eps}, {"hm_rank_entries", (double)hm.nt},
+        {"hm_full_entries", (double)hm.full_entries},
+        {"hm_dense_entries", (double)hm.dense_entries},
+        {"hm_lowrank_entries", (double)hm.lowrank_entries},
+        {"hm_lowrank_blocks", (double)hm.lowrank_blocks},
+        {"hm_dense_blocks", (double)hm.dense_blocks}, {"hm_max_rank", (double)hm.max_rank},
+        {"hm_fallback_blocks", (double)hm.fallback_blocks}, {"hm_build_s", hm.build_s},
+        {"hm_p1_panels", (double)hm.p1.panels}, {"hm_p2_panels", (double)hm.p2.panels},
+        {"hm_p1_entries", (double)hm.p1.entries}, {"hm_p2_entries", (double)hm.p2.entries},
         {"bem_rows", b.rows}, {"bem_wg_per_cu", b.wg_per_cu}, {"bem_waves", b.waves},
         {"bem_grid", b.grid}, {"pot_rows", p.rows}, {"pot_wg_per_cu", p.wg_per_cu},
         {"pot_waves", p.waves}, {"pot_grid", p.grid}, {"n_magnets", h->d.ninst},
-        {"potential", p.A ? 1.0 : 0.0}, {"tuned_bem_us", h->tuned_bem_us},
+        {"potential", h->use_hmatrix ? 2.0 : p.A ? 1.0 : 0.0},
+        {"tuned_bem_us", h->tuned_bem_us},
         {"tuned_cg_us", h->tuned_cg_us}, {"tuned_pot_us", h->tuned_pot_us},
-        {"potential_bytes", (double)dense_bytes(p.type, p.elems)},
+        {"potential_bytes", pot_bytes},
         {"bem_bytes", (double)dense_bytes(b.type, b.elems)}};
     const auto it = simple.find(s);
     if (it != simple.end()) {

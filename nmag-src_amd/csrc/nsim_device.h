@@ -48,6 +48,38 @@ struct DenseOp {
   const int* halt = nullptr;               // as DeviceOps::halt
 };
 
+// One phase of the H-matrix potential operator on the device: panels of 16
+// rows (see nsim_hmatrix_build.h for the layout).
+struct PanelDesc;
+struct PanelOp {
+  int type = kBemF64;  // entries in FP64 or FP32
+  int panels = 0;  // work items
+  long long entries = 0;
+  size_t bytes = 0;  // entries and index arrays
+  PanelDesc* desc = nullptr;  // per panel: entries, groups, rows
+  int* gsrc = nullptr;      // per group: first source (>= 0 rho in cluster
+                            // order, < 0 the t entry -s - 1); G consecutive
+  int* rowdst = nullptr;
+  double* partial = nullptr;  // row sums of the chunks of long panels
+  unsigned* cnt = nullptr;    // per chunked panel: chunks finished (reset to 0)
+  void* data = nullptr;
+};
+
+// The H-matrix potential operator: phase 1 (t = V^T rho per low-rank block)
+// and phase 2 (phi = dense near field + U t).
+struct HMatrixOp {
+  PanelOp p1, p2;
+  double* t = nullptr;      // phase-1 results, nt entries (+ padding)
+  int nt = 0;
+  int* pos = nullptr;       // node -> cluster-order position (null: no H-matrix)
+  double* xp = nullptr;     // rho in cluster order, written by k_exch_div
+  double eps = 0.0;        // relative tolerance per block
+  const int* halt = nullptr;
+  long long full_entries = 0, dense_entries = 0, lowrank_entries = 0;
+  int lowrank_blocks = 0, dense_blocks = 0, max_rank = 0, fallback_blocks = 0;
+  double build_s = 0.0;    // host compression time
+};
+
 // Operators resident in HBM (one copy per precision actually used).
 struct DeviceOps {
   int n = 0, nb = 0, ntri = 0, nslices = 0, sell_total = 0;
@@ -67,6 +99,7 @@ struct DeviceOps {
   DenseOp bem;                      // BEM matrix, one segment per magnet
   std::vector<double> h_bem_geo_eps;  // per segment geometric tolerance
   DenseOp pot;                      // dense demag potential operator (A null: iterative)
+  HMatrixOp hm;                     // compressed potential operator (p2.panels 0: none)
   int* iota = nullptr;              // identity map of the nodes
   const int* halt = nullptr;        // nonzero word: the field kernels return at once
                                     // (set by a device-driven integrator run that
@@ -139,6 +172,14 @@ hipError_t launch_cg(const CgDevice& c, CgWork& w, bool dirichlet, bool coarse,
                      const double* b, double* x, double rtol, int maxit, const double* xadd,
                      double* xsum, double* restrict_out, const int* blocal,
                      hipStream_t st);
+// Uploads a compressed operator built on the host (entries converted to
+// `type`, kBemF64 or kBemF32); allocations are appended to `track`.
+struct HMatrixHost;
+hipError_t upload_hmatrix(const HMatrixHost& H, int type, int num_cus, HMatrixOp& op,
+                          std::vector<void*>* track);
+// phi = M rho with the H-matrix potential operator (two launches); rho is
+// read in the cluster order from op.xp, which k_exch_div fills.
+hipError_t launch_hmatrix(const HMatrixOp& op, double* y, hipStream_t st);
 // Grouped GEMV of a DenseOp (see there).
 hipError_t launch_dense_gemv(const DenseOp& op, const double* x, double* y, hipStream_t st);
 // Host plan of the grouped GEMV for a tile shape (rows_per_pass 1, 2 or 4 rows

@@ -47,7 +47,8 @@ struct nsim_handle {
   std::vector<nsim::Coeffs> coefs;  // host copy of the per-magnet table
   std::vector<int> node_off;     // magnet k owns nodes [node_off[k], node_off[k+1])
   std::vector<int> bnd_off;      // and boundary nodes [bnd_off[k], bnd_off[k+1])
-  bool use_potential = false;    // demag through the dense potential operator
+  bool use_potential = false;    // demag through the potential operator M
+  bool use_hmatrix = false;      // M compressed as an H-matrix (else dense)
   double tuned_bem_us = 0.0, tuned_cg_us = 0.0, tuned_pot_us = 0.0;  // autotuner's best
   nsim_options opt{};
   bool mixed = false;

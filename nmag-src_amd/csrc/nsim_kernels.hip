@@ -73,7 +73,9 @@ __global__ __launch_bounds__(256) void k_exch_div(int n, const int* __restrict__
                                                   const double* __restrict__ m,
                                                   double* __restrict__ Hexch,
                                                   double* __restrict__ rho, int want_rho,
-                                                  const int* __restrict__ halt) {
+                                                  const int* __restrict__ halt,
+                                                  const int* __restrict__ rho_pos,
+                                                  double* __restrict__ rho_perm) {
   if (halt && *halt) return;  // queued by a finished integrator run
   __shared__ double sh[4 * kSplit][64];
   const int s = blockIdx.x, t = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -101,7 +103,11 @@ __global__ __launch_bounds__(256) void k_exch_div(int n, const int* __restrict__
   Hexch[3 * row] = f * v[0];
   Hexch[3 * row + 1] = f * v[1];
   Hexch[3 * row + 2] = f * v[2];
-  if (want_rho) rho[row] = c.Ms * v[3];
+  if (want_rho) {
+    rho[row] = c.Ms * v[3];
+    // the H-matrix route reads rho in its cluster order
+    if (rho_perm) rho_perm[rho_pos[row]] = c.Ms * v[3];
+  }
 }
 
 // H_demag_i = sum_j G_ij phi_j (op_grad_phi + CFBOX, nmag_lam.py:755-759),
@@ -222,11 +228,13 @@ hipError_t launch_exch_div(const DeviceOps& d, bool mixed, const double* m, doub
   if (mixed)
     k_exch_div<float><<<d.nslices, 256, 0, st>>>(d.n, d.sell_off, d.sell_col, d.Kf,
                                                          d.D, d.Vinv, d.coefs, d.inst, m,
-                                                         Hexch, rho, want_rho ? 1 : 0, d.halt);
+                                                         Hexch, rho, want_rho ? 1 : 0, d.halt,
+                                                         d.hm.pos, d.hm.xp);
   else
     k_exch_div<double><<<d.nslices, 256, 0, st>>>(d.n, d.sell_off, d.sell_col, d.K,
                                                           d.D, d.Vinv, d.coefs, d.inst, m,
-                                                          Hexch, rho, want_rho ? 1 : 0, d.halt);
+                                                          Hexch, rho, want_rho ? 1 : 0, d.halt,
+                                                          d.hm.pos, d.hm.xp);
   return hipGetLastError();
 }
 

@@ -23,11 +23,14 @@
 #include <rocsolver/rocsolver.h>
 
 #include <algorithm>
+#include <chrono>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "nsim_dense.h"
 #include "nsim_handle.h"
+#include "nsim_hmatrix_build.h"
 
 namespace nsim {
 
@@ -119,8 +122,26 @@ int build_potential(nsim_handle* h) {
     elems += (long long)nk * ld;
   }
   p.elems = elems;
-  HCHK(hipMalloc(&p.A, dense_bytes(p.type, elems)), "potential operator alloc");
-  h->track(p.A);
+  // H-matrix route: each magnet's M is built dense in FP64 as below, then
+  // compressed on the host; FP64 entries for an FP64 handle, FP32 otherwise
+  const bool hm = h->use_hmatrix;
+  const int hm_type = h->opt.precision == NSIM_FP64 ? kBemF64 : kBemF32;
+  const double hm_eps = h->opt.compress_eps > 0.0 ? h->opt.compress_eps
+                        : hm_type == kBemF64      ? 1e-12
+                                                  : 1e-7;
+  HMatrixHost H;
+  H.group = hm_type == kBemF32 ? 4 : 2;
+  std::vector<double> hostP;
+  double hm_seconds = 0.0;
+  const int hm_threads =
+      std::max(1, std::min(16, (int)std::thread::hardware_concurrency()));
+  if (hm) {
+    elems = 0;
+    p.elems = 0;
+  } else {
+    HCHK(hipMalloc(&p.A, dense_bytes(p.type, elems)), "potential operator alloc");
+    h->track(p.A);
+  }
   for (auto* v : {&p.h_n, &p.h_ld, &p.h_off}) {
     int* dv = nullptr;
     HCHK(hipMalloc(&dv, v->size() * sizeof(int)), "potential tables");
@@ -224,13 +245,45 @@ int build_potential(nsim_handle* h) {
     }
     s.release(W);
 
-    // store row-major in the storage type
-    HCHK(dense_store_rowmajor(p.type, N, p.h_ld[k], P, N,
-                              (char*)p.A + dense_bytes(p.type, p.h_aoff[k]), st),
-         "store");
-    HCHK(hipStreamSynchronize(st), "sync");
+    if (hm) {
+      // to the host, then the H-matrix compression of this magnet
+      hostP.resize((size_t)N * N);
+      HCHK(hipMemcpyAsync(hostP.data(), P, sizeof(double) * (size_t)N * N,
+                          hipMemcpyDeviceToHost, st),
+           "potential operator to host");
+      HCHK(hipStreamSynchronize(st), "sync");
+      const auto t0 = std::chrono::steady_clock::now();
+      const std::string err = hmatrix_compress(N, hostP.data(), N, &o.x[3 * (size_t)o0], o0,
+                                               hm_eps, 2.0, hm_threads, H);
+      hm_seconds +=
+          std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+      if (!err.empty()) return fail("potential set-up: " + err);
+    } else {
+      // store row-major in the storage type
+      HCHK(dense_store_rowmajor(p.type, N, p.h_ld[k], P, N,
+                                (char*)p.A + dense_bytes(p.type, p.h_aoff[k]), st),
+           "store");
+      HCHK(hipStreamSynchronize(st), "sync");
+    }
     s.release(P);
     for (int* q : {dbl, dil, dposb, dposi}) s.release(q);
+  }
+  if (hm) {
+    hostP.clear();
+    hostP.shrink_to_fit();
+    hmatrix_finish(H);
+    HMatrixOp& op = d.hm;
+    HCHK(upload_hmatrix(H, hm_type, h->num_cus, op, &h->allocations), "H-matrix upload");
+    op.eps = hm_eps;
+    op.full_entries = H.full_entries;
+    op.dense_entries = H.dense_entries;
+    op.lowrank_entries = H.lowrank_entries;
+    op.lowrank_blocks = H.lowrank_blocks;
+    op.dense_blocks = H.dense_blocks;
+    op.max_rank = H.max_rank;
+    op.fallback_blocks = H.fallback_blocks;
+    op.build_s = hm_seconds;
+    op.halt = d.halt;
   }
   return 0;
 }

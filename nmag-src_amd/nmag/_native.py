@@ -34,10 +34,10 @@ class NsimOptions(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int), ("do_demag", ctypes.c_int),
                 ("nbc_rtol", ctypes.c_double), ("dbc_rtol", ctypes.c_double),
                 ("nbc_maxits", ctypes.c_int), ("dbc_maxits", ctypes.c_int),
-                ("potential", ctypes.c_int)]
+                ("potential", ctypes.c_int), ("compress_eps", ctypes.c_double)]
 
 
-POTENTIAL = {"auto": 0, "dense": 1, "iterative": -1}
+POTENTIAL = {"auto": 0, "dense": 1, "iterative": -1, "hmatrix": 2}
 
 
 class NsimTraj(ctypes.Structure):
@@ -140,15 +140,16 @@ class NativeMagnet:
 
     def __init__(self, coords, tets, material, precision="fp64", do_demag=True,
                  nbc_rtol=1e-5, dbc_rtol=1e-5, nbc_maxits=10000, dbc_maxits=10000,
-                 potential="auto"):
+                 potential="auto", compress_eps=0.0):
         """potential: "iterative" (the two Poisson solves around the BEM, as
-        the reference), "dense" (the precomputed potential operator) or "auto"
-        (dense when it fits)."""
+        the reference), "dense" (the precomputed potential operator), "hmatrix"
+        (the same operator compressed to compress_eps per block; 0: the
+        precision's default) or "auto" (dense when it fits)."""
         self._create([(coords, tets)], [material], precision, do_demag, nbc_rtol, dbc_rtol,
-                     nbc_maxits, dbc_maxits, potential)
+                     nbc_maxits, dbc_maxits, potential, compress_eps)
 
     def _create(self, meshes, materials, precision, do_demag, nbc_rtol, dbc_rtol, nbc_maxits,
-                dbc_maxits, potential="auto"):
+                dbc_maxits, potential="auto", compress_eps=0.0):
         L = lib()
         self._h = None
         if len(meshes) != len(materials) or not meshes:
@@ -168,7 +169,7 @@ class NativeMagnet:
         prec = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
         pot = POTENTIAL[potential] if isinstance(potential, str) else int(potential)
         opt = NsimOptions(prec, 1 if do_demag else 0, nbc_rtol, dbc_rtol, nbc_maxits, dbc_maxits,
-                          pot)
+                          pot, float(compress_eps))
         h = _vp()
         if len(meshes) == 1:
             _check(L.nsim_create(cmeshes, cmats, ctypes.byref(opt), ctypes.byref(h)),
@@ -271,6 +272,10 @@ class NativeMagnet:
     @property
     def dense_potential(self):
         return self.stat("potential") == 1.0
+
+    @property
+    def hmatrix_potential(self):
+        return self.stat("potential") == 2.0
 
     def stat(self, name):
         v = ctypes.c_double(0)
@@ -399,9 +404,10 @@ class NativeBatch(NativeMagnet):
     """
 
     def __init__(self, meshes, materials, precision="fp64", do_demag=True, nbc_rtol=1e-5,
-                 dbc_rtol=1e-5, nbc_maxits=10000, dbc_maxits=10000, potential="auto"):
+                 dbc_rtol=1e-5, nbc_maxits=10000, dbc_maxits=10000, potential="auto",
+                 compress_eps=0.0):
         self._create(list(meshes), list(materials), precision, do_demag, nbc_rtol, dbc_rtol,
-                     nbc_maxits, dbc_maxits, potential)
+                     nbc_maxits, dbc_maxits, potential, compress_eps)
 
     def split(self, a):
         a = np.asarray(a)
