@@ -11,9 +11,11 @@ from .simulation import (Simulation, every, at, get_subfield_from_h5file,
                          get_subfield_positions_from_h5file)
 from .when import never
 from .datafile import read_ndt
+from .hlib import HMatrixSetup, default_hmatrix_setup
 from . import _native
 
 __all__ = ["SI", "mu0", "to_su", "from_su", "MagMaterial", "NmagUserError",
            "uniaxial_anisotropy", "kuhn_cuboid", "cuboid_sizes", "read_nmesh", "write_nmesh",
            "Simulation", "every", "at", "never", "get_subfield_from_h5file",
-           "get_subfield_positions_from_h5file", "read_ndt"]
+           "get_subfield_positions_from_h5file", "read_ndt", "HMatrixSetup",
+           "default_hmatrix_setup"]

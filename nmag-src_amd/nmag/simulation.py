@@ -87,8 +87,13 @@ class Simulation:
         if not 1 <= int(timestepper_krylov_max) <= 500:
             raise NmagUserError("timestepper_krylov_max %s: Krylov bases of 1 to 500 vectors "
                                 "are available" % timestepper_krylov_max)
+        # phi_BEM=HMatrixSetup(...) (main.py:964-981): the compressed demag
+        # operator, to HMatrixSetup's eps per block
+        self._potential, self._compress_eps = "auto", 0.0
         if phi_BEM is not None:
-            raise NmagUserError("H-matrix BEM compression (phi_BEM) is not available on this path")
+            if not hasattr(phi_BEM, "compress_options"):
+                raise NmagUserError("phi_BEM must be an HMatrixSetup")
+            self._potential, self._compress_eps = phi_BEM.compress_options()
         if periodic_bc is not None:
             raise NmagUserError("periodic boundary conditions are not available on this path")
         if temperature is not None:
@@ -180,7 +185,8 @@ class Simulation:
             self._coords_su, self._tets, self._mat.su_coeffs(), precision=self.precision,
             do_demag=self.do_demag, nbc_rtol=kt.get("NBC.rtol", 1e-5),
             dbc_rtol=kt.get("DBC.rtol", 1e-5), nbc_maxits=int(kt.get("NBC.maxits", 10000)),
-            dbc_maxits=int(kt.get("DBC.maxits", 10000)))
+            dbc_maxits=int(kt.get("DBC.maxits", 10000)), potential=self._potential,
+            compress_eps=self._compress_eps)
         an = self._mat.su_anisotropy()
         if an is not None:
             k1, k2, axis = an

@@ -89,3 +89,25 @@ def test_element_to_node_projection_reduces_to_uniform_and_conserves_energy_weig
     V = np.bincount(tets.reshape(-1), weights=np.repeat(vol / 4, 4), minlength=len(coords))
     K = np.random.default_rng(1).uniform(0, 1, len(tets))
     assert abs(V @ (P @ K) - vol @ K) <= 1e-12 * (vol @ K)
+
+
+def test_hmatrix_setup_keeps_the_reference_interface():
+    """HMatrixSetup (interface/nmag/hlib.py:28-165): HCA2 only, its defaults,
+    user values over set_default_params, and the mapping to the compressed
+    route (eps per block)."""
+    from nmag import HMatrixSetup, default_hmatrix_setup
+    p = default_hmatrix_setup.get_params()
+    assert p["eta"] == 2.0 and p["eps"] == 1e-3 and p["eps_aca"] == 1e-7 and p["nmin"] == 30
+    assert default_hmatrix_setup.compress_options() == ("hmatrix", 1e-3)
+    h = HMatrixSetup(eps=1e-7)
+    h.set_default_params(eps=1e-2, nmin=10)
+    assert h.get_params()["eps"] == 1e-7 and h.get_params()["nmin"] == 10
+    assert h.compress_options() == ("hmatrix", 1e-7)
+    with pytest.raises(NmagUserError):
+        HMatrixSetup(algorithm="ACA")
+    with pytest.raises(NmagUserError):
+        HMatrixSetup(foo=1)
+    with pytest.raises(NmagUserError):
+        HMatrixSetup(eta=1.0).compress_options()
+    s = Simulation(phi_BEM=HMatrixSetup(eps=1e-7))
+    assert (s._potential, s._compress_eps) == ("hmatrix", 1e-7)

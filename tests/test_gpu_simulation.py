@@ -108,3 +108,21 @@ def test_saved_averages_fields_and_restart_on_the_device():
     assert sim2.clock.step == sim.clock.step
     sim.set_m(m)
     assert rel_err(sim2.get_subfield("H_demag"), sim.get_subfield("H_demag")) < 1e-9
+
+
+def test_phi_bem_hmatrix_setup_selects_the_compressed_operator():
+    """Simulation(phi_BEM=HMatrixSetup(eps=...)) (interface/nmag/main.py:964-981)
+    runs the H-matrix route: H_demag within the requested tolerance of the
+    oracle's."""
+    from nmag import HMatrixSetup
+    mat = MagMaterial("Py", llg_damping=0.5, **PERMALLOY)
+    coords, tets = kuhn_cuboid(12, 10, 3, 2.5)
+    orc = oracle.OracleMesh(coords, tets)
+    for eps, bar in ((1e-10, 1e-9), (1e-4, 1e-3)):
+        sim = Simulation(phi_BEM=HMatrixSetup(eps=eps))
+        sim.load_mesh((coords, tets), [("Py", mat)], unit_length=SI(1e-9, "m"))
+        sim.set_m(lambda p: [1.0, math.sin(p[0] * 2e8), math.cos(p[1] * 3e8)])
+        assert sim._magnet.hmatrix_potential
+        m = sim.get_subfield("m_Py")
+        ref = orc.fields(orc_params(mat), m, rtol=1e-13)
+        assert rel_err(sim.get_subfield("H_demag") * 1e-6, ref["H_demag"]) < bar, eps
