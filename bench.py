@@ -148,6 +148,12 @@ def run_gpu(prec, route, coords, tets, mat, Hext, states, warmup, steps, rtol, t
         "potential_bytes": g.stat("potential_bytes"),
         "bem_bytes": g.stat("bem_bytes"),
     }
+    if route == "hmatrix":
+        res["hmatrix"] = {k[3:]: g.stat(k) for k in (
+            "hm_eps", "hm_rank_entries", "hm_full_entries", "hm_dense_entries",
+            "hm_lowrank_entries", "hm_lowrank_blocks", "hm_dense_blocks", "hm_max_rank",
+            "hm_fallback_blocks", "hm_build_s", "hm_p1_panels", "hm_p2_panels",
+            "hm_p1_entries", "hm_p2_entries")}
     g.close()
     return res, out
 
@@ -797,6 +803,9 @@ def main():
             "kernel_ms": mixed_dense["kernel_ms"],
             "setup_s": mixed_dense["setup_s"]},
         "headline_route": head_key,
+        # the compressed operator: blocks, ranks, entries per launch
+        "hmatrix": results["mixed/hmatrix"].get("hmatrix") if "mixed/hmatrix" in results
+        else None,
         # host arrays in and out per evaluation (the CVODE-callback form),
         # PCIe copies included; never the headline value
         "host_pointer_evals_per_s": {

@@ -4,17 +4,20 @@
 //
 // Each panel is R = 16 rows by C columns, stored as groups of G columns
 // (G = 16 bytes / entry size) with a group's R x G entries contiguous.  A
-// workgroup of 256 threads takes one panel: thread (r, l), r = tid % 16,
-// reads row r of the groups l, l + 16, ... with one 16-byte non-temporal load
-// each, so a wavefront reads 1 KiB of contiguous memory per step; it multiplies by the group's G consecutive sources (entries of rho in
-// the cluster order, which k_exch_div writes beside rho, or of the phase-1
-// vector t) and accumulates in FP64.  The 16 partial sums of a row
-// meet in LDS in a fixed order (deterministic), and the row's destination
-// receives the total.  Panels are many (about N/16 in phase 2) and of uneven
-// length: one workgroup each, largest first.
+// workgroup of 256 threads takes one work item (a panel, or a chunk of at
+// most kItemGroups groups of one): thread (r, l), r = tid % 16, reads row r
+// of the groups l, l + 16, ... with one 16-byte non-temporal load each, so a
+// wavefront reads 1 KiB of contiguous memory per load.  It multiplies them by
+// the group's G consecutive sources (entries of rho in the cluster order,
+// which k_exch_div writes beside rho, or of the phase-1 vector t) and
+// accumulates in FP64.  The 16 partial sums of a row meet in LDS in a fixed
+// order (deterministic), and the row's destination receives the total.
+// Items are many (about 4,000 and 6,000 at 20k nodes) and of uneven length:
+// one workgroup each, largest first.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
 #include <vector>
 
 #include "nsim_dev_common.h"
@@ -41,48 +44,68 @@ struct Vec16<double> {
   typedef double type __attribute__((ext_vector_type(2)));
 };
 
+// Broadcast of lane e of each quad of lanes (DPP quad_perm [e, e, e, e]) of
+// a double, as its two 32-bit halves.
+template <int E>
+__device__ inline double quad_bcast(double x) {
+  constexpr int ctrl = E * 0x55;
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(x), ctrl, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(x), ctrl, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+
 // Row r of work item d, over the groups cl, cl + 16, ...; RUN: the item's
-// columns are one run of sources (no per-group index to load first).
+// columns are one run of sources, else gs (in LDS) holds each group's first
+// source.  Every group's G sources are G consecutive doubles; lane r of a row
+// loads only the (r mod G)-th of them, and the quad's DPP broadcasts hand each
+// lane the others: a quarter of the source bytes through the vector memory
+// path.  Groups go U = 4 at a time, then 2, then 1 (a short item still has
+// all of a lane's loads in flight at once); the loads of a step are issued
+// before any of its products.
 template <typename T, bool RUN>
 __device__ inline double item_row(const T* __restrict__ data, const PanelDesc& d,
-                                  const int* __restrict__ gsrc,
-                                  const double* __restrict__ xp,
+                                  const int* gs, const double* __restrict__ xp,
                                   const double* __restrict__ t, int r, int cl) {
   constexpr int G = 16 / sizeof(T);
   constexpr int R = kRows;
   constexpr int CL = kThreads / R;
-  constexpr int U = 4;  // groups in flight per thread
   typedef typename Vec16<T>::type V;
   const V* base = reinterpret_cast<const V*>(data + d.off);
-  const int* gs = gsrc + (RUN ? 0 : d.g0);
   const double* run = d.src0 >= 0 ? xp + d.src0 : t + (-d.src0 - 1);
   auto source = [&](int g) -> const double* {
     if (RUN) return run + g * G;
     const int s = gs[g];
     return s >= 0 ? xp + s : t + (-s - 1);
   };
+  const int me = r & (G - 1);
   const int ng = d.ng;
   double acc = 0.0;
-  int g = cl;
-  for (; g + (U - 1) * CL < ng; g += U * CL) {
+  auto step = [&](int g, auto UU) {
+    constexpr int U = decltype(UU)::value;
     V v[U];
-    const double* src[U];
+    double sm[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) v[u] = __builtin_nontemporal_load(base + (size_t)(g + u * CL) * R + r);
+#pragma unroll
+    for (int u = 0; u < U; u++) sm[u] = source(g + u * CL)[me];
+    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int u = 0; u < U; u++) {
-      v[u] = __builtin_nontemporal_load(base + (size_t)(g + u * CL) * R + r);
-      src[u] = source(g + u * CL);
+      acc += (double)v[u][0] * quad_bcast<0>(sm[u]);
+      acc += (double)v[u][1] * quad_bcast<1>(sm[u]);
+      if constexpr (G == 4) {
+        acc += (double)v[u][2] * quad_bcast<2>(sm[u]);
+        acc += (double)v[u][3] * quad_bcast<3>(sm[u]);
+      }
     }
-#pragma unroll
-    for (int u = 0; u < U; u++)
-#pragma unroll
-      for (int e = 0; e < G; e++) acc += (double)v[u][e] * src[u][e];
+  };
+  int g = cl;
+  for (; g + 3 * CL < ng; g += 4 * CL) step(g, std::integral_constant<int, 4>());
+  if (g + CL < ng) {
+    step(g, std::integral_constant<int, 2>());
+    g += 2 * CL;
   }
-  for (; g < ng; g += CL) {
-    const V v = __builtin_nontemporal_load(base + (size_t)g * R + r);
-    const double* src = source(g);
-#pragma unroll
-    for (int e = 0; e < G; e++) acc += (double)v[e] * src[e];
-  }
+  if (g < ng) step(g, std::integral_constant<int, 1>());
   return acc;
 }
 
@@ -106,10 +129,17 @@ __global__ __launch_bounds__(kThreads) void k_panel_gemv(
   constexpr int CL = kThreads / R;
   __shared__ double red[CL][R];
   __shared__ int last;
+  __shared__ int gs[kItemGroups];
   const PanelDesc d = desc[blockIdx.x];
   const int r = threadIdx.x % R, cl = threadIdx.x / R;
-  red[cl][r] = d.g0 < 0 ? item_row<T, true>(data, d, gsrc, xp, t, r, cl)
-                        : item_row<T, false>(data, d, gsrc, xp, t, r, cl);
+  if (d.g0 >= 0) {
+    // the item's group sources, read once into LDS (every row reads them)
+    for (int i = threadIdx.x; i < d.ng; i += kThreads) gs[i] = gsrc[d.g0 + i];
+    __syncthreads();
+    red[cl][r] = item_row<T, false>(data, d, gs, xp, t, r, cl);
+  } else {
+    red[cl][r] = item_row<T, true>(data, d, nullptr, xp, t, r, cl);
+  }
   __syncthreads();
   double sum = 0.0;
   if (threadIdx.x < R) {
@@ -154,6 +184,9 @@ hipError_t upload_vec(T** dst, const std::vector<T>& v, std::vector<void*>* trac
 
 hipError_t upload_panels(const HPanels& P, int type, int num_cus, PanelOp& op,
                          std::vector<void*>* track) {
+  // the kernel stages an item's group sources in LDS: at most kItemGroups
+  for (const PanelDesc& d : P.desc)
+    if (d.ng < 0 || d.ng > kItemGroups) return hipErrorInvalidValue;
   op.type = type;
   op.panels = P.panels();
   op.entries = P.data.size();

@@ -18,7 +18,7 @@ constexpr int kLeafMax = 128;  // largest leaf cluster
 constexpr int kAlign = 64;     // cluster sizes in whole multiples of this
 constexpr int kProbes = 6;     // Gaussian probe vectors of the block check
 constexpr int kRows = 16;      // rows per panel
-constexpr int kChunk = 256;    // column groups per work item (64 KiB of entries)
+constexpr int kChunk = kItemGroups;  // column groups per work item
 
 struct Cluster {
   int b = 0, e = 0;  // range of the cluster in the tree's node order

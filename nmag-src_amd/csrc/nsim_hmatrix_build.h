@@ -38,6 +38,10 @@
 
 namespace nsim {
 
+// Column groups of one work item at most: longer panels are cut into chunks
+// of this many groups (64 KiB of FP32 entries).
+constexpr int kItemGroups = 256;
+
 // One work item: a panel, or a run of at most 256 of its column groups (a
 // chunk).  Its entries start at data[off]; its ng column groups' sources are
 // gsrc[g0 ...] (or, when the panel's columns are one run of sources, src0 +
