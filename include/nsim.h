@@ -137,6 +137,20 @@ int nsim_traj_adjoint(nsim_handle* h, const nsim_traj* tr, const double* amps,
  * used to vary Ms in gradient checks. */
 int nsim_set_material(nsim_handle* h, int magnet, const nsim_material* mat);
 
+/* Polynomial anisotropy of magnet k (k = -1: every magnet): the energy
+ * density E(m) = sum_j coeffs[j] mx^p0 my^p1 mz^p2, powers[3j..3j+2] = p,
+ * in simulation units (0..256 terms, powers 0..16; nterms 0 clears it).  The
+ * field is H_anis = -(g - m (m.g)) / (mu0 Ms), g = grad E, added to the
+ * uniaxial field of the K1/K2/axis fields when both are set.  Replaces the
+ * LAM equations that nsim_anisotropy_equations builds from the material's
+ * fitted polynomial (src/nsim_anisotropy.ml:216-234, called from
+ * interface/nmag/nmag_lam.py:482-501); the fit itself is the caller's
+ * (interface/nmag/anisotropy.py: uniaxial, cubic or any function with an
+ * order).  Like the reference's default (do_anisotropy_jacobian = False,
+ * nmag_lam.py:133), nsim_jtimes leaves it out of J. */
+int nsim_set_anisotropy(nsim_handle* h, int magnet, int nterms, const int* powers,
+                        const double* coeffs);
+
 /* Autotuner (no counterpart in the reference, whose PETSc kernels have no
  * tile shapes): times the candidates on the device and keeps the fastest.
  * what bit 0: the BEM GEMV tile shape (1/2/4 rows streamed together per

@@ -17,6 +17,7 @@
 #include <cstdint>
 
 #include "nsim_adjoint.h"
+#include "nsim_anis.h"
 #include "nsim_dev_common.h"
 #include "nsim_device.h"
 
@@ -72,8 +73,10 @@ __global__ __launch_bounds__(256) void k_adj_local(int n, const Coeffs* __restri
     const double gm = dot3(gv, m), gt = dot3(gv, ax) - u * gm;  // g.(a - u m)
     for (int d = 0; d < 3; d++) av[d] += (qp * gt - q * gm) * ax[d] - q * u * gv[d];
     if (acc.k1bar) acc.k1bar[row] += 2.0 * u * inv * gt;
-    for (int d = 0; d < 3; d++) ha[d] = bf.Hanis[3 * row + d];
   }
+  if (f.poly) poly_field_vjp(f, inst ? inst[row] : 0, m, 1.0 / (c.mu0 * c.Ms), gv, av);
+  if (f.k1 || f.poly)
+    for (int d = 0; d < 3; d++) ha[d] = bf.Hanis[3 * row + d];
   if (acc.msbar) {
     // H_exch and H_anis scale as 1/Ms, H_demag as Ms
     double s = 0.0;

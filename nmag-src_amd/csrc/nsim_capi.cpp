@@ -156,6 +156,10 @@ int setup_cg(nsim_handle* h, int rpt) {
   return 0;
 }
 
+}  // namespace
+
+namespace nsim {
+
 FieldArgs field_args(nsim_handle* h, const double* m) {
   FieldArgs f;
   f.m = m;
@@ -164,12 +168,10 @@ FieldArgs field_args(nsim_handle* h, const double* m) {
   f.k2 = (h->have_anis && h->have_k2) ? h->k2 : nullptr;
   f.axis = h->have_anis ? h->axis : nullptr;
   f.pin = h->have_pin ? h->pin : nullptr;
+  f.poly = h->poly;
+  f.poly_off = h->poly ? h->poly_off : nullptr;
   return f;
 }
-
-}  // namespace
-
-namespace nsim {
 
 // Full field evaluation for magnetisation m (device) into dmdt (device).
 namespace {
@@ -195,7 +197,7 @@ hipError_t dense_rhs_launches(nsim_handle* h, const double* m, double* dmdt, boo
 
 bool same_args(const FieldArgs& a, const FieldArgs& b) {
   return a.m == b.m && a.Hext == b.Hext && a.k1 == b.k1 && a.k2 == b.k2 && a.axis == b.axis &&
-         a.pin == b.pin;
+         a.pin == b.pin && a.poly == b.poly && a.poly_off == b.poly_off;
 }
 
 // Replays the captured evaluation for (m, dmdt) or captures it first.  Graph
@@ -654,6 +656,59 @@ int nsim_set_material(nsim_handle* h, int magnet, const nsim_material* mat) {
   return refresh_coefficients(h);
 }
 
+int nsim_set_anisotropy(nsim_handle* h, int magnet, int nterms, const int* powers,
+                        const double* coeffs) {
+  if (!h) return fail("nsim_set_anisotropy: null handle");
+  if (magnet < -1 || magnet >= h->d.ninst) return fail("nsim_set_anisotropy: no such magnet");
+  if (nterms < 0 || nterms > kMaxAnisTerms)
+    return fail("nsim_set_anisotropy: 0 to " + std::to_string(kMaxAnisTerms) + " terms");
+  if (nterms > 0 && (!powers || !coeffs)) return fail("nsim_set_anisotropy: null argument");
+  std::vector<AnisTerm> terms(nterms);
+  for (int j = 0; j < nterms; j++) {
+    for (int d = 0; d < 3; d++) {
+      if (powers[3 * j + d] < 0 || powers[3 * j + d] > kMaxAnisPower)
+        return fail("nsim_set_anisotropy: powers must be 0 to " +
+                    std::to_string(kMaxAnisPower));
+      terms[j].p[d] = powers[3 * j + d];
+    }
+    terms[j].pad = 0;
+    if (!std::isfinite(coeffs[j])) return fail("nsim_set_anisotropy: coefficient not finite");
+    terms[j].c = coeffs[j];
+  }
+  h->poly_terms.resize(h->d.ninst);
+  for (int k = 0; k < h->d.ninst; k++)
+    if (magnet < 0 || k == magnet) h->poly_terms[k] = terms;
+  // one table of every magnet's terms, offsets per magnet
+  std::vector<AnisTerm> all;
+  std::vector<int> off(1, 0);
+  for (const auto& t : h->poly_terms) {
+    all.insert(all.end(), t.begin(), t.end());
+    off.push_back((int)all.size());
+  }
+  // the old table may still be read by queued launches
+  if (hipStreamSynchronize(h->stream) != hipSuccess) return fail("nsim_set_anisotropy: sync");
+  drop_rhs_graphs(h);
+  if (h->poly) (void)hipFree(h->poly);
+  if (h->poly_off) (void)hipFree(h->poly_off);
+  h->poly = nullptr;
+  h->poly_off = nullptr;
+  if (all.empty()) return 0;
+  if (hipMalloc(&h->poly, sizeof(AnisTerm) * all.size()) != hipSuccess ||
+      hipMalloc(&h->poly_off, sizeof(int) * off.size()) != hipSuccess ||
+      hipMemcpy(h->poly, all.data(), sizeof(AnisTerm) * all.size(), hipMemcpyHostToDevice) !=
+          hipSuccess ||
+      hipMemcpy(h->poly_off, off.data(), sizeof(int) * off.size(), hipMemcpyHostToDevice) !=
+          hipSuccess) {
+    if (h->poly) (void)hipFree(h->poly);
+    if (h->poly_off) (void)hipFree(h->poly_off);
+    h->poly = nullptr;
+    h->poly_off = nullptr;
+    h->poly_terms.clear();
+    return fail("nsim_set_anisotropy: device upload failed");
+  }
+  return 0;
+}
+
 int nsim_get_batch_layout(nsim_handle* h, int* node_off, int* boundary_off) {
   if (!h) return fail("nsim_get_batch_layout: null handle");
   if (node_off) std::copy(h->node_off.begin(), h->node_off.end(), node_off);
@@ -674,6 +729,8 @@ void nsim_destroy(nsim_handle* h) {
     }
   for (void* p : h->allocations) (void)hipFree(p);
   for (void* p : h->cg_allocs) (void)hipFree(p);
+  if (h->poly) (void)hipFree(h->poly);
+  if (h->poly_off) (void)hipFree(h->poly_off);
   for (DenseOp* op : {&h->d.bem, &h->d.pot}) {
     if (op->wg_seg) (void)hipFree(op->wg_seg);
     if (op->wg_row0) (void)hipFree(op->wg_row0);

@@ -136,6 +136,14 @@ struct CgWork {
   bool cooperative = false;
 };
 
+// One term c mx^p[0] my^p[1] mz^p[2] of a polynomial anisotropy energy
+// density (simulation units).
+struct AnisTerm {
+  int p[3];
+  int pad;
+  double c;
+};
+
 // Field inputs of one evaluation (device pointers; optional ones may be null).
 struct FieldArgs {
   const double* m;
@@ -144,6 +152,10 @@ struct FieldArgs {
   const double* k2;
   const double* axis;
   const double* pin;
+  // polynomial anisotropy: magnet k's terms are poly[poly_off[k] ...
+  // poly_off[k + 1]) (null: none on any magnet)
+  const AnisTerm* poly;
+  const int* poly_off;
 };
 
 struct FieldBufs {

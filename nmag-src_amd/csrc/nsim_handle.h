@@ -65,6 +65,11 @@ struct nsim_handle {
          *phi1b = nullptr;
   double *io_a = nullptr, *io_b = nullptr, *io_c = nullptr;  // staging for host-pointer calls
   bool have_hext = false, have_anis = false, have_k2 = false, have_pin = false;
+  // polynomial anisotropy (nsim_set_anisotropy): each magnet's terms, and
+  // their device table with per-magnet offsets (null: none)
+  std::vector<std::vector<nsim::AnisTerm>> poly_terms;
+  nsim::AnisTerm* poly = nullptr;
+  int* poly_off = nullptr;
   long rhs_evals = 0;
   bool rhs_graph = true;               // replay captured dense-route evaluations
   std::vector<nsim::RhsGraph> graphs;  // at most kMaxRhsGraphs, oldest first
@@ -102,6 +107,11 @@ namespace nsim {
 // Memory the dense potential operators of a handle may take.
 constexpr double kPotentialBudgetBytes = 32.0 * 1024 * 1024 * 1024;
 
+// Polynomial anisotropy: terms per magnet and the largest power of a
+// component (order 16 is the reference's practical limit, ~153 terms).
+constexpr int kMaxAnisTerms = 256;
+constexpr int kMaxAnisPower = 16;
+
 // Captured evaluations kept per handle (one per (m, dmdt) buffer pair in use).
 // A pair is captured on its second evaluation, so callers that stream through
 // many buffers once each (a sweep over stored states) stay on eager launches.
@@ -110,6 +120,8 @@ constexpr size_t kMaxRhsSeen = 16;
 
 // Shared by the C-ABI translation units.
 int device_rhs(nsim_handle* h, const double* m, double* dmdt);
+// The field inputs of an evaluation at m (device) from the handle's fields.
+FieldArgs field_args(nsim_handle* h, const double* m);
 // Destroys the captured evaluations; called whenever a launch shape changes.
 void drop_rhs_graphs(nsim_handle* h);
 // Builds the dense demag potential operator of every magnet (nsim_dense.cpp).

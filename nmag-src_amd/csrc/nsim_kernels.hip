@@ -17,6 +17,7 @@
 
 #include "nsim_dev_common.h"
 #include "nsim_device.h"
+#include "nsim_anis.h"
 
 namespace nsim {
 
@@ -115,7 +116,8 @@ __global__ __launch_bounds__(256) void k_exch_div(int n, const int* __restrict__
 // barycentric gradients of a tetrahedron sum to zero), so this is the same
 // field, and with FP32-stored G it stays blind to the arbitrary constant of
 // the Neumann potential;
-// H_anis from the uniaxial tangential gradient (src/nsim_anisotropy.ml:167-234);
+// H_anis from the uniaxial tangential gradient (src/nsim_anisotropy.ml:167-234)
+// and the polynomial anisotropy of the magnet (nsim_anis.h);
 // H_total (nmag_lam.py:377-411); dm/dt (nmag_lam.py:417-430) times pin.
 template <typename TV>
 __global__ __launch_bounds__(256) void k_demag_llg(int n, const int* __restrict__ off,
@@ -160,6 +162,9 @@ __global__ __launch_bounds__(256) void k_demag_llg(int n, const int* __restrict_
     const double k2 = f.k2 ? f.k2[row] : 0.0;
     const double sc = (2.0 * f.k1[row] * uu + 4.0 * k2 * uu * uu * uu) / (c.mu0 * c.Ms);
     for (int d = 0; d < 3; d++) ha[d] = sc * (ax[d] - uu * m[d]);
+  }
+  if (f.poly) poly_field(f, inst ? inst[row] : 0, m, 1.0 / (c.mu0 * c.Ms), ha);
+  if (f.k1 || f.poly) {
     bf.Hanis[3 * row] = ha[0];
     bf.Hanis[3 * row + 1] = ha[1];
     bf.Hanis[3 * row + 2] = ha[2];

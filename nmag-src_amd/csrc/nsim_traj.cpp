@@ -97,16 +97,7 @@ int ensure_vjp(nsim_handle* h) {
   return 0;
 }
 
-FieldArgs field_args_of(nsim_handle* h, const double* m) {
-  FieldArgs f;
-  f.m = m;
-  f.Hext = h->have_hext ? h->Hext : nullptr;
-  f.k1 = h->have_anis ? h->k1 : nullptr;
-  f.k2 = (h->have_anis && h->have_k2) ? h->k2 : nullptr;
-  f.axis = h->have_anis ? h->axis : nullptr;
-  f.pin = h->have_pin ? h->pin : nullptr;
-  return f;
-}
+FieldArgs field_args_of(nsim_handle* h, const double* m) { return field_args(h, m); }
 
 // out = J(y)^T lam, plus the parameter accumulations of `acc`.  Evaluates the
 // right-hand side at y first, so the handle's fields are those of y.

@@ -6,6 +6,7 @@ advance_time, relax, get_subfield.  All compute runs in libnsim_hip.so.
 """
 from .units import SI, mu0, to_su, from_su
 from .material import MagMaterial, NmagUserError, uniaxial_anisotropy
+from .anisotropy import PredefinedAnisotropy, cubic_anisotropy
 from .mesh import kuhn_cuboid, cuboid_sizes, read_nmesh, write_nmesh
 from .simulation import (Simulation, every, at, get_subfield_from_h5file,
                          get_subfield_positions_from_h5file)
@@ -15,7 +16,7 @@ from .hlib import HMatrixSetup, default_hmatrix_setup
 from . import _native
 
 __all__ = ["SI", "mu0", "to_su", "from_su", "MagMaterial", "NmagUserError",
-           "uniaxial_anisotropy", "kuhn_cuboid", "cuboid_sizes", "read_nmesh", "write_nmesh",
+           "uniaxial_anisotropy", "cubic_anisotropy", "PredefinedAnisotropy", "kuhn_cuboid", "cuboid_sizes", "read_nmesh", "write_nmesh",
            "Simulation", "every", "at", "never", "get_subfield_from_h5file",
            "get_subfield_positions_from_h5file", "read_ndt", "HMatrixSetup",
            "default_hmatrix_setup"]

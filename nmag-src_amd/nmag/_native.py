@@ -84,6 +84,9 @@ SIGNATURES = {
     "nsim_autotune": (ctypes.c_int, [_vp, ctypes.c_int]),
     "nsim_get_potential": (ctypes.c_int, [_vp, _dp]),
     "nsim_set_material": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(NsimMaterial)]),
+    "nsim_set_anisotropy": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int,
+                                           ctypes.POINTER(ctypes.c_int),
+                                           ctypes.POINTER(ctypes.c_double)]),
     "nsim_rhs_vjp": (ctypes.c_int, [_vp, ctypes.c_double, _dp, _dp, _dp, _dp, _dp, _dp]),
     "nsim_traj_forward": (ctypes.c_int, [_vp, ctypes.POINTER(NsimTraj), _dp]),
     "nsim_traj_adjoint": (ctypes.c_int, [_vp, ctypes.POINTER(NsimTraj), _dp, _dp, _dp, _dp,
@@ -339,6 +342,17 @@ class NativeMagnet:
         mat = NsimMaterial(*[float(material[k]) for k in ("Ms", "negJ", "c1", "c2", "c3", "mu0")])
         _check(lib().nsim_set_material(self._h, int(magnet), ctypes.byref(mat)),
                "nsim_set_material")
+
+    def set_anisotropy(self, powers, coeffs, magnet=-1):
+        """Polynomial anisotropy energy sum_j coeffs[j] m^powers[j] (simulation
+        units) of one magnet (-1: all); empty arrays clear it."""
+        P = np.ascontiguousarray(powers, dtype=np.int32).reshape(-1, 3)
+        c = np.ascontiguousarray(coeffs, dtype=np.float64).reshape(-1)
+        if len(P) != len(c):
+            raise ValueError("one exponent triple per coefficient")
+        _check(lib().nsim_set_anisotropy(
+            self._h, int(magnet), len(c), P.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+            c.ctypes.data_as(ctypes.POINTER(ctypes.c_double))), "nsim_set_anisotropy")
 
     def rhs_vjp(self, y, lam, t=0.0):
         """J(y)^T lam and the parameter cotangents (K1 per node, Ms, H_ext)."""

@@ -1,37 +1,14 @@
-"""MagMaterial and the predefined uniaxial anisotropy.
+"""MagMaterial: a material's coefficients in simulation units.
 
-Mirrors interface/nmag/main.py:340-471 (coefficients in simulation units) and
-interface/nmag/anisotropy.py:92-136 (uniaxial_anisotropy).
+Mirrors interface/nmag/material.py:20-230 and interface/nmag/main.py:423-471.
 """
 import numpy as np
 
+from .anisotropy import (NmagUserError, PredefinedAnisotropy, fit_polynomial, su_energy,
+                         uniaxial_anisotropy)
 from .units import SI, mu0, to_su
 
-
-class NmagUserError(Exception):
-    """User error, as raised by the reference front end."""
-
-
-class PredefinedAnisotropy:
-    def __init__(self, anis_type, function, order, axis1=None, K1=None, K2=None):
-        self.anis_type = anis_type
-        self.function = function
-        self.order = order
-        self.axis1 = axis1
-        self.K1 = K1
-        self.K2 = K2
-
-
-def uniaxial_anisotropy(axis, K1, K2=0):
-    """E_anis = -K1 <axis,m>^2 - K2 <axis,m>^4 (interface/nmag/anisotropy.py:92-136)."""
-    axis = np.asarray(axis, dtype=np.float64)
-    axis = axis / np.linalg.norm(axis)
-
-    def f(m):
-        a = float(np.dot(axis, m))
-        return -K1 * a ** 2 - K2 * a ** 4
-
-    return PredefinedAnisotropy("uniaxial", f, 4 if K2 else 2, axis1=axis, K1=K1, K2=K2)
+__all__ = ["MagMaterial", "NmagUserError", "uniaxial_anisotropy"]
 
 
 class MagMaterial:
@@ -47,20 +24,29 @@ class MagMaterial:
         self.do_precession = do_precession
         self.exchange_coupling = exchange_coupling
         self.properties = list(properties)
+        # interface/nmag/material.py:175-196: a PredefinedAnisotropy carries
+        # its order; a custom energy function a(m) needs anisotropy_order
         if isinstance(anisotropy, PredefinedAnisotropy):
             if anisotropy_order:
                 raise NmagUserError("Cannot specify custom anisotropy_order when using a "
                                     "predefined anisotropy.")
-            if anisotropy.anis_type != "uniaxial":
-                raise NmagUserError("only uniaxial anisotropy is implemented on this path")
+            if not anisotropy.has_function():
+                raise NmagUserError("the anisotropy has no energy function")
             self.anisotropy = anisotropy
             self.anisotropy_order = anisotropy.order
-        elif anisotropy is not None:
-            raise NmagUserError("custom anisotropy functions are not supported on this path; "
-                                "use uniaxial_anisotropy(axis, K1, K2)")
+        elif anisotropy is not None or anisotropy_order is not None:
+            if anisotropy is None or not callable(anisotropy):
+                raise NmagUserError("anisotropy must be a PredefinedAnisotropy or a function "
+                                    "of the magnetisation direction")
+            if not anisotropy_order:
+                raise NmagUserError("You need to specify the anisotropy_order when using a "
+                                    "custom anisotropy function.")
+            self.anisotropy = PredefinedAnisotropy(anisotropy, anisotropy_order)
+            self.anisotropy_order = anisotropy_order
         else:
             self.anisotropy = None
             self.anisotropy_order = None
+        self._poly = None
 
         # simulation-unit coefficients (interface/nmag/main.py:423-471)
         self.su_Ms = float(to_su(Ms, "A/m"))
@@ -86,9 +72,21 @@ class MagMaterial:
                 "mu0": self.su_mu0}
 
     def su_anisotropy(self):
-        """(K1_su, K2_su, axis) of a uniaxial anisotropy, or None."""
-        if self.anisotropy is None:
-            return None
+        """(K1_su, K2_su, axis) of a uniaxial anisotropy (the device's closed
+        form, per-node K1 and K2), or None for none or another kind."""
         a = self.anisotropy
+        if a is None or a.anis_type != "uniaxial":
+            return None
         return (float(to_su(a.K1, "J/m^3")), float(to_su(a.K2, "J/m^3")) if a.K2 else 0.0,
                 np.asarray(a.axis1, dtype=np.float64))
+
+    def su_anisotropy_poly(self):
+        """(powers, coeffs) of any other anisotropy: the homogeneous polynomial
+        of its order equal to its energy density (simulation units) on the unit
+        sphere, or None."""
+        a = self.anisotropy
+        if a is None or a.anis_type == "uniaxial":
+            return None
+        if self._poly is None:
+            self._poly = fit_polynomial(su_energy(a.function), a.order)
+        return self._poly

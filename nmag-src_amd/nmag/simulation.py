@@ -18,6 +18,7 @@ import numpy as np
 
 from . import _native
 from . import datafile
+from .anisotropy import poly_energy
 from .material import MagMaterial, NmagUserError
 from .mesh import read_nmesh
 from .units import SI, su_factor, to_su
@@ -195,6 +196,9 @@ class Simulation:
             self._magnet.set_field("K1", np.full(n, k1))
             if k2:
                 self._magnet.set_field("K2", np.full(n, k2))
+        poly = self._mat.su_anisotropy_poly()
+        if poly is not None:
+            self._magnet.set_anisotropy(*poly)
         self._volumes = self._magnet.get_field("volumes")
         return self
 
@@ -493,13 +497,14 @@ class Simulation:
         e = {"E_demag": -0.5 * mu0 * Ms * (m * f["H_demag"]).sum(1),
              "E_exch": -0.5 * mu0 * Ms * (m * f["H_exch"]).sum(1),
              "E_ext": -1.0 * mu0 * Ms * (m * f["H_ext"]).sum(1)}
-        an = self._mat.su_anisotropy()
+        an, poly = self._mat.su_anisotropy(), self._mat.su_anisotropy_poly()
+        e["E_anis"] = np.zeros(len(m))
         if an is not None:
             k1, k2, axis = an
             u = m @ axis
             e["E_anis"] = -k1 * u ** 2 - k2 * u ** 4
-        else:
-            e["E_anis"] = np.zeros(len(m))
+        if poly is not None:
+            e["E_anis"] = e["E_anis"] + poly_energy(poly[0], poly[1], m)
         e["E_total"] = e["E_demag"] + e["E_exch"] + e["E_ext"] + e["E_anis"]
         return e
 

@@ -59,8 +59,20 @@ def lib():
         L.orc_cg.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_void_p, ctypes.c_double,
                              ctypes.c_int, _dp]
         L.orc_jtimes.argtypes = [ctypes.c_void_p, ctypes.POINTER(OrcParams)] + [_dp] * 4
+        L.orc_poly_anis.argtypes = [ctypes.c_int, _dp, ctypes.c_int, _ip, _dp, ctypes.c_double,
+                                    _dp]
         _lib = L
     return _lib
+
+
+def poly_anis(m, powers, coeffs, s):
+    """H = -s (g - m (m.g)), g the gradient of sum_j c_j m^p_j (orc_poly_anis)."""
+    m = np.ascontiguousarray(m, dtype=np.float64).reshape(-1, 3)
+    P = np.ascontiguousarray(powers, dtype=np.int32).reshape(-1, 3)
+    c = np.ascontiguousarray(coeffs, dtype=np.float64).reshape(-1)
+    H = np.zeros_like(m)
+    lib().orc_poly_anis(len(m), _p(m), len(c), _p(P, np.int32), _p(c), float(s), _p(H))
+    return H
 
 
 def _p(a, dtype=np.float64):
@@ -172,10 +184,20 @@ class OracleMesh:
         return x, it, res.value
 
     def fields(self, params, m, H_ext=None, k1=None, k2=None, axis=None, rtol=1e-13,
-               maxit=100000, do_demag=True, phi1=None, phi2=None):
-        """All effective-field contributions and dm/dt for magnetisation m (n,3)."""
+               maxit=100000, do_demag=True, phi1=None, phi2=None, poly=None):
+        """All effective-field contributions and dm/dt for magnetisation m (n,3).
+        poly = (powers (k,3), coeffs (k,)): a polynomial anisotropy energy
+        (SU), its field added to H_anis and H_total."""
         n = self.n
         m = np.ascontiguousarray(m, dtype=np.float64).reshape(n, 3)
+        if poly is not None:
+            Hp = poly_anis(m, poly[0], poly[1], 1.0 / (params.mu0 * params.Ms))
+            He0 = np.zeros((n, 3)) if H_ext is None else np.broadcast_to(
+                np.asarray(H_ext, dtype=np.float64), (n, 3))
+            out = self.fields(params, m, He0 + Hp, k1, k2, axis, rtol, maxit, do_demag,
+                              phi1, phi2)
+            out["H_anis"] = out["H_anis"] + Hp
+            return out
         out = {k: np.zeros((n, 3)) for k in ("H_exch", "H_demag", "H_anis", "H_total", "dmdt")}
         out["rho"] = np.zeros(n)
         p1 = np.zeros(n) if phi1 is None else np.array(phi1, dtype=np.float64)

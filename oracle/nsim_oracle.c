@@ -595,6 +595,39 @@ int orc_fields(const orc_mesh* M, const orc_params* p, const double* mag,
   return 0;
 }
 
+/* Polynomial anisotropy (src/nsim_anisotropy.ml:151-234): the energy density
+ * E(m) = sum_j c_j mx^px_j my^py_j mz^pz_j, homogeneous of degree `order`
+ * (the fit of taylor_coefficients_of_spherical_function, :119-133); the field
+ * is minus the unit-sphere tangential gradient,
+ *   H = -s (g - m (m.g)),  g = grad E(m),  s = 1/(mu0 Ms)
+ * (unit_sphere_tangential_polypotential_gradient, :167-197, with the
+ * correction factor 1/(mu0 Ms) of interface/nmag/nmag_lam.py:490-497).
+ * Adds H to Ha (n x 3). */
+static double ipow(double x, int k) {
+  double r = 1.0;
+  for (int i = 0; i < k; i++) r *= x;
+  return r;
+}
+
+void orc_poly_anis(int n, const double* mag, int nterms, const int* powers,
+                   const double* coeffs, double s, double* Ha) {
+  for (int i = 0; i < n; i++) {
+    const double* m = mag + 3 * i;
+    double g[3] = {0.0, 0.0, 0.0};
+    for (int j = 0; j < nterms; j++) {
+      const int* p = powers + 3 * j;
+      for (int d = 0; d < 3; d++) {
+        if (p[d] == 0) continue;
+        double t = coeffs[j] * p[d];
+        for (int e = 0; e < 3; e++) t *= ipow(m[e], e == d ? p[e] - 1 : p[e]);
+        g[d] += t;
+      }
+    }
+    const double mg = dot3(m, g);
+    for (int d = 0; d < 3; d++) Ha[3 * i + d] += -s * (g[d] - m[d] * mg);
+  }
+}
+
 void orc_jtimes(const orc_mesh* M, const orc_params* p, const double* mag,
                 const double* H, const double* v, double* Jv) {
   int n = M->n;

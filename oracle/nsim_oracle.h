@@ -92,6 +92,9 @@ int orc_cg(const orc_mesh* m, const double* b, double* x, const unsigned char* f
 
 /* Jacobian-times-vector of the LLG rhs with H_total frozen except for the
  * exchange part (dH/dm = negJ K / V), as in the reference's jacobi_ts1. */
+/* Polynomial anisotropy field, added to Ha (src/nsim_anisotropy.ml:151-234). */
+void orc_poly_anis(int n, const double* mag, int nterms, const int* powers,
+                   const double* coeffs, double s, double* Ha);
 void orc_jtimes(const orc_mesh* m, const orc_params* p, const double* mag,
                 const double* H_total, const double* v, double* Jv);
 
