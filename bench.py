@@ -5,9 +5,10 @@ Kuhn-split cubes (80 x 40 x 5 cubes, 96,000 tets, 19,926 nodes, 7,602 boundary
 nodes), alpha = 0.02, 25 mT applied at 170 deg in-plane, seeded S-state.  One
 step = one full right-hand-side evaluation y -> H_exch + H_demag + H_ext ->
 dm/dt on a sequence of magnetisation states along a trajectory.  The demag is
-timed on both routes: the dense potential operator (phi = M rho, one GEMV; the
-headline) and the reference's structure (two warm-started PCG solves around
-the BEM GEMV).  Inputs are resident in HBM before timing starts.
+timed on three routes: the potential operator M compressed as an H-matrix
+(phi = M rho in two launches; the headline, FP32-mixed), the same operator
+dense (one GEMV), and the reference's structure (two warm-started PCG solves
+around the BEM GEMV).  Inputs are resident in HBM before timing starts.
 
 Further legs: sim-ns/hour of the BDF integrator, the 64-magnet batch against
 64 sequential handles, the differentiation form (adjoint of a 2 ns RK4
@@ -150,7 +151,7 @@ def run_gpu(prec, route, coords, tets, mat, Hext, states, warmup, steps, rtol, t
     }
     if route == "hmatrix":
         res["hmatrix"] = {k[3:]: g.stat(k) for k in (
-            "hm_eps", "hm_rank_entries", "hm_full_entries", "hm_dense_entries",
+            "hm_eps", "hm_eta", "hm_rank_entries", "hm_full_entries", "hm_dense_entries",
             "hm_lowrank_entries", "hm_lowrank_blocks", "hm_dense_blocks", "hm_max_rank",
             "hm_fallback_blocks", "hm_build_s", "hm_p1_panels", "hm_p2_panels",
             "hm_p1_entries", "hm_p2_entries")}

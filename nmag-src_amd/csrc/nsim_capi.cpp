@@ -428,11 +428,14 @@ int create_impl(const char* who, int ninst, const nsim_mesh* meshes,
     h->opt.nbc_maxits = h->opt.dbc_maxits = 10000;
     h->opt.potential = 0;
     h->opt.compress_eps = 0.0;
+    h->opt.compress_eta = 0.0;
   }
   if (h->opt.potential > 2)
     return cleanup(fail(me + ": unknown potential route " + std::to_string(h->opt.potential)));
   if (!(h->opt.compress_eps >= 0.0) || h->opt.compress_eps >= 1.0)
     return cleanup(fail(me + ": compress_eps must be in [0, 1)"));
+  if (!(h->opt.compress_eta >= 0.0) || h->opt.compress_eta > 100.0)
+    return cleanup(fail(me + ": compress_eta must be in [0, 100]"));
   if (h->opt.precision < NSIM_FP64 || h->opt.precision > NSIM_MIXED_F24)
     return cleanup(fail(me + ": unknown precision"));
   if (h->opt.nbc_maxits <= 0) h->opt.nbc_maxits = 10000;
@@ -1109,7 +1112,7 @@ int nsim_get_stat(nsim_handle* h, const char* name, double* out) {
     const double pot_bytes = h->use_hmatrix ? (double)(hm.p1.bytes + hm.p2.bytes)
                                             : (double)dense_bytes(p.type, p.elems);
     const std::map<std::string, double> simple = {
-        {"hm_eps", hm.eps}, {"hm_rank_entries", (double)hm.nt},
+        {"hm_eps", hm.eps}, {"hm_eta", hm.eta}, {"hm_rank_entries", (double)hm.nt},
         {"hm_full_entries", (double)hm.full_entries},
         {"hm_dense_entries", (double)hm.dense_entries},
         {"hm_lowrank_entries", (double)hm.lowrank_entries},

@@ -74,6 +74,7 @@ struct HMatrixOp {
   int* pos = nullptr;       // node -> cluster-order position (null: no H-matrix)
   double* xp = nullptr;     // rho in cluster order, written by k_exch_div
   double eps = 0.0;        // relative tolerance per block
+  double eta = 0.0;        // admissibility parameter
   const int* halt = nullptr;
   long long full_entries = 0, dense_entries = 0, lowrank_entries = 0;
   int lowrank_blocks = 0, dense_blocks = 0, max_rank = 0, fallback_blocks = 0;

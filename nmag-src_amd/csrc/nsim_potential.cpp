@@ -129,6 +129,7 @@ int build_potential(nsim_handle* h) {
   const double hm_eps = h->opt.compress_eps > 0.0 ? h->opt.compress_eps
                         : hm_type == kBemF64      ? 1e-12
                                                   : 1e-7;
+  const double hm_eta = h->opt.compress_eta > 0.0 ? h->opt.compress_eta : 3.0;
   HMatrixHost H;
   H.group = hm_type == kBemF32 ? 4 : 2;
   std::vector<double> hostP;
@@ -254,7 +255,7 @@ int build_potential(nsim_handle* h) {
       HCHK(hipStreamSynchronize(st), "sync");
       const auto t0 = std::chrono::steady_clock::now();
       const std::string err = hmatrix_compress(N, hostP.data(), N, &o.x[3 * (size_t)o0], o0,
-                                               hm_eps, 2.0, hm_threads, H);
+                                               hm_eps, hm_eta, hm_threads, H);
       hm_seconds +=
           std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
       if (!err.empty()) return fail("potential set-up: " + err);
@@ -275,6 +276,7 @@ int build_potential(nsim_handle* h) {
     HMatrixOp& op = d.hm;
     HCHK(upload_hmatrix(H, hm_type, h->num_cus, op, &h->allocations), "H-matrix upload");
     op.eps = hm_eps;
+    op.eta = hm_eta;
     op.full_entries = H.full_entries;
     op.dense_entries = H.dense_entries;
     op.lowrank_entries = H.lowrank_entries;

@@ -34,7 +34,8 @@ class NsimOptions(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int), ("do_demag", ctypes.c_int),
                 ("nbc_rtol", ctypes.c_double), ("dbc_rtol", ctypes.c_double),
                 ("nbc_maxits", ctypes.c_int), ("dbc_maxits", ctypes.c_int),
-                ("potential", ctypes.c_int), ("compress_eps", ctypes.c_double)]
+                ("potential", ctypes.c_int), ("compress_eps", ctypes.c_double),
+                ("compress_eta", ctypes.c_double)]
 
 
 POTENTIAL = {"auto": 0, "dense": 1, "iterative": -1, "hmatrix": 2}
@@ -143,16 +144,17 @@ class NativeMagnet:
 
     def __init__(self, coords, tets, material, precision="fp64", do_demag=True,
                  nbc_rtol=1e-5, dbc_rtol=1e-5, nbc_maxits=10000, dbc_maxits=10000,
-                 potential="auto", compress_eps=0.0):
+                 potential="auto", compress_eps=0.0, compress_eta=0.0):
         """potential: "iterative" (the two Poisson solves around the BEM, as
         the reference), "dense" (the precomputed potential operator), "hmatrix"
-        (the same operator compressed to compress_eps per block; 0: the
-        precision's default) or "auto" (dense when it fits)."""
+        (the same operator compressed to compress_eps per block with
+        admissibility compress_eta; 0: the defaults) or "auto" (dense when it
+        fits)."""
         self._create([(coords, tets)], [material], precision, do_demag, nbc_rtol, dbc_rtol,
-                     nbc_maxits, dbc_maxits, potential, compress_eps)
+                     nbc_maxits, dbc_maxits, potential, compress_eps, compress_eta)
 
     def _create(self, meshes, materials, precision, do_demag, nbc_rtol, dbc_rtol, nbc_maxits,
-                dbc_maxits, potential="auto", compress_eps=0.0):
+                dbc_maxits, potential="auto", compress_eps=0.0, compress_eta=0.0):
         L = lib()
         self._h = None
         if len(meshes) != len(materials) or not meshes:
@@ -172,7 +174,7 @@ class NativeMagnet:
         prec = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
         pot = POTENTIAL[potential] if isinstance(potential, str) else int(potential)
         opt = NsimOptions(prec, 1 if do_demag else 0, nbc_rtol, dbc_rtol, nbc_maxits, dbc_maxits,
-                          pot, float(compress_eps))
+                          pot, float(compress_eps), float(compress_eta))
         h = _vp()
         if len(meshes) == 1:
             _check(L.nsim_create(cmeshes, cmats, ctypes.byref(opt), ctypes.byref(h)),
@@ -419,9 +421,9 @@ class NativeBatch(NativeMagnet):
 
     def __init__(self, meshes, materials, precision="fp64", do_demag=True, nbc_rtol=1e-5,
                  dbc_rtol=1e-5, nbc_maxits=10000, dbc_maxits=10000, potential="auto",
-                 compress_eps=0.0):
+                 compress_eps=0.0, compress_eta=0.0):
         self._create(list(meshes), list(materials), precision, do_demag, nbc_rtol, dbc_rtol,
-                     nbc_maxits, dbc_maxits, potential, compress_eps)
+                     nbc_maxits, dbc_maxits, potential, compress_eps, compress_eta)
 
     def split(self, a):
         a = np.asarray(a)

@@ -90,11 +90,11 @@ class Simulation:
                                 "are available" % timestepper_krylov_max)
         # phi_BEM=HMatrixSetup(...) (main.py:964-981): the compressed demag
         # operator, to HMatrixSetup's eps per block
-        self._potential, self._compress_eps = "auto", 0.0
+        self._potential, self._compress_eps, self._compress_eta = "auto", 0.0, 0.0
         if phi_BEM is not None:
             if not hasattr(phi_BEM, "compress_options"):
                 raise NmagUserError("phi_BEM must be an HMatrixSetup")
-            self._potential, self._compress_eps = phi_BEM.compress_options()
+            self._potential, self._compress_eps, self._compress_eta = phi_BEM.compress_options()
         if periodic_bc is not None:
             raise NmagUserError("periodic boundary conditions are not available on this path")
         if temperature is not None:
@@ -187,7 +187,7 @@ class Simulation:
             do_demag=self.do_demag, nbc_rtol=kt.get("NBC.rtol", 1e-5),
             dbc_rtol=kt.get("DBC.rtol", 1e-5), nbc_maxits=int(kt.get("NBC.maxits", 10000)),
             dbc_maxits=int(kt.get("DBC.maxits", 10000)), potential=self._potential,
-            compress_eps=self._compress_eps)
+            compress_eps=self._compress_eps, compress_eta=self._compress_eta)
         an = self._mat.su_anisotropy()
         if an is not None:
             k1, k2, axis = an

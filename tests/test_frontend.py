@@ -98,16 +98,17 @@ def test_hmatrix_setup_keeps_the_reference_interface():
     from nmag import HMatrixSetup, default_hmatrix_setup
     p = default_hmatrix_setup.get_params()
     assert p["eta"] == 2.0 and p["eps"] == 1e-3 and p["eps_aca"] == 1e-7 and p["nmin"] == 30
-    assert default_hmatrix_setup.compress_options() == ("hmatrix", 1e-3)
+    assert default_hmatrix_setup.compress_options() == ("hmatrix", 1e-3, 0.0)
     h = HMatrixSetup(eps=1e-7)
     h.set_default_params(eps=1e-2, nmin=10)
     assert h.get_params()["eps"] == 1e-7 and h.get_params()["nmin"] == 10
-    assert h.compress_options() == ("hmatrix", 1e-7)
+    assert h.compress_options() == ("hmatrix", 1e-7, 0.0)
+    assert HMatrixSetup(eta=2.0).compress_options() == ("hmatrix", 1e-3, 2.0)
     with pytest.raises(NmagUserError):
         HMatrixSetup(algorithm="ACA")
     with pytest.raises(NmagUserError):
         HMatrixSetup(foo=1)
     with pytest.raises(NmagUserError):
-        HMatrixSetup(eta=1.0).compress_options()
+        HMatrixSetup(eta=-1.0).compress_options()
     s = Simulation(phi_BEM=HMatrixSetup(eps=1e-7))
-    assert (s._potential, s._compress_eps) == ("hmatrix", 1e-7)
+    assert (s._potential, s._compress_eps, s._compress_eta) == ("hmatrix", 1e-7, 0.0)

@@ -73,7 +73,7 @@ def slab_fields():
         g.execute()
         out[(prec, route)] = {k: g.get_field(k) for k in ("H_exch", "H_demag", "dmdt")}
         if route == "hmatrix":
-            out["stats"] = {k: g.stat(k) for k in ("potential_bytes", "hm_eps", "hm_max_rank",
+            out["stats"] = {k: g.stat(k) for k in ("potential_bytes", "hm_eps", "hm_eta", "hm_max_rank",
                                                     "hm_lowrank_blocks", "hm_p1_panels",
                                                     "hm_p2_panels")}
         g.close()
@@ -86,7 +86,7 @@ def test_mixed_hmatrix_error_at_the_slab(slab_fields):
     H_demag, 1.8e-8 dm/dt measured)."""
     ref, got = slab_fields[("fp64", "dense")], slab_fields[("mixed", "hmatrix")]
     st = slab_fields["stats"]
-    assert st["hm_eps"] == 1e-7
+    assert st["hm_eps"] == 1e-7 and st["hm_eta"] == 3.0
     assert st["potential_bytes"] < 0.3 * 4 * 19926 ** 2  # under 30 % of the FP32 dense bytes
     assert rel_err(got["H_exch"], ref["H_exch"]) < 1e-7
     assert rel_err(got["H_demag"], ref["H_demag"]) < 3e-7
@@ -166,6 +166,8 @@ def test_hmatrix_integration_follows_the_dense_fp64_route():
 def test_options_are_checked():
     coords, tets = kuhn_cuboid(2, 2, 1, 2.5)
     for kw in ({"potential": 3}, {"potential": "hmatrix", "compress_eps": 1.5},
-               {"potential": "hmatrix", "compress_eps": -1.0}):
+               {"potential": "hmatrix", "compress_eps": -1.0},
+               {"potential": "hmatrix", "compress_eta": -1.0},
+               {"potential": "hmatrix", "compress_eta": 1e3}):
         with pytest.raises(Exception):
             NativeMagnet(coords, tets, material().su_coeffs(), **kw)
