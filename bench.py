@@ -75,7 +75,13 @@ def graphs(g):
 # cannot be read inside a timed run, so the figure is reported for this
 # workload only.
 MEASURED_TRAFFIC = {(4, 19926): 777846 * 2048 + 179 * 1024,
-                    (8, 19926): 1554955 * 2048 + 185 * 1024}
+                    (8, 19926): 1554955 * 2048 + 185 * 1024,
+                    # both k_panel_gemv<float> launches of the compressed
+                    # operator (eps 1e-7, eta 3: 292,995,808 stored bytes;
+                    # profiles/potential_hmatrix_c2_slab.md); any other
+                    # operator gets null
+                    ("hmatrix", 19926, 292995808): int((48021.4 + 99293.5) * 2048
+                                                       + (777.8 + 843.7) * 1024)}
 KERNELS = {"dense": ("exch_div", "potential", "demag_llg"),
            "hmatrix": ("exch_div", "potential", "demag_llg"),
            "iterative": ("exch_div", "cg_nbc", "bem", "cg_dbc", "demag_llg")}
@@ -532,8 +538,10 @@ def main():
     ap.add_argument("--no-dynamics", action="store_true")
     ap.add_argument("--dynamics-routes", default="dense,hmatrix",
                     help="demag routes timed in the dynamics leg")
+    ap.add_argument("--dense-precs", default="fp64,mixed,mixed24",
+                    help="precisions timed on the dense route (fp64 first)")
     ap.add_argument("--routes", default="dense,iterative,hmatrix",
-                    help="demag routes timed in the field-evaluation leg (dense first)")
+                    help="demag routes timed in the field-evaluation leg")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default tile shapes (short profiler runs)")
     ap.add_argument("--no-batch", action="store_true",
@@ -555,8 +563,12 @@ def main():
     global AUTOTUNE, GRAPHS
     AUTOTUNE = not args.no_autotune
     GRAPHS = not args.no_graph
-    if not args.routes.startswith("dense"):
-        print("--routes must start with dense (the headline route)", file=sys.stderr)
+    if not args.dense_precs.startswith("fp64") or not set(args.dense_precs.split(",")) <= {
+            "fp64", "mixed", "mixed24"}:
+        print("--dense-precs: fp64 first, then any of mixed, mixed24", file=sys.stderr)
+        sys.exit(2)
+    if not args.routes or not set(args.routes.split(",")) <= {"dense", "iterative", "hmatrix"}:
+        print("--routes: any of dense, iterative, hmatrix", file=sys.stderr)
         sys.exit(2)
     if args.gpus != 1:
         print("bench.py measures one GPU: the library has no multi-GPU path "
@@ -599,7 +611,7 @@ def main():
     # the precisions timed per route: the 24-bit mode differs from FP32-mixed
     # only in the dense operator; the compressed operator is timed with FP32
     # entries (in FP64 at 1e-12 per block it compresses too little to pay)
-    route_precs = {"dense": ("fp64", "mixed", "mixed24"), "iterative": ("fp64", "mixed"),
+    route_precs = {"dense": tuple(args.dense_precs.split(",")), "iterative": ("fp64", "mixed"),
                    "hmatrix": ("mixed",)}
     for route in ("dense", "iterative", "hmatrix"):
         if route not in args.routes.split(","):
@@ -649,7 +661,8 @@ def main():
         rooflines["potential"] = roofline(
             "k_panel_gemv<float> x2 H-matrix potential operator (N=%d, %.0f MB)" % (
                 n, mixed["potential_bytes"] / 1e6),
-            mixed["potential_bytes"] + 16 * n, kern["potential"], None)
+            mixed["potential_bytes"] + 16 * n, kern["potential"],
+            MEASURED_TRAFFIC.get(("hmatrix", n, int(mixed["potential_bytes"]))))
     if mixed_dense:
         rooflines["potential_dense" if head_key == "mixed/hmatrix" else "potential"] = roofline(
             "k_dense_gemv<float> potential operator (N=%d)" % n, pot_bytes(4),
