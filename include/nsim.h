@@ -267,7 +267,11 @@ int nsim_ts_set_tols(nsim_handle* h, double ksp_rtol, double ksp_atol, double ks
  * (re)initialises the device BDF integrator from the handle's m at t0 with
  * scalar tolerances (CVodeReInit semantics: history is discarded).
  * max_order <= 0 keeps 2, krylov_max <= 0 keeps 300 (main.py:838-839).
- * max_order above 2 and krylov_max above 500 are errors (status < 0). */
+ * BDF orders 1 to 5 (CVODE's maxord): with max_order 1 or 2 the order rises
+ * to it as the history allows; with 3 to 5 the order is chosen step by step
+ * among q - 1, q, q + 1 from local-error estimates, as CVODE does
+ * (nsim_set_tuning "ts_fixed_order" 1 makes it rise to max_order instead).
+ * max_order above 5 and krylov_max above 500 are errors (status < 0). */
 int nsim_ts_init(nsim_handle* h, double t0, double rtol, double atol, int max_order,
                  int krylov_max);
 /* Replaces lam_ts_advance (src/nsimexec.ml:4130-4146 -> src/nsim.ml:2537-2580):
@@ -283,7 +287,8 @@ int nsim_advance(nsim_handle* h, double t_final, int max_steps, int exact_tstop,
  * integrator (cvode_get_stats, src/sundials_sp_stubs.c:1409-1528):
  * "ts_steps", "ts_rhs", "ts_newton", "ts_linear", "ts_err_fails",
  * "ts_conv_fails", "ts_h", "ts_order", "ts_t", "ts_max_dm" (max_i |m_i(t1) -
- * m_i(t0)| over the last nsim_advance, as mwe_norm_dist_fun). */
+ * m_i(t0)| over the last nsim_advance, as mwe_norm_dist_fun), and
+ * "ts_order_steps_<q>" (accepted steps at order q = 1..5 since ts_init). */
 int nsim_get_stat(nsim_handle* h, const char* name, double* out);
 
 /* Per-kernel device timers, the analogue of the reference's TSTART/TSTOP
@@ -292,7 +297,9 @@ int nsim_get_stat(nsim_handle* h, const char* name, double* out);
  * "demag_llg"; out[0] = total ms, out[1] = launches. */
 int nsim_timers_enable(nsim_handle* h, int on);
 /* Launch-shape knobs (used by the tile autotuner); key "cg_rows_per_thread"
- * takes 1, 2, 4 or 8.  No counterpart in the reference (PETSc picks its own). */
+ * takes 1, 2, 4 or 8.  No counterpart in the reference (PETSc picks its own).
+ * Integrator keys: "ts_fixed_order" (see nsim_ts_init), "gmres_device",
+ * "gmres_pc_sweeps", "gmres_slices_per_wg", "cooperative_launch". */
 int nsim_set_tuning(nsim_handle* h, const char* key, int value);
 int nsim_get_timer(nsim_handle* h, const char* name, double* out);
 int nsim_synchronize(nsim_handle* h);

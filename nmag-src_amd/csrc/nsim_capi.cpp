@@ -782,6 +782,11 @@ int nsim_set_tuning(nsim_handle* h, const char* key, int value) {
     if (h->ts) h->ts->options().device_gmres = h->ts_opt.device_gmres;
     return 0;
   }
+  if (k == "ts_fixed_order") {
+    h->ts_opt.fixed_order = value != 0;
+    if (h->ts) h->ts->options().fixed_order = h->ts_opt.fixed_order;
+    return 0;
+  }
   if (k == "gmres_pc_sweeps") {
     if (value < 0 || value > 8) return fail("gmres_pc_sweeps must be 0..8");
     h->ts_opt.gmres_pc_sweeps = value;
@@ -1086,6 +1091,9 @@ int nsim_get_stat(nsim_handle* h, const char* name, double* out) {
     else if (s == "ts_conv_fails") *out = (double)t.conv_fails;
     else if (s == "ts_h") *out = t.h_last;
     else if (s == "ts_order") *out = t.q_last;
+    else if (s.rfind("ts_order_steps_", 0) == 0 && s.size() == 16 && s[15] >= '1' &&
+             s[15] <= '0' + kMaxOrder)
+      *out = (double)t.order_steps[s[15] - '0'];
     else if (s == "ts_t") *out = t.t;
     else if (s == "ts_max_dm") *out = t.max_dm;
     else if (s == "ts_gmres_grid") *out = h->ts->gmres_grid();
@@ -1171,11 +1179,11 @@ int nsim_ts_init(nsim_handle* h, double t0, double rtol, double atol, int max_or
                  int krylov_max) {
   if (!h) return fail("nsim_ts_init: null handle");
   if (!(rtol >= 0.0) || !(atol > 0.0)) return fail("nsim_ts_init: need rtol >= 0, atol > 0");
-  // BDF orders 1 and 2 (the reference's default maximum, main.py:838) are
-  // built; Krylov bases up to 500 vectors
-  if (max_order > 2)
+  // BDF orders 1 to 5 (CVODE's maxord range; the reference's default is 2,
+  // main.py:838); Krylov bases up to 500 vectors
+  if (max_order > kMaxOrder)
     return fail("nsim_ts_init: max_order " + std::to_string(max_order) +
-                " not available (BDF orders 1 and 2 are)");
+                " not available (BDF orders 1 to 5 are)");
   if (krylov_max > 500)
     return fail("nsim_ts_init: krylov_max " + std::to_string(krylov_max) +
                 " above the largest Krylov basis (500)");

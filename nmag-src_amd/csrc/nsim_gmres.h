@@ -26,7 +26,7 @@ struct GmresWork {
   unsigned long long* tag = nullptr;  // 2 * gmax phase tags, kGmresTagStride apart
   double* R = nullptr;        // maxl * maxl triangular factor (workgroup 0)
   double* y = nullptr;        // maxl: the least-squares solution, broadcast
-  double* status = nullptr;   // [0] iterations [1] sum (x ewt)^2 [2] error flag
+  double* status = nullptr;   // [0] iterations [1] sum (x ewt)^2 [2] error flag, [3..5] sums
   unsigned* err = nullptr;    // exchange error flag
   int gmax = 0, pstride = 0;
   int resident = 0;              // workgroups the device holds at once (gmres_grid)
@@ -65,13 +65,16 @@ struct NewtonSystem {
   // controller's step on this Newton iteration's results (ts_control), so no
   // separate controller launch is needed
   TsState* ts = nullptr;
+  double* const* slots = nullptr;  // device table of the controller's history ring
 };
 
 // Solves (I - gamma J) x = avec + gamma f - ycur by right-preconditioned
 // GMRES in the space scaled by ewt (J the jacobi_ts1 linearisation at ycur)
 // to |residual| <= tol (scaled Euclidean) or maxl iterations, then sets
 // ycur += x.  Writes w.status: [0] iterations, [1] sum (x ewt)^2,
-// [2] != 0 if an exchange timed out, [3] sum ((ycur - yp) ewt)^2.
+// [2] != 0 if an exchange timed out, [3] sum ((ycur - yp) ewt)^2,
+// [4], [5] the sums of squares of the controller's order q - 1 and q + 1
+// error estimates (0 unless ts asks for them).
 // Refuses a grid above w.resident.
 hipError_t launch_newton_gmres(const DeviceOps& d, bool mixed, GmresWork& w, int grid, int spw,
                                int maxl, const NewtonSystem& s, hipStream_t st);

@@ -81,6 +81,7 @@ struct GmresArgs {
   unsigned long long epoch;
   const double* gamma_p;  // if set, gamma is read here (set by the device controller)
   const int* halt;        // nonzero word: every workgroup returns at once
+  double* const* slots;  // the controller's history ring (read with ts)
   // If set, workgroup 0 runs the step controller on the results at the end.
   // Every workgroup reads gamma and halt (fields of this record) at its start,
   // and workgroup 0 reaches the end only after the last exchange, to which
@@ -297,15 +298,25 @@ __device__ void local_dots(const GmresArgs<TV>& a, GShared& sm, const double* v,
   }
 }
 
-// The step controller as a call rather than inlined code: inlined, its
-// registers would count against the whole kernel and lower its occupancy.
+// The step controller as a call on the record in place: inlined, or on a
+// register copy of the record, its registers would count against the whole
+// kernel.  The kernel's occupancy target (three waves per SIMD) makes the
+// controller spill what does not fit instead; it runs on one thread.
+// (Measured at the slab: in place 88.3 us per Newton iteration, a copy in LDS
+// 88.7-90.8 us, a separate one-thread launch 83.8 + 9.9 us.)
 __device__ __attribute__((noinline)) void run_controller(TsState* s, double its, double xw2,
-                                                         double bad, double errsq, int n3) {
-  ts_control(s, its, xw2, bad, errsq, n3);
+                                                         double bad, double errsq,
+                                                         double errsq_m, double errsq_p,
+                                                         int n3) {
+  ts_control_at(*s, its, xw2, bad, errsq, errsq_m, errsq_p, n3);
 }
 
 template <typename TV>
-__global__ __launch_bounds__(256) void k_gmres(GmresArgs<TV> a) {
+// Three waves per SIMD: what the LDS allows (see GShared) and what the
+// exchanges want; the step controller called at the end must fit the same
+// register budget.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) void k_gmres(
+    GmresArgs<TV> a) {
   if (a.halt && *a.halt) return;  // queued by a finished integrator run
   if (a.gamma_p) a.gamma = *a.gamma_p;
   __shared__ GShared sm;
@@ -510,8 +521,11 @@ __global__ __launch_bounds__(256) void k_gmres(GmresArgs<TV> a) {
   }
   if (ok) {
     // x = Z (y zs); the Newton update ycur += x, with the weighted sums of
-    // squares of the correction and of the distance from the predictor
-    double sx = 0.0, se = 0.0;
+    // squares of the correction, of the distance from the predictor and of
+    // the controller's order q - 1 and q + 1 error estimates (combinations of
+    // the new iterate and the history, weights set by the controller)
+    const int em = a.ts ? a.ts->em : 0, ep = a.ts ? a.ts->ep : 0;
+    double sx = 0.0, se = 0.0, sem = 0.0, sep = 0.0;
     for (int e = e0 + tid; e < e1; e += 256) {
       double xv = 0.0;
       for (int i = 0; i < k; i++) xv += sm.tot[i] * load_coh(z_rsrc(a, i), e);
@@ -526,21 +540,41 @@ __global__ __launch_bounds__(256) void k_gmres(GmresArgs<TV> a) {
       dy += -1.0 * a.yp[e];
       const double v = dy * w;
       se += v * v;
+      if (em > 0) {
+        double t = a.ts->Em[0] * yv;
+        for (int i = 1; i <= em; i++) t += a.ts->Em[i] * a.slots[a.ts->sh[i - 1]][e];
+        t *= w;
+        sem += t * t;
+      }
+      if (ep > 0) {
+        double t = a.ts->Ep[0] * yv;
+        for (int i = 1; i <= ep; i++) t += a.ts->Ep[i] * a.slots[a.ts->sh[i - 1]][e];
+        t *= w;
+        sep += t * t;
+      }
     }
     block_sum2(sx, se, sm.red);
+    block_sum2(sem, sep, sm.red);
     if (tid == 0) {
       sm.loc[0] = sx;
       sm.loc[1] = se;
+      sm.loc[2] = sem;
+      sm.loc[3] = sep;
     }
-    ok = exchange(a, sm, ++ph, 2, true);
+    ok = exchange(a, sm, ++ph, 4, true);
   }
   if (blockIdx.x == 0 && tid == 0) {
     const double xw2 = ok ? sm.tot[0] : NAN, errsq = ok ? sm.tot[1] : NAN;
+    const double errsq_m = ok ? sm.tot[2] : NAN, errsq_p = ok ? sm.tot[3] : NAN;
     a.status[0] = (double)k;
     a.status[1] = xw2;
     a.status[2] = ok ? 0.0 : 1.0;
     a.status[3] = errsq;
-    if (a.ts) run_controller(a.ts, (double)k, xw2, ok ? 0.0 : 1.0, errsq, (int)a.ld);
+    a.status[4] = errsq_m;
+    a.status[5] = errsq_p;
+    if (a.ts)
+      run_controller(a.ts, (double)k, xw2, ok ? 0.0 : 1.0, errsq, errsq_m, errsq_p,
+                     (int)a.ld);
   }
 }
 
@@ -557,7 +591,7 @@ hipError_t launch_typed(const DeviceOps& d, const TV* K, GmresWork& w, int grid,
                   s.ewt,   s.kappa,   s.Minv,   s.gamma,  s.tol,   s.avec,    s.f,
                   s.yp,    s.x,       w.V,      w.Z,      3 * (size_t)d.n,    w.part,
                   w.tag,   w.R,       w.y,      w.status, w.err,   ++w.epoch,
-                  s.gamma_p, s.halt, s.ts};
+                  s.gamma_p, s.halt, s.slots, s.ts};
   void* args[] = {&a};
   if (w.cooperative)
     return hipLaunchCooperativeKernel(kernel_ptr<TV>(), dim3(grid), dim3(256), args, 0, st);

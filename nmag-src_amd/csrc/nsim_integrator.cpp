@@ -2,7 +2,7 @@
 //
 // All vectors stay in HBM.  The step controller (nsim_tsctl.h) is the same
 // code on host and device:
-//  * BDF of order q <= 2 on the actual (variable) step history: the corrector
+//  * BDF of order q <= 5 on the actual (variable) step history: the corrector
 //    is P'(t_{n+1}) = f(y_{n+1}) for the polynomial P through y_{n+1..n+1-q}.
 //  * predictor = extrapolation of the polynomial through y_{n..n-q} (explicit
 //    Euler on the first step).  The local error is estimated by Milne's device,
@@ -45,7 +45,7 @@ Integrator::Integrator(nsim_handle* h, const TsOptions& o) : h_(h), opt_(o) {
   nn_ = handle_n(h);
   n3_ = 3 * nn_;
   opt_.krylov_max = std::max(1, std::min(opt_.krylov_max, 500));
-  opt_.max_order = std::max(1, std::min(opt_.max_order, 2));
+  opt_.max_order = std::max(1, std::min(opt_.max_order, kMaxOrder));
   auto alloc = [&](double** p, size_t count) {
     if (hipMalloc(reinterpret_cast<void**>(p), count * sizeof(double)) != hipSuccess) return false;
     allocs_.push_back(*p);
@@ -53,7 +53,9 @@ Integrator::Integrator(nsim_handle* h, const TsOptions& o) : h_(h), opt_(o) {
   };
   const size_t n = n3_;
   np_ = vec_reduce_grid(n3_);
-  ok_ = alloc(&slot_[0], n) && alloc(&slot_[1], n) && alloc(&slot_[2], n) && alloc(&fn_, n) &&
+  ok_ = true;
+  for (int i = 0; i < kHist && ok_; i++) ok_ = alloc(&slot_[i], n);
+  ok_ = ok_ && alloc(&fn_, n) &&
         alloc(&yp_, n) && alloc(&ycur_, n) && alloc(&avec_, n) && alloc(&f_, n) &&
         alloc(&res_, n) && alloc(&del_, n) && alloc(&tmp_, n) && alloc(&tmp2_, n) &&
         alloc(&ewt_, n) && alloc(&ystart_, n) && alloc(&status_, 8) &&
@@ -70,8 +72,9 @@ Integrator::Integrator(nsim_handle* h, const TsOptions& o) : h_(h), opt_(o) {
     ok_ = hipMalloc(reinterpret_cast<void**>(&ds_), sizeof(TsState)) == hipSuccess &&
           hipMemset(ds_, 0, sizeof(TsState)) == hipSuccess &&
           hipHostMalloc(reinterpret_cast<void**>(&pin_), 3 * sizeof(TsState)) == hipSuccess &&
-          hipMalloc(reinterpret_cast<void**>(&slots_dev_), 3 * sizeof(double*)) == hipSuccess &&
-          hipMemcpy(slots_dev_, slot_, 3 * sizeof(double*), hipMemcpyHostToDevice) ==
+          hipMalloc(reinterpret_cast<void**>(&slots_dev_), kHist * sizeof(double*)) ==
+              hipSuccess &&
+          hipMemcpy(slots_dev_, slot_, kHist * sizeof(double*), hipMemcpyHostToDevice) ==
               hipSuccess &&
           hipEventCreateWithFlags(&ev_[0], hipEventDisableTiming) == hipSuccess &&
           hipEventCreateWithFlags(&ev_[1], hipEventDisableTiming) == hipSuccess;
@@ -148,13 +151,15 @@ int Integrator::init(double t0) {
                       s_));
   HTRY(hipStreamSynchronize(s_));
   hs_ = TsState{};
-  hs_.tn = hs_.tnm1 = hs_.tnm2 = t0;
-  hs_.sn = 0;
-  hs_.snm1 = 1;
-  hs_.snm2 = 2;
+  for (int i = 0; i < kHist; i++) {
+    hs_.th[i] = t0;
+    hs_.sh[i] = i;
+  }
   hs_.hnext = 0.0;
   hs_.qnext = 1;
+  hs_.qwait = 2;
   hs_.max_order = opt_.max_order;
+  hs_.select = opt_.max_order >= 3 && !opt_.fixed_order;
   hs_.crate = 1.0;
   hs_.gamma_crate = 0.0;
   hs_.copy = kSlotNone;
@@ -298,6 +303,7 @@ int Integrator::enqueue_cycle() {
   sys.tol = kLinCoef * kNewtonCoef * std::sqrt((double)n3_);
   sys.sweeps = opt_.gmres_pc_sweeps;
   sys.ts = ds_;
+  sys.slots = slots_dev_;
   gw_.cooperative = opt_.cooperative;
   HTRY(launch_newton_gmres(handle_ops(h_), handle_mixed(h_), gw_, ggrid_, gspw_,
                            opt_.krylov_max, sys, s_));
@@ -315,7 +321,7 @@ int Integrator::run_device_cycles() {
     double left = 1.0;
     if (m.max_steps > 0) left = m.max_steps - m.steps_call;
     if (m.h > 0.0) {
-      const double steps = std::ceil((m.t_final - m.tn) / m.h);
+      const double steps = std::ceil((m.t_final - m.th[0]) / m.h);
       left = m.max_steps > 0 ? std::min(left, steps) : steps;
     }
     return (int)std::min<double>(maxb, std::max(1.0, left));
@@ -344,6 +350,22 @@ int Integrator::run_device_cycles() {
   }
 }
 
+// WRMS norm of W[0] ycur + sum_{i=1..cnt} W[i] y_{n+1-i} (0 when cnt is 0).
+int Integrator::estimate(int cnt, const double* W, double* out) {
+  *out = 0.0;
+  if (cnt <= 0) return 0;
+  HTRY(vec_lincomb(s_, n3_, W[0], ycur_, 0, nullptr, 0, nullptr, 0, nullptr, tmp_));
+  for (int i = 1; i <= cnt; i += 3) {
+    const double* y1 = slot_[hs_.sh[i - 1]];
+    const double* y2 = i + 1 <= cnt ? slot_[hs_.sh[i]] : nullptr;
+    const double* y3 = i + 2 <= cnt ? slot_[hs_.sh[i + 1]] : nullptr;
+    HTRY(vec_lincomb(s_, n3_, 1.0, tmp_, W[i], y1, y2 ? W[i + 1] : 0.0, y2,
+                     y3 ? W[i + 2] : 0.0, y3, tmp_));
+  }
+  *out = wrms(tmp_);
+  return 0;
+}
+
 // Host-driven GMRES: one cycle at a time, the controller's state read back
 // after each.
 int Integrator::run_host_cycles() {
@@ -362,11 +384,17 @@ int Integrator::run_host_cycles() {
     const double d = wrms(del_);
     HTRY(vec_lincomb(s_, n3_, 1.0, ycur_, -1.0, yp_, 0, nullptr, 0, nullptr, tmp_));
     const double e = wrms(tmp_);
+    // the order q - 1 and q + 1 estimates the controller asked for
+    double em = 0.0, ep = 0.0;
+    TRY(estimate(hs_.em, hs_.Em, &em));
+    TRY(estimate(hs_.ep, hs_.Ep, &ep));
     hhost_[0] = its;
     hhost_[1] = d * d * n3_;
     hhost_[2] = 0.0;
     hhost_[3] = e * e * n3_;
-    HTRY(hipMemcpyAsync(status_, hhost_, 4 * sizeof(double), hipMemcpyHostToDevice, s_));
+    hhost_[4] = em * em * n3_;
+    hhost_[5] = ep * ep * n3_;
+    HTRY(hipMemcpyAsync(status_, hhost_, 6 * sizeof(double), hipMemcpyHostToDevice, s_));
     HTRY(launch_ts_ctrl(s_, ds_, status_, n3_));
     TRY(download_state());
     if (hs_.mode == kTsDone || hs_.mode == kTsError) return 0;
@@ -389,15 +417,16 @@ int Integrator::advance(double t_final, int max_steps, bool exact_tstop, double*
   hs_.rtol = opt_.rtol;
   hs_.atol = opt_.atol;
   hs_.max_order = opt_.max_order;
+  hs_.select = opt_.max_order >= 3 && !opt_.fixed_order;
   if (!ts_finished(hs_)) {
-    double* yn = slot_[hs_.sn];
+    double* yn = slot_[hs_.sh[0]];
     if (hs_.hnext <= 0.0) {  // first step: h0 from ||f(y0)||
       HTRY(vec_ewt(s_, n3_, yn, opt_.rtol, opt_.atol, ewt_));
       TRY(device_rhs(h_, yn, fn_));
       hs_.rhs++;
       const double fnorm = wrms(fn_);
       double h0 = fnorm > 0 ? 0.1 / fnorm : 1.0;
-      h0 = std::min(h0, 0.1 * std::fabs(t_final - hs_.tn));
+      h0 = std::min(h0, 0.1 * std::fabs(t_final - hs_.th[0]));
       hs_.hnext = h0 > 0 ? h0 : 1e-6;
       hs_.qnext = 1;
     }
@@ -427,7 +456,8 @@ int Integrator::advance(double t_final, int max_steps, bool exact_tstop, double*
     st_.conv_fails = hs_.conv_fails;
     st_.h_last = hs_.h_last;
     st_.q_last = hs_.q_last;
-    st_.t = hs_.tn;
+    st_.t = hs_.th[0];
+    for (int q = 0; q <= kMaxOrder; q++) st_.order_steps[q] = hs_.order_steps[q];
     if (hs_.mode == kTsError) {
       if (hs_.err_code == 1) {
         (void)hipMemset(gw_.err, 0, sizeof(unsigned));
@@ -435,29 +465,31 @@ int Integrator::advance(double t_final, int max_steps, bool exact_tstop, double*
         return -1;
       }
       set_error("integrator: step size underflow or repeated failures at t=" +
-                std::to_string(hs_.tn));
+                std::to_string(hs_.th[0]));
       return 1;
     }
   }
-  const double* yn = slot_[hs_.sn];
-  if (!exact_tstop && hs_.tn > t_final && hs_.nhist >= 1) {
-    // interpolate the step polynomial back to t_final
-    const int q = hs_.nhist;
-    const double P[3] = {hs_.tn, hs_.tnm1, hs_.tnm2};
-    const double* Y[3] = {yn, slot_[hs_.snm1], slot_[hs_.snm2]};
-    double L[3] = {0, 0, 0};
+  const double* yn = slot_[hs_.sh[0]];
+  if (!exact_tstop && hs_.th[0] > t_final && hs_.nhist >= 1) {
+    // interpolate the polynomial of the last step's order back to t_final
+    const int q = std::min(hs_.nhist, std::max(1, hs_.q_last));
+    double L[kHist] = {};
     for (int j = 0; j <= q; j++) {
       double w = 1.0;
       for (int i = 0; i <= q; i++)
-        if (i != j) w *= (t_final - P[i]) / (P[j] - P[i]);
+        if (i != j) w *= (t_final - hs_.th[i]) / (hs_.th[j] - hs_.th[i]);
       L[j] = w;
     }
-    HTRY(vec_lincomb(s_, n3_, L[0], Y[0], L[1], Y[1], q >= 2 ? L[2] : 0.0,
-                     q >= 2 ? Y[2] : nullptr, 0, nullptr, m));
+    HTRY(vec_lincomb(s_, n3_, L[0], yn, L[1], slot_[hs_.sh[1]], q >= 2 ? L[2] : 0.0,
+                     q >= 2 ? slot_[hs_.sh[2]] : nullptr, q >= 3 ? L[3] : 0.0,
+                     q >= 3 ? slot_[hs_.sh[3]] : nullptr, m));
+    if (q >= 4)
+      HTRY(vec_lincomb(s_, n3_, 1.0, m, L[4], slot_[hs_.sh[4]], q >= 5 ? L[5] : 0.0,
+                       q >= 5 ? slot_[hs_.sh[5]] : nullptr, 0, nullptr, m));
     *t_reached = t_final;
   } else {
     HTRY(hipMemcpyAsync(m, yn, sizeof(double) * n3_, hipMemcpyDeviceToDevice, s_));
-    *t_reached = hs_.tn;
+    *t_reached = hs_.th[0];
   }
   int np = 0;
   HTRY(vec_maxdist_partials(s_, nn_, m, ystart_, part_, &np));

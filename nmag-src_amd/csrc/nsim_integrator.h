@@ -1,4 +1,4 @@
-// Device-resident time integrator: variable-step BDF (order <= 2) with a
+// Device-resident time integrator: variable-step BDF (orders 1 to 5) with a
 // Newton iteration whose linear systems (I - gamma J) delta = -G are solved by
 // scaled GMRES with the analytic jacobi_ts1 J.v and a block-Jacobi right
 // preconditioner.  It stands in for CVODE (BDF + Newton + SPGMR) behind the
@@ -53,6 +53,8 @@ struct TsOptions {
   int gmres_pc_sweeps = 1;          // local Jacobi sweeps of the persistent GMRES's
                                     // preconditioner (0: 3x3 block Jacobi only)
   int max_batch = 16;               // Newton cycles queued per batch
+  bool fixed_order = false;         // orders 3-5: rise to max_order as the history
+                                    // allows instead of choosing the order
   bool cooperative = false;         // persistent GMRES as a cooperative launch
 };
 
@@ -60,6 +62,7 @@ struct TsStats {
   long steps = 0, rhs = 0, newton = 0, linear = 0, err_fails = 0, conv_fails = 0;
   double h_last = 0.0, t = 0.0, max_dm = 0.0;
   int q_last = 0;
+  long long order_steps[kMaxOrder + 1] = {};  // accepted steps per order
 };
 
 class Integrator {
@@ -87,6 +90,7 @@ class Integrator {
   int apply_A(const double* z, double* w);
   double wrms(const double* x);
   int reduce(int nparts, int k, std::vector<double>* out);
+  int estimate(int cnt, const double* W, double* out);
   int check(hipError_t e, const char* what);
 
   nsim_handle* h_;
@@ -99,7 +103,7 @@ class Integrator {
   TsState* ds_ = nullptr;     // its device copy, which the controller kernel updates
   TsState* pin_ = nullptr;    // pinned: [0], [1] batch mirrors, [2] upload staging
   hipEvent_t ev_[2] = {nullptr, nullptr};
-  double* slot_[3] = {nullptr, nullptr, nullptr};  // ring of y_n, y_{n-1}, y_{n-2}
+  double* slot_[kHist] = {};     // ring of y_n, y_{n-1}, ..., y_{n-5}
   double** slots_dev_ = nullptr;                   // the same pointers on the device
   double *fn_ = nullptr;    // f(y_n) (first step's predictor)
   double *yp_ = nullptr, *ycur_ = nullptr, *avec_ = nullptr, *f_ = nullptr, *res_ = nullptr,

@@ -23,25 +23,39 @@ __global__ __launch_bounds__(256) void k_ts_begin(int n3, const TsState* __restr
                                                   double* __restrict__ ewt) {
   if (s->mode != kTsBegin || s->halt) return;
   const int copy = s->copy;
-  const double* yn = slots[s->sn];
+  const double* yn = slots[s->sh[0]];
   double* dst = copy >= 0 ? slots[copy] : nullptr;
-  const double* p1 = src_of(slots, fn, s->p1);
-  const double* p2 = src_of(slots, fn, s->p2);
-  const double* a2v = src_of(slots, fn, s->a2s);
-  const double L0 = s->L[0], L1 = s->L[1], L2 = s->L[2], a1 = s->a1, a2 = s->a2;
+  // index 0 of the predictor and of the history term is y_n itself
+  const double* P[kHist];
+  double L[kHist];
+  const double* Av[kMaxOrder];
+  double A[kMaxOrder];
+#pragma unroll
+  for (int j = 0; j < kHist; j++) {
+    P[j] = j > 0 ? src_of(slots, fn, s->p[j]) : nullptr;
+    L[j] = s->L[j];
+  }
+#pragma unroll
+  for (int j = 0; j < kMaxOrder; j++) {
+    Av[j] = j > 0 ? src_of(slots, fn, s->as[j]) : nullptr;
+    A[j] = s->A[j];
+  }
   const double rtol = s->rtol, atol = s->atol;
   for (int i = blockIdx.x * 256 + threadIdx.x; i < n3; i += gridDim.x * 256) {
     // the accepted iterate becomes y_n (same values as the ring slot gets)
     const double y = dst ? ycur[i] : yn[i];
     if (dst) dst[i] = y;
     ewt[i] = 1.0 / (rtol * fabs(y) + atol);
-    double v = L0 * y;
-    if (p1) v += L1 * p1[i];
-    if (p2) v += L2 * p2[i];
+    double v = L[0] * y;
+#pragma unroll
+    for (int j = 1; j < kHist; j++)
+      if (P[j]) v += L[j] * P[j][i];
     yp[i] = v;
     ycur[i] = v;
-    double a = a1 * y;
-    if (a2v) a += a2 * a2v[i];
+    double a = A[0] * y;
+#pragma unroll
+    for (int j = 1; j < kMaxOrder; j++)
+      if (Av[j]) a += A[j] * Av[j][i];
     avec[i] = a;
   }
 }
@@ -50,7 +64,7 @@ __global__ __launch_bounds__(256) void k_ts_begin(int n3, const TsState* __restr
 // whole, so the loads are in flight together rather than one field at a time).
 __global__ void k_ts_ctrl(TsState* s, const double* __restrict__ status, int n3) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  ts_control(s, status[0], status[1], status[2], status[3], n3);
+  ts_control(s, status[0], status[1], status[2], status[3], status[4], status[5], n3);
 }
 
 }  // namespace

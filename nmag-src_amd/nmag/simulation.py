@@ -82,8 +82,8 @@ class Simulation:
                  do_anisotropy_jacobian=False, temperature=None, thermal_delta_t=None,
                  timestepper_max_order=2, timestepper_krylov_max=300, ksp_tolerances={},
                  adjust_tolerances=False, precision="fp64"):
-        if not 1 <= int(timestepper_max_order) <= 2:
-            raise NmagUserError("timestepper_max_order %s: BDF orders 1 and 2 are available"
+        if not 1 <= int(timestepper_max_order) <= 5:
+            raise NmagUserError("timestepper_max_order %s: BDF orders 1 to 5 are available"
                                 % timestepper_max_order)
         if not 1 <= int(timestepper_krylov_max) <= 500:
             raise NmagUserError("timestepper_krylov_max %s: Krylov bases of 1 to 500 vectors "

@@ -1,8 +1,10 @@
-"""The BDF step controller's failure handling and stop conditions, forced from
-the host: tests/ctl/ts_controller_harness.cpp drives the controller's own
-host functions (nmag-src_amd/csrc/nsim_tsctl.h, the code the device runs in
-its one-thread control kernel) with made-up Newton results.  Every scenario
-must end, and end in the documented state.  CPU only."""
+"""The BDF step controller, driven from the host: tests/ctl/ts_controller_harness.cpp
+runs the controller's own host functions (nmag-src_amd/csrc/nsim_tsctl.h, the
+code the device runs at the end of every Newton iteration) through forced
+failure and stop scenarios with made-up Newton results -- every scenario must
+end, and end in the documented state -- and through whole integrations of a
+linear ODE with exact Newton solves, which pin the BDF orders 1 to 5 and the
+order selection against the exact solution.  CPU only."""
 import json
 import os
 import shutil
@@ -27,8 +29,11 @@ def runs(tmp_path_factory):
                     "-packed-fp32-ops", "-I" + os.path.join(ROOT, "nmag-src_amd", "csrc"),
                     os.path.join(ROOT, "tests", "ctl", "ts_controller_harness.cpp"),
                     "-o", exe], check=True, capture_output=True, timeout=300)
-    out = subprocess.run([exe], check=True, capture_output=True, text=True, timeout=60).stdout
-    return {r["name"]: r for r in map(json.loads, out.splitlines())}
+    out = subprocess.run([exe], check=True, capture_output=True, text=True, timeout=120).stdout
+    rows = [json.loads(line) for line in out.splitlines()]
+    res = {r["name"]: r for r in rows}
+    res["_all"] = rows
+    return res
 
 
 def test_error_test_failures_end_in_an_error(runs):
@@ -59,3 +64,47 @@ def test_two_rejections_drop_the_order(runs):
     # error-test rejections at err >> 1 take h to 0.2 h each
     assert r["h"] == pytest.approx(0.01 * 0.2 * 0.2, rel=1e-12)
     assert r["q_last"] == 1
+
+
+# Whole integrations of y' = A y (a damped rotation and a decay, exact
+# solution known) to t = 10 by the controller with exact Newton solves on the
+# host: BDF of fixed order q, and with order selection up to max_order.
+
+
+def _integrations(runs_all, name):
+    return {(r["max_order"], r["tol"]): r for r in runs_all if r["name"] == name}
+
+
+@pytest.fixture(scope="module")
+def runs_all(runs):
+    return runs["_all"]
+
+
+@pytest.mark.parametrize("q", [1, 2, 3, 4, 5])
+def test_fixed_order_step_counts_and_errors_follow_the_order(runs_all, q):
+    """Local-error control at order q takes h ~ tol^(1/(q+1)): a 100-fold
+    smaller tolerance multiplies the steps by 100^(1/(q+1)) and divides the
+    global error by 100^(q/(q+1))."""
+    fx = _integrations(runs_all, "fixed")
+    for lo, hi in ((1e-6, 1e-8), (1e-8, 1e-10)):
+        a, b = fx[(q, lo)], fx[(q, hi)]
+        assert a["mode"] == DONE == b["mode"] and a["tn"] == 10.0
+        assert a["order_steps"][q - 1] >= 0.9 * a["steps"]
+        steps = b["steps"] / a["steps"]
+        assert 0.8 * 100 ** (1 / (q + 1)) < steps < 1.25 * 100 ** (1 / (q + 1)), (q, steps)
+        gain = a["err"] / b["err"]
+        assert 0.6 * 100 ** (q / (q + 1)) < gain < 1.6 * 100 ** (q / (q + 1)), (q, gain)
+
+
+@pytest.mark.parametrize("q", [3, 4, 5])
+def test_order_selection_climbs_to_the_maximum_order(runs_all, q):
+    sel, fx = _integrations(runs_all, "select"), _integrations(runs_all, "fixed")
+    for tol in (1e-6, 1e-8, 1e-10):
+        r = sel[(q, tol)]
+        assert r["mode"] == DONE and r["q_last"] == q
+        assert r["order_steps"][q - 1] > 0.7 * r["steps"]
+        # as accurate as the fixed order and no more steps
+        assert r["err"] < 1.5 * fx[(q, tol)]["err"]
+        assert r["steps"] <= fx[(q, tol)]["steps"]
+    # order 5 against order 2 at tight tolerance: a fraction of the steps
+    assert sel[(5, 1e-10)]["steps"] < fx[(2, 1e-10)]["steps"] / 20

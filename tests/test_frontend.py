@@ -66,7 +66,7 @@ def test_unsupported_options_fail_loudly():
     with pytest.raises(NmagUserError):
         s.magnet
     with pytest.raises(NmagUserError):
-        Simulation(timestepper_max_order=5)
+        Simulation(timestepper_max_order=6)
     with pytest.raises(NmagUserError):
         Simulation(timestepper_krylov_max=600)
     assert every("step", 5).delta == 5

@@ -1,6 +1,7 @@
 """The device BDF/Newton-GMRES integrator: closed-form macrospin dynamics,
 trajectories against the oracle's rhs integrated by SciPy (DOP853, tight
-tolerances), CVODE-like stop semantics, and the psetup/psolve blocks."""
+tolerances) at BDF orders up to 2 and up to 5, CVODE-like stop semantics, and
+the psetup/psolve blocks."""
 import math
 
 import numpy as np
@@ -134,9 +135,9 @@ def test_stop_semantics_and_statistics():
     assert g.stat("ts_max_dm") > 0
     for k in ("ts_rhs", "ts_newton", "ts_linear"):
         assert g.stat(k) > 0
-    # orders above 2 and Krylov bases above 500 are refused, not clamped
+    # orders above 5 and Krylov bases above 500 are refused, not clamped
     with pytest.raises(Exception, match="max_order"):
-        g.ts_init(0.0, rtol=1e-6, atol=1e-6, max_order=3)
+        g.ts_init(0.0, rtol=1e-6, atol=1e-6, max_order=6)
     with pytest.raises(Exception, match="krylov_max"):
         g.ts_init(0.0, rtol=1e-6, atol=1e-6, krylov_max=501)
 
@@ -250,3 +251,43 @@ def test_persistent_gmres_matches_host_driven_gmres(precision):
         assert np.abs(res["host"][0] - res[key][0]).max() < 1e-6, key
     assert abs(s0["linear"] - res["dev"][1]["linear"]) <= 0.05 * s0["linear"] + 5
     assert res["dev_sweeps"][1]["linear"] < s0["linear"]
+
+
+@pytest.mark.parametrize("device", [1, 0])
+def test_orders_up_to_five_against_scipy(device):
+    """max_order 5 (order selection) on the persistent and the host-driven
+    GMRES: the trajectory matches the oracle's rhs integrated by SciPy, orders
+    3 to 5 are taken, and at rtol 1e-9 it needs fewer steps than order 2."""
+    mat = material(alpha=0.3)
+    coords, tets, g = _magnet(6, 5, 3, 2.5, mat)
+    x, y = coords[:, 0], coords[:, 1]
+    m0 = np.stack([np.ones_like(x), 0.4 * np.sin(x / 7.0), 0.3 * np.cos(y / 5.0)], axis=1)
+    m0 /= np.linalg.norm(m0, axis=1, keepdims=True)
+    Hx = np.tile([0.0, 0.02, 0.0], (len(coords), 1))
+    g.set_field("H_ext", Hx)
+    g.tune("gmres_device", device)
+    orc = oracle.OracleMesh(coords, tets)
+    P = orc_params(mat)
+    state = {"p1": None, "p2": None}
+
+    def f(t, yv):
+        r = orc.fields(P, yv.reshape(-1, 3), H_ext=Hx, rtol=1e-13, phi1=state["p1"],
+                       phi2=state["p2"])
+        state["p1"], state["p2"] = r["phi1"], r["phi2"]
+        return r["dmdt"].reshape(-1)
+
+    T = 10.0
+    sol = solve_ivp(f, (0.0, T), m0.reshape(-1), method="DOP853", rtol=1e-10, atol=1e-12)
+    ref = sol.y[:, -1].reshape(-1, 3)
+    steps = {}
+    for q in (2, 5):
+        g.set_field("m", m0)
+        g.ts_init(0.0, rtol=1e-9, atol=1e-9, max_order=q)
+        assert g.advance(T) == pytest.approx(T, abs=1e-12)
+        assert np.abs(g.get_field("m") - ref).max() < 2e-6, q
+        steps[q] = g.stat("ts_steps")
+        by_order = [g.stat("ts_order_steps_%d" % k) for k in range(1, 6)]
+        assert sum(by_order) == steps[q]
+        if q == 5:
+            assert sum(by_order[2:]) > 0.5 * steps[q], by_order
+    assert steps[5] < 0.8 * steps[2], steps
