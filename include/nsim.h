@@ -136,6 +136,51 @@ int nsim_traj_adjoint(nsim_handle* h, const nsim_traj* tr, const double* amps,
                       const double* m_bar_final, double* m_bar_initial, double* K1_bar,
                       double* Ms_bar, double* amps_bar);
 
+/* ---- The BDF trajectory and its discrete adjoint (no counterpart in the
+ * reference, which has no adjoint; the forward is the reference's stepper,
+ * src/nsim.ml:2537-2580).  A single-magnet handle.
+ *
+ * nsim_bdf_forward integrates from the handle's m at t = 0 to t_final with
+ * the BDF integrator (tolerances, max_order as nsim_ts_init), never stepping
+ * across a pulse edge (the applied field is the handle's H_ext plus
+ * amps[w] * pulse_dir during [w pulse_len, (w+1) pulse_len)), and records
+ * every accepted step: its state Y_{k+1}, step size, order, gamma and
+ * history coefficients, so that Y_{k+1} - gamma f(Y_{k+1}) - sum_j A_j Y_{k-j}
+ * = 0.  All states stay in device memory (max_steps + 1 of 3N doubles).
+ * The handle's m becomes Y_K.
+ *
+ * nsim_bdf_replay re-integrates the recorded step sequence (the same step
+ * sizes, orders and coefficients) from the handle's m with the current
+ * material, anisotropy, H_ext and amps, solving every step's equation by
+ * Newton to an RMS residual below newton_rtol; its states replace the
+ * recorded ones.  With the step sequence fixed the map from parameters to
+ * Y_K is smooth, and finite differences of it check the adjoint.
+ *
+ * nsim_bdf_adjoint: the discrete adjoint of the recorded (or replayed)
+ * trajectory, each step's equation taken as solved exactly: from
+ * m_bar_final = dLoss/dY_K it returns dLoss/dY_0 (3N), dLoss/dK1 per node,
+ * dLoss/dMs and dLoss/damps, solving (I - gamma J(Y_{k+1})^T) mu = Ybar_{k+1}
+ * per step by GMRES to lin_rtol with the full transposed Jacobian (demag
+ * included).  Outputs may be null. */
+typedef struct {
+  double t_final;       /* end of the trajectory, from t = 0 (ps)            */
+  double rtol, atol;    /* the integrator's tolerances                       */
+  int max_order;        /* 1..5                                              */
+  int max_steps;        /* steps the state buffer holds                      */
+  int n_pulses;         /* as nsim_traj                                      */
+  double pulse_len;
+  double pulse_dir[3];
+} nsim_bdf_traj;
+
+int nsim_bdf_forward(nsim_handle* h, const nsim_bdf_traj* tr, const double* amps, int* n_steps);
+/* The recorded steps: end times t_{k+1}, step sizes and orders (up to max
+ * entries each; arrays may be null).  Returns the number of steps. */
+int nsim_bdf_steps(nsim_handle* h, int max, double* t, double* hs, int* orders);
+int nsim_bdf_replay(nsim_handle* h, const double* amps, double newton_rtol);
+int nsim_bdf_adjoint(nsim_handle* h, const double* amps, const double* m_bar_final,
+                     double* m_bar_initial, double* K1_bar, double* Ms_bar, double* amps_bar,
+                     double lin_rtol);
+
 /* Changes magnet k's material coefficients in place (the operators do not
  * depend on them).  Replaces what the reference does when a simulation is
  * rebuilt with new MagMaterial parameters (interface/nmag/main.py:423-471);

@@ -395,8 +395,10 @@ int Integrator::run_host_cycles() {
     hhost_[4] = em * em * n3_;
     hhost_[5] = ep * ep * n3_;
     HTRY(hipMemcpyAsync(status_, hhost_, 6 * sizeof(double), hipMemcpyHostToDevice, s_));
+    const TsState attempt = hs_;
     HTRY(launch_ts_ctrl(s_, ds_, status_, n3_));
     TRY(download_state());
+    if (hook_) TRY(hook_(attempt, hs_));
     if (hs_.mode == kTsDone || hs_.mode == kTsError) return 0;
   }
 }
@@ -432,7 +434,7 @@ int Integrator::advance(double t_final, int max_steps, bool exact_tstop, double*
     }
     ts_prepare(hs_, hs_.hnext, hs_.qnext);
     TRY(upload_state());
-    const bool dev = opt_.device_gmres && ggrid_ > 0;
+    const bool dev = opt_.device_gmres && ggrid_ > 0 && !hook_;
     const int rc = dev ? run_device_cycles() : run_host_cycles();
     // cycles queued after the end run as empty launches; behind them on the
     // stream, lift the halt word so that later field evaluations run

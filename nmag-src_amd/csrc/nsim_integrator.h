@@ -12,6 +12,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -65,6 +66,11 @@ struct TsStats {
   long long order_steps[kMaxOrder + 1] = {};  // accepted steps per order
 };
 
+// Called after every controller step of a host-driven run with the state of
+// the attempt that ran and the state the controller left; a non-zero return
+// ends the run with that status.
+using CycleHook = std::function<int(const TsState& attempt, const TsState& next)>;
+
 class Integrator {
  public:
   Integrator(nsim_handle* h, const TsOptions& o);
@@ -79,6 +85,16 @@ class Integrator {
   TsOptions& options() { return opt_; }
   // workgroups of the persistent GMRES launch (0: host-driven GMRES only)
   int gmres_grid() const { return ggrid_; }
+  // With a hook set, advance runs the host-driven cycles and calls it after
+  // each; iterate() is the Newton iterate (the accepted y_{n+1} after an
+  // accepting step).
+  void set_cycle_hook(CycleHook f) { hook_ = std::move(f); }
+  // Points the handle's field kernels at this integrator's halt word again
+  // (after another integrator on the same handle has gone).
+  void rebind_halt() {
+    if (ok_) handle_set_halt(h_, &ds_->halt);
+  }
+  const double* iterate() const { return ycur_; }
 
  private:
   int run_device_cycles();
@@ -121,6 +137,7 @@ class Integrator {
   std::vector<double> host_part_;
   double gamma_ = 0.0;      // Newton matrix coefficient of the host-driven GMRES
   std::vector<void*> allocs_;
+  CycleHook hook_;
 };
 
 }  // namespace nsim

@@ -42,5 +42,7 @@ hipError_t vec_maxdist_partials(hipStream_t st, int nn, const double* a, const d
 hipError_t bjac_setup(hipStream_t st, int nn, const Coeffs* ct, const int* inst, double gamma,
                       const double* m, const double* H, const double* kappa, double* Minv);
 hipError_t bjac_solve(hipStream_t st, int nn, const double* Minv, const double* r, double* z);
+// z = Minv^T r per node
+hipError_t bjac_solve_t(hipStream_t st, int nn, const double* Minv, const double* r, double* z);
 
 }  // namespace nsim

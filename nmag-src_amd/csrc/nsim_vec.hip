@@ -157,6 +157,18 @@ __global__ __launch_bounds__(256) void k_bjac_solve(int nn, const double* Minv, 
   z[3 * i + 2] = o[6] * a + o[7] * b + o[8] * d;
 }
 
+// z = Minv^T r per node (the transposed block preconditioner of the adjoint)
+__global__ __launch_bounds__(256) void k_bjac_solve_t(int nn, const double* Minv,
+                                                      const double* r, double* z) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= nn) return;
+  const double* o = Minv + 9 * i;
+  const double a = r[3 * i], b = r[3 * i + 1], d = r[3 * i + 2];
+  z[3 * i] = o[0] * a + o[3] * b + o[6] * d;
+  z[3 * i + 1] = o[1] * a + o[4] * b + o[7] * d;
+  z[3 * i + 2] = o[2] * a + o[5] * b + o[8] * d;
+}
+
 inline int grid_for(int n) {
   const int g = (n + 255) / 256;
   return g < kReduceBlocks ? g : kReduceBlocks;
@@ -254,6 +266,11 @@ hipError_t bjac_setup(hipStream_t st, int nn, const Coeffs* ct, const int* inst,
 
 hipError_t bjac_solve(hipStream_t st, int nn, const double* Minv, const double* r, double* z) {
   k_bjac_solve<<<(nn + 255) / 256, 256, 0, st>>>(nn, Minv, r, z);
+  return hipGetLastError();
+}
+
+hipError_t bjac_solve_t(hipStream_t st, int nn, const double* Minv, const double* r, double* z) {
+  k_bjac_solve_t<<<(nn + 255) / 256, 256, 0, st>>>(nn, Minv, r, z);
   return hipGetLastError();
 }
 

@@ -47,6 +47,13 @@ class NsimTraj(ctypes.Structure):
                 ("pulse_len", ctypes.c_double), ("pulse_dir", ctypes.c_double * 3)]
 
 
+class NsimBdfTraj(ctypes.Structure):
+    _fields_ = [("t_final", ctypes.c_double), ("rtol", ctypes.c_double),
+                ("atol", ctypes.c_double), ("max_order", ctypes.c_int),
+                ("max_steps", ctypes.c_int), ("n_pulses", ctypes.c_int),
+                ("pulse_len", ctypes.c_double), ("pulse_dir", ctypes.c_double * 3)]
+
+
 # name -> (restype, argtypes); every symbol include/nsim.h declares
 SIGNATURES = {
     "nsim_create": (ctypes.c_int, [ctypes.POINTER(NsimMesh), ctypes.POINTER(NsimMaterial),
@@ -92,6 +99,12 @@ SIGNATURES = {
     "nsim_traj_forward": (ctypes.c_int, [_vp, ctypes.POINTER(NsimTraj), _dp]),
     "nsim_traj_adjoint": (ctypes.c_int, [_vp, ctypes.POINTER(NsimTraj), _dp, _dp, _dp, _dp,
                                          _dp, _dp]),
+    "nsim_bdf_forward": (ctypes.c_int, [_vp, ctypes.POINTER(NsimBdfTraj), _dp,
+                                        ctypes.POINTER(ctypes.c_int)]),
+    "nsim_bdf_steps": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp,
+                                      ctypes.POINTER(ctypes.c_int)]),
+    "nsim_bdf_replay": (ctypes.c_int, [_vp, _dp, ctypes.c_double]),
+    "nsim_bdf_adjoint": (ctypes.c_int, [_vp, _dp, _dp, _dp, _dp, _dp, _dp, ctypes.c_double]),
 }
 
 _lib = None
@@ -397,6 +410,49 @@ class NativeMagnet:
         _check(lib().nsim_traj_adjoint(self._h, ctypes.byref(tr), _dptr(a) if len(a) else None,
                                        _dptr(mb), _dptr(m0b), _dptr(k1b), ctypes.byref(ms),
                                        _dptr(ab)), "nsim_traj_adjoint")
+        return {"m0": m0b.reshape(-1, 3), "K1": k1b, "Ms": ms.value, "amps": ab[:len(a)]}
+
+    def bdf_forward(self, t_final, rtol=1e-6, atol=1e-6, max_order=2, max_steps=100000,
+                    amps=(), pulse_len=1.0, pulse_dir=(1.0, 0.0, 0.0)):
+        """BDF trajectory from the current m at t = 0 to t_final, every accepted
+        step recorded for the adjoint; returns the number of steps."""
+        a = np.ascontiguousarray(amps, dtype=np.float64).reshape(-1)
+        d = (ctypes.c_double * 3)(*[float(v) for v in pulse_dir])
+        tr = NsimBdfTraj(float(t_final), float(rtol), float(atol), int(max_order),
+                         int(max_steps), len(a), float(pulse_len), d)
+        k = ctypes.c_int(0)
+        _check(lib().nsim_bdf_forward(self._h, ctypes.byref(tr), _dptr(a) if len(a) else None,
+                                      ctypes.byref(k)), "nsim_bdf_forward")
+        self._bdf_pulses = len(a)
+        return k.value
+
+    def bdf_steps(self):
+        """(end times, step sizes, orders) of the recorded steps."""
+        k = lib().nsim_bdf_steps(self._h, 0, None, None, None)
+        _check(0 if k >= 0 else k, "nsim_bdf_steps")
+        t, hs, q = np.zeros(max(k, 1)), np.zeros(max(k, 1)), np.zeros(max(k, 1), dtype=np.int32)
+        lib().nsim_bdf_steps(self._h, k, _dptr(t), _dptr(hs), q.ctypes.data_as(_ip))
+        return t[:k], hs[:k], q[:k]
+
+    def bdf_replay(self, amps=(), newton_rtol=1e-12):
+        """Re-integrate the recorded step sequence from the current m with the
+        current parameters; m becomes the final state."""
+        a = np.ascontiguousarray(amps, dtype=np.float64).reshape(-1)
+        _check(lib().nsim_bdf_replay(self._h, _dptr(a) if len(a) else None, float(newton_rtol)),
+               "nsim_bdf_replay")
+
+    def bdf_adjoint(self, m_bar_final, amps=(), lin_rtol=1e-10):
+        """Discrete adjoint of the last bdf_forward / bdf_replay: cotangents of
+        m(0), K1 per node, Ms and the pulse amplitudes for dLoss/dm(T)."""
+        a = np.ascontiguousarray(amps, dtype=np.float64).reshape(-1)
+        mb = np.ascontiguousarray(m_bar_final, dtype=np.float64).reshape(-1)
+        m0b = np.zeros_like(mb)
+        k1b = np.zeros(self.n)
+        ms = ctypes.c_double(0.0)
+        ab = np.zeros(max(len(a), 1))
+        _check(lib().nsim_bdf_adjoint(self._h, _dptr(a) if len(a) else None, _dptr(mb),
+                                      _dptr(m0b), _dptr(k1b), ctypes.byref(ms), _dptr(ab),
+                                      float(lin_rtol)), "nsim_bdf_adjoint")
         return {"m0": m0b.reshape(-1, 3), "K1": k1b, "Ms": ms.value, "amps": ab[:len(a)]}
 
     def ts_init(self, t0=0.0, rtol=1e-6, atol=1e-6, max_order=2, krylov_max=300):
