@@ -402,20 +402,23 @@ def run_precision(slab, window_ps=1000.0, checkpoint_ps=10.0, seed=1, rtol=1e-10
     out = {"workload": "Py 30x30x10 nm cuboid, 2.5 nm Kuhn mesh (845 nodes), alpha 0.5, "
                        "seeded Gaussian m (seed %d), relaxed %.0f ps" % (seed, window_ps)}
     ref, avg = {}, {}
-    # "fp64_tight": FP64 integrated at rtol = atol = 1e-8, the size of the
-    # integration error itself at the working tolerance 1e-6
-    for prec in ("fp64", "mixed", "mixed24", "bf16", "fp64_tight"):
-        g = graphs(NativeMagnet(coords, tets, mat.su_coeffs(),
-                                precision="fp64" if prec == "fp64_tight" else prec,
+    # "<prec>_tight": integrated at rtol = atol = 1e-8.  FP64 tight against
+    # FP64 at the working tolerance 1e-6 is the size of the integration error
+    # itself; mixed / 24-bit tight against FP64 tight is the storage error's
+    # effect once the integration error no longer dominates
+    for prec in ("fp64", "mixed", "mixed24", "bf16", "fp64_tight", "mixed_tight",
+                 "mixed24_tight"):
+        base = prec.replace("_tight", "")
+        g = graphs(NativeMagnet(coords, tets, mat.su_coeffs(), precision=base,
                                 nbc_rtol=rtol, dbc_rtol=rtol))
         g.set_field("m", m0)
         g.execute()
         f = {k: g.get_field(k) for k in fields}
         if prec == "fp64":
             ref = f
-        elif prec != "fp64_tight":
+        elif prec == base:
             out[prec] = {k + "_max_rel_err": rel(f[k], ref[k]) for k in fields}
-        tol = 1e-8 if prec == "fp64_tight" else 1e-6
+        tol = 1e-8 if prec.endswith("_tight") else 1e-6
         g.ts_init(0.0, tol, tol)
         V = g.get_field("volumes")
         t, series = 0.0, []
@@ -429,6 +432,9 @@ def run_precision(slab, window_ps=1000.0, checkpoint_ps=10.0, seed=1, rtol=1e-10
         out[prec]["mean_m_max_dev_vs_fp64"] = float(np.abs(avg[prec] - avg["fp64"]).max())
     out["fp64_rtol_1e-8_mean_m_max_dev_vs_fp64"] = float(
         np.abs(avg["fp64_tight"] - avg["fp64"]).max())
+    for prec in ("mixed", "mixed24"):
+        out[prec]["mean_m_max_dev_vs_fp64_both_rtol_1e-8"] = float(
+            np.abs(avg[prec + "_tight"] - avg["fp64_tight"]).max())
     # the slab at one state
     coords, tets, mat, Hext, m = slab
     sref = {}

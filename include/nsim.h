@@ -332,8 +332,9 @@ int nsim_advance(nsim_handle* h, double t_final, int max_steps, int exact_tstop,
  * integrator (cvode_get_stats, src/sundials_sp_stubs.c:1409-1528):
  * "ts_steps", "ts_rhs", "ts_newton", "ts_linear", "ts_err_fails",
  * "ts_conv_fails", "ts_h", "ts_order", "ts_t", "ts_max_dm" (max_i |m_i(t1) -
- * m_i(t0)| over the last nsim_advance, as mwe_norm_dist_fun), and
- * "ts_order_steps_<q>" (accepted steps at order q = 1..5 since ts_init). */
+ * m_i(t0)| over the last nsim_advance, as mwe_norm_dist_fun),
+ * "ts_order_steps_<q>" (accepted steps at order q = 1..5 since ts_init) and
+ * "ts_linear_max" (most Krylov iterations of one Newton iteration). */
 int nsim_get_stat(nsim_handle* h, const char* name, double* out);
 
 /* Per-kernel device timers, the analogue of the reference's TSTART/TSTOP
@@ -344,7 +345,11 @@ int nsim_timers_enable(nsim_handle* h, int on);
 /* Launch-shape knobs (used by the tile autotuner); key "cg_rows_per_thread"
  * takes 1, 2, 4 or 8.  No counterpart in the reference (PETSc picks its own).
  * Integrator keys: "ts_fixed_order" (see nsim_ts_init), "gmres_device",
- * "gmres_pc_sweeps", "gmres_slices_per_wg", "cooperative_launch". */
+ * "gmres_pc_sweeps", "gmres_slices_per_wg", "cooperative_launch",
+ * "gmres_onex_steps" (0 .. 32, default 32: the persistent GMRES's Arnoldi
+ * steps that take one grid exchange each; the later ones take two), "gmres_precond"
+ * (1, the default: block-Jacobi preconditioning of the Newton systems; 0:
+ * none). */
 int nsim_set_tuning(nsim_handle* h, const char* key, int value);
 int nsim_get_timer(nsim_handle* h, const char* name, double* out);
 int nsim_synchronize(nsim_handle* h);

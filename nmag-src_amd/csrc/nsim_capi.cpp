@@ -793,6 +793,18 @@ int nsim_set_tuning(nsim_handle* h, const char* key, int value) {
     if (h->ts) h->ts->options().gmres_pc_sweeps = value;
     return 0;
   }
+  if (k == "gmres_precond") {  // 0: no preconditioner (tests of long Krylov solves)
+    h->ts_opt.precond = value != 0 ? 1 : 0;
+    if (h->ts) h->ts->options().precond = h->ts_opt.precond;
+    return 0;
+  }
+  if (k == "gmres_onex_steps") {
+    if (value < 0 || value > kGmresOneExMax)
+      return fail("gmres_onex_steps must be 0.." + std::to_string(kGmresOneExMax));
+    h->ts_opt.gmres_onex_steps = value;
+    if (h->ts) h->ts->options().gmres_onex_steps = value;
+    return 0;
+  }
   if (k == "gmres_slices_per_wg") {  // takes effect at the next nsim_ts_init
     if (value < 1 || value > 64) return fail("gmres_slices_per_wg must be 1..64");
     h->ts_opt.gmres_spw = value;
@@ -1087,6 +1099,7 @@ int nsim_get_stat(nsim_handle* h, const char* name, double* out) {
     else if (s == "ts_rhs") *out = (double)t.rhs;
     else if (s == "ts_newton") *out = (double)t.newton;
     else if (s == "ts_linear") *out = (double)t.linear;
+    else if (s == "ts_linear_max") *out = (double)t.linear_max;
     else if (s == "ts_err_fails") *out = (double)t.err_fails;
     else if (s == "ts_conv_fails") *out = (double)t.conv_fails;
     else if (s == "ts_h") *out = t.h_last;

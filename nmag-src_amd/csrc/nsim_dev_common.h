@@ -98,6 +98,12 @@ __device__ inline double load_coh(__amdgpu_buffer_rsrc_t r, int idx) {
   return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, idx * 8, 0, kSc1));
 }
 
+// Device-coherent load of the 64-bit word `idx` (a phase tag).
+__device__ inline unsigned long long load_tag(__amdgpu_buffer_rsrc_t r, size_t idx) {
+  return __builtin_bit_cast(unsigned long long,
+                            __builtin_amdgcn_raw_buffer_load_b64(r, (int)(idx * 8), 0, kSc1));
+}
+
 // One row of the jacobi_ts1 Jacobian-times-vector (nmag_lam.py:432-436,
 // src/nsim.ml:1004-1042): with H_total frozen except for its exchange part h,
 //   Jv = c1 (v x H + m x h) + c2 (v x (m x H) + m x (v x H) + m x (m x h)).

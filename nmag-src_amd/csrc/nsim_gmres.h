@@ -17,6 +17,17 @@ namespace nsim {
 // this size are what keep three workgroups' LDS within one CU.
 constexpr int kGmresMaxl = 320;
 constexpr int kGmresTagStride = dev::kTagStride;  // tag words per workgroup and parity
+// Arnoldi steps the kernel takes with one grid exchange each (the later ones
+// take two).
+constexpr int kGmresOneExMax = 32;
+
+// Partial sums per workgroup and exchange: up to maxl + 2 for a two-exchange
+// step, 2 j + 4 for one-exchange step j; rounded to whole 64-byte lines.
+inline int gmres_pstride(int maxl) {
+  const int one = 2 * (maxl < kGmresOneExMax ? maxl : kGmresOneExMax) + 4;
+  const int need = maxl + 2 > one ? maxl + 2 : one;
+  return (need + 7) & ~7;
+}
 
 // Device buffers of the solver (owned by the caller).
 struct GmresWork {
@@ -59,6 +70,8 @@ struct NewtonSystem {
   double* x;            // the correction
   double gamma, tol;
   int sweeps = 0;       // local Jacobi sweeps over each workgroup's own nodes
+  int onex_steps = kGmresOneExMax;  // Arnoldi steps with one grid exchange each
+                                    // (0 .. kGmresOneExMax; the rest take two)
   const double* gamma_p = nullptr;  // device word holding gamma (overrides gamma)
   const int* halt = nullptr;        // nonzero device word: the launch does nothing
   // step controller record: when set, the launch's last act is the

@@ -6,27 +6,29 @@
 //
 // GMRES runs right-preconditioned in the space scaled by the error weights
 // (the Euclidean residual is sqrt(n) times its WRMS norm) in its flexible
-// form: the preconditioned directions z_j are stored, the Krylov basis V is
-// orthonormalised by classical Gram-Schmidt applied twice, and x = Z y.  Each
-// workgroup owns whole SELL slices (64 nodes each) and every vector element
-// of their nodes; everything it computes on them is local except the exchange
-// part of J.z, which gathers z at neighbouring nodes.  Grid-wide data meet in
-// two exchanges per iteration:
-//   1. the first Gram-Schmidt column;
-//   2. the second column together with |v'|^2 of the once-projected vector
-//      v', whose direction z_{j+1} = M^-1 (v' / ewt) is published at the same
-//      time.  The new norm follows as |v'|^2 - |h2|^2 (v' - V h2 is v'
-//      without its components along the orthonormal V); when that difference
-//      loses more than two digits, a third exchange measures it directly.
-// Using z_{j+1} from v' rather than from the twice-projected vector is what
-// flexible GMRES permits: the Arnoldi relation A Z = V H holds for the z
-// actually used, and the solution is formed from the same Z.
-// An exchange publishes per-workgroup partial sums with agent-coherent stores,
-// then a phase tag; every workgroup polls all tags and sums the partials in
-// the same fixed order, so all of them hold bit-identical totals, take the
-// same Givens rotations and stop at the same iteration without further
-// communication.  Workgroup 0 keeps the triangular factor, solves the
-// least-squares problem at the end and broadcasts its solution.
+// form: the directions z_j are stored, the Krylov basis V is orthonormalised
+// by classical Gram-Schmidt applied twice, and x = Z y.  Each workgroup owns
+// whole SELL slices (64 nodes each) and every vector element of their nodes;
+// everything it computes on them is local except the exchange part of J.z,
+// which gathers z at neighbouring nodes.  Grid-wide sums meet in exchanges:
+// each workgroup publishes its partial sums with agent-coherent stores, then
+// a phase tag; every workgroup polls all tags and sums the partials in the
+// same fixed order, so all of them hold bit-identical totals, take the same
+// Givens rotations and stop at the same iteration without further
+// communication.
+//
+// The first kOneExMax Arnoldi steps take one exchange each: the second
+// Gram-Schmidt pass of a basis vector travels with the first pass of the next
+// (delayed reorthogonalisation), the next direction is formed from the
+// operator's output before the exchange (a Newton basis, z_{j+1} = (M^-1 A -
+// I) z_j: flexible GMRES admits any directions, and these span the same
+// Krylov space), and the norm of the next basis vector is estimated ahead
+// (|w|^2 - |h|^2), so a converged solve stops at the exchange that shows it.
+// Later steps take two exchanges: the first Gram-Schmidt column, then the
+// second with |v'|^2 of the once-projected v', whose direction z_{j+1} = M^-1
+// (v' / ewt) is published at the same time (the new norm follows as |v'|^2 -
+// |h2|^2, measured directly when that difference loses more than two
+// digits).  Both keep the Arnoldi relation A Z = V H for the z actually used.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -48,11 +50,16 @@ constexpr int kPcNodes = 256;              // largest node block of the local sw
 // Krylov iterations whose R every workgroup keeps in LDS (what fits beside
 // the rest while three workgroups share a CU's 160 KiB)
 constexpr int kRtCap = 32;
+// Arnoldi steps taken with one exchange each before the two-exchange steps
+// take over (the Newton basis of those steps stays well conditioned below it)
+constexpr int kOneExMax = kGmresOneExMax;
+constexpr int kSumChunk = 4;  // exchanged entries a wavefront sums per round trip
 
 template <typename TV>
 struct GmresArgs {
   int n, nslices, spw, maxl, gmax, pstride;
   int sweeps;          // local Jacobi sweeps of the preconditioner (0: block Jacobi)
+  int onex;            // Arnoldi steps with one exchange each (at most kOneExMax)
   const int* off;
   const int* col;
   const TV* K;
@@ -93,6 +100,7 @@ struct GShared {
   double cs[kGmresMaxl], sn[kGmresMaxl], g[kGmresMaxl + 1];
   double zs[kGmresMaxl];       // z_j = zs[j] * Z_j
   double col[kGmresMaxl + 1];  // Hessenberg column of this iteration
+  double colp[kOneExMax + 1];  // first-pass entries of the pending column (one-exchange steps)
   double loc[kGmresMaxl + 2];  // this workgroup's partial sums
   double tot[kGmresMaxl + 2];  // grid totals
   double gp[3 * kSplit][64];   // J.z gather partials
@@ -100,6 +108,7 @@ struct GShared {
   double pz[2][3 * kPcNodes];  //                 sweep iterates
   double red[8];
   double hn2, vn2;  // |v|^2 from |v'|^2 - |h2|^2, and |v'|^2
+  int stop;         // the look-ahead column met the tolerance
   // the first kRtCap columns of R, upper triangle packed column by column
   // (R[i][j] at j (j + 1) / 2 + i): every workgroup holds the same R, so up
   // to kRtCap iterations each solves R y = g itself, with no broadcast
@@ -134,45 +143,70 @@ __device__ inline __amdgpu_buffer_rsrc_t z_rsrc(const GmresArgs<TV>& a, int j) {
 
 // Publishes sm.loc[0, cnt) for phase `ph` and, unless this workgroup is not
 // the root of a root_only exchange, sums every workgroup's values into
-// sm.tot[0, cnt): wavefront w takes the entries w, w + 4, ..., lane l the
-// workgroups l, l + 64, ..., then a fixed butterfly; the same order in every
-// workgroup.  Every wave's earlier coherent stores (the published direction)
-// complete before the tag.  Returns false if the exchange timed out or another
-// workgroup reported an error (the spin is bounded, so every workgroup leaves
-// and the grid drains).
+// sm.tot[0, cnt).  The values are stored entry-major.  Thread t polls the
+// tags of workgroups t, t + 256, ... with their loads in flight together.
+// Then wavefront w sums the entries
+// w, w + 4, ..., kSumChunk of them per pass with all their loads in flight:
+// lane l over the workgroups l, l + 64, ... in increasing order, then a fixed
+// butterfly.  The order is the same in every workgroup, so all of them hold
+// bit-identical totals.  Every wave's earlier coherent stores (the published
+// direction) complete before the tag.  Returns false if the exchange timed
+// out or another workgroup reported an error (the spin is bounded, so every
+// workgroup leaves and the grid drains).
 template <typename TV>
 __device__ bool exchange(const GmresArgs<TV>& a, GShared& sm, unsigned ph, int cnt,
                          bool root_only) {
+  constexpr int Q = kMaxGrid / 256;  // workgroups per polling thread
   const int par = ph & 1, G = gridDim.x;
   const unsigned long long tg = (a.epoch << 24) | ph;
   __syncthreads();  // sm.loc complete
-  double* mine = a.part + ((size_t)par * a.gmax + blockIdx.x) * a.pstride;
-  for (int i = threadIdx.x; i < cnt; i += 256) st_coh(mine + i, sm.loc[i]);
+  // entry-major: entry i of every workgroup is contiguous, so a wavefront's
+  // reads of one entry coalesce into whole lines
+  double* part = a.part + (size_t)par * a.pstride * a.gmax;
+  for (int i = threadIdx.x; i < cnt; i += 256) st_coh(part + (size_t)i * a.gmax + blockIdx.x, sm.loc[i]);
   wait_all_memory();
   __syncthreads();
   if (threadIdx.x == 0) st_coh(&a.tag[tag_slot(par, a.gmax, blockIdx.x)], tg);
   if (root_only && blockIdx.x != 0) return true;
+  const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
+  const __amdgpu_buffer_rsrc_t rt = make_rsrc(a.tag, (unsigned)(2 * a.gmax * kTagStride * 8));
+  unsigned need = 0u;
+#pragma unroll
+  for (int q = 0; q < Q; q++)
+    if (tid + 256 * q < G) need |= 1u << q;
   int bad = 0;
-  for (int t = threadIdx.x; t < G && !bad; t += 256) {
-    unsigned spins = 0;
-    while (ld_coh(&a.tag[tag_slot(par, a.gmax, t)]) != tg) {
-      if ((++spins & 255u) == 0u && (ld_coh(a.err) != 0u || spins > kSpinLimit)) {
-        st_coh(a.err, 1u);
-        bad = 1;
-        break;
-      }
+  unsigned spins = 0;
+  while (need) {
+    unsigned long long v[Q];
+#pragma unroll
+    for (int q = 0; q < Q; q++)
+      v[q] = (need >> q) & 1u ? load_tag(rt, tag_slot(par, a.gmax, tid + 256 * q)) : tg;
+#pragma unroll
+    for (int q = 0; q < Q; q++)
+      if (v[q] == tg) need &= ~(1u << q);
+    if (need && (++spins & 255u) == 0u && (ld_coh(a.err) != 0u || spins > kSpinLimit)) {
+      st_coh(a.err, 1u);
+      bad = 1;
+      break;
     }
   }
   if (__syncthreads_or(bad)) return false;
-  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
-  const __amdgpu_buffer_rsrc_t rp =
-      make_rsrc(a.part + (size_t)par * a.gmax * a.pstride, (unsigned)(a.gmax * a.pstride * 8));
-  for (int i = w; i < cnt; i += 4) {
-    double s = 0.0;
-#pragma unroll 8
-    for (int t = l; t < G; t += 64) s += load_coh(rp, t * a.pstride + i);
-    s = wave_sum(s);
-    if (l == 0) sm.tot[i] = s;
+  const __amdgpu_buffer_rsrc_t rp = make_rsrc(part, (unsigned)(a.gmax * a.pstride * 8));
+  for (int i0 = w; i0 < cnt; i0 += 4 * kSumChunk) {
+    double sv[kSumChunk];
+#pragma unroll
+    for (int c = 0; c < kSumChunk; c++) sv[c] = 0.0;
+#pragma unroll 4
+    for (int t = l; t < G; t += 64) {
+#pragma unroll
+      for (int c = 0; c < kSumChunk; c++)
+        if (i0 + 4 * c < cnt) sv[c] += load_coh(rp, (i0 + 4 * c) * a.gmax + t);
+    }
+#pragma unroll
+    for (int c = 0; c < kSumChunk; c++) {
+      const double s = wave_sum(sv[c]);
+      if (l == 0 && i0 + 4 * c < cnt) sm.tot[i0 + 4 * c] = s;
+    }
   }
   __syncthreads();
   return true;
@@ -203,18 +237,22 @@ __device__ inline void block_total(double s, GShared& sm, int slot) {
   if (threadIdx.x == 0) sm.loc[slot] = s;
 }
 
-// Z_j = M^-1 t on this workgroup's nodes, published coherently, with
-// t = v / ewt (or, for the first direction, t = the Newton residual).  M is
+// Z_j = M^-1 t - sub Z_jsub on this workgroup's nodes (no subtraction when
+// jsub < 0), published coherently, with t = v / ewt (or, for the first
+// direction, t = the Newton residual); returns this thread's part of |Z_j|^2.  M is
 // the 3x3 block-Jacobi splitting A = D - N of A = I - gamma J, improved by
 // `sweeps` Jacobi sweeps z <- D^-1 (t + N z) over the couplings inside the
 // workgroup's own nodes (an additive-Schwarz block solve; couplings to other
 // workgroups' nodes are dropped, so M stays local and fixed).  N z at node i
 // is gamma (c1 m x h + c2 m x (m x h)) with h = negJ/V_i sum_{j != i} K_ij z_j.
 template <typename TV>
-__device__ void publish_direction(const GmresArgs<TV>& a, GShared& sm, const double* v,
-                                  bool residual, int j, int s0, int s1, int r0, int r1) {
+__device__ double publish_direction(const GmresArgs<TV>& a, GShared& sm, const double* v,
+                                    bool residual, int j, int s0, int s1, int r0, int r1,
+                                    int jsub, double sub) {
   double* Zj = a.Z + (size_t)j * a.ld;
   const bool local = a.Minv && a.sweeps > 0 && r1 - r0 <= kPcNodes;
+  const __amdgpu_buffer_rsrc_t rs = z_rsrc(a, jsub >= 0 ? jsub : 0);
+  double zz = 0.0;
   for (int i = r0 + threadIdx.x; i < r1; i += 256) {
     double t[3], z[3];
     for (int d = 0; d < 3; d++) {
@@ -234,10 +272,14 @@ __device__ void publish_direction(const GmresArgs<TV>& a, GShared& sm, const dou
         sm.pz[0][3 * (i - r0) + d] = z[d];
       }
     } else {
-      for (int d = 0; d < 3; d++) st_coh(Zj + 3 * i + d, z[d]);
+      for (int d = 0; d < 3; d++) {
+        const double zd = jsub >= 0 ? z[d] - sub * load_coh(rs, 3 * i + d) : z[d];
+        zz += zd * zd;
+        st_coh(Zj + 3 * i + d, zd);
+      }
     }
   }
-  if (!local) return;
+  if (!local) return zz;
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
   int cur = 0;
   for (int sw = 0; sw < a.sweeps; sw++) {
@@ -281,7 +323,13 @@ __device__ void publish_direction(const GmresArgs<TV>& a, GShared& sm, const dou
     cur ^= 1;
   }
   for (int i = r0 + threadIdx.x; i < r1; i += 256)
-    for (int d = 0; d < 3; d++) st_coh(Zj + 3 * i + d, sm.pz[cur][3 * (i - r0) + d]);
+    for (int d = 0; d < 3; d++) {
+      const double zp = sm.pz[cur][3 * (i - r0) + d];
+      const double zd = jsub >= 0 ? zp - sub * load_coh(rs, 3 * i + d) : zp;
+      zz += zd * zd;
+      st_coh(Zj + 3 * i + d, zd);
+    }
+  return zz;
 }
 
 // sm.loc[i] = this workgroup's part of V_i . v for i <= j.
@@ -296,6 +344,99 @@ __device__ void local_dots(const GmresArgs<TV>& a, GShared& sm, const double* v,
     s = wave_sum(s);
     if (lane == 0) sm.loc[i] = s;
   }
+}
+
+// Vn = (z - gamma J z) ewt on this workgroup's rows, z = zs[j] Z_j: the
+// exchange part of J z gathers z at neighbouring nodes (published by other
+// workgroups, read device-coherently).
+template <typename TV>
+__device__ void apply_op(const GmresArgs<TV>& a, GShared& sm, int j, double* Vn, int s0,
+                         int s1) {
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const __amdgpu_buffer_rsrc_t rz = z_rsrc(a, j);
+  const double sc = sm.zs[j];
+  for (int s = s0; s < s1; s++) {
+    const int row = 64 * s + lane;
+    const bool live = row < a.n;
+    const int b = a.off[s], w = (a.off[s + 1] - b) >> 6;
+    double hv[3] = {0.0, 0.0, 0.0};
+    if (live)
+      for (int q = wv; q < w; q += kSplit) {
+        const int e = b + (q << 6) + lane;
+        const int c = a.col[e];
+        const double kv = (double)a.K[e];
+        hv[0] += kv * load_coh(rz, 3 * c);
+        hv[1] += kv * load_coh(rz, 3 * c + 1);
+        hv[2] += kv * load_coh(rz, 3 * c + 2);
+      }
+    for (int q = 0; q < 3; q++) sm.gp[wv * 3 + q][lane] = hv[q];
+    __syncthreads();
+    if (wv == 0 && live) {
+      for (int q = 0; q < 3; q++) {
+        double acc = sm.gp[q][lane];
+        for (int u = 1; u < kSplit; u++) acc += sm.gp[u * 3 + q][lane];
+        hv[q] = acc;
+      }
+      const Coeffs& cf = a.ct[a.inst ? a.inst[row] : 0];
+      const double fct = cf.negJ * a.Vinv[row] * sc;
+      const double h[3] = {fct * hv[0], fct * hv[1], fct * hv[2]};
+      const double mi[3] = {a.m[3 * row], a.m[3 * row + 1], a.m[3 * row + 2]};
+      const double Hi[3] = {a.Ht[3 * row], a.Ht[3 * row + 1], a.Ht[3 * row + 2]};
+      double zi[3], jv[3];
+      for (int d = 0; d < 3; d++) zi[d] = sc * load_coh(rz, 3 * row + d);
+      llg_jv_row(cf.c1, cf.c2, h, mi, zi, Hi, jv);
+      for (int d = 0; d < 3; d++)
+        Vn[3 * row + d] = (zi[d] - a.gamma * jv[d]) * a.ewt[3 * row + d];
+    }
+    __syncthreads();
+  }
+}
+
+// sm.loc[i] = this workgroup's part of V_i . u and, when w is given,
+// sm.loc[j + i] = its part of V_i . w, for i < j.
+template <typename TV>
+__device__ void pair_dots(const GmresArgs<TV>& a, GShared& sm, const double* u, const double* w,
+                          int j, int e0, int e1) {
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int i = wv; i < j; i += 4) {
+    const double* Vi = a.V + (size_t)i * a.ld;
+    double sa = 0.0, sb = 0.0;
+    for (int e = e0 + lane; e < e1; e += 64) {
+      const double vi = Vi[e];
+      sa += vi * u[e];
+      if (w) sb += vi * w[e];
+    }
+    sa = wave_sum(sa);
+    sb = wave_sum(sb);
+    if (lane == 0) {
+      sm.loc[i] = sa;
+      sm.loc[j + i] = sb;
+    }
+  }
+}
+
+// Thread 0: Hessenberg column j (sm.col[0 .. j + 1]) through the earlier
+// Givens rotations and a new one, the residual vector g updated; the column
+// of R kept in LDS (j < kRtCap) and, by workgroup 0, in R.
+template <typename TV>
+__device__ void rotate_column(const GmresArgs<TV>& a, GShared& sm, int j) {
+  double* c = sm.col;
+  for (int i = 0; i < j; i++) {
+    const double p = c[i], q = c[i + 1];
+    c[i] = sm.cs[i] * p + sm.sn[i] * q;
+    c[i + 1] = -sm.sn[i] * p + sm.cs[i] * q;
+  }
+  const double p = c[j], q = c[j + 1];
+  const double rr = hypot(p, q);
+  sm.cs[j] = rr > 0.0 ? p / rr : 1.0;
+  sm.sn[j] = rr > 0.0 ? q / rr : 0.0;
+  c[j] = rr;
+  sm.g[j + 1] = -sm.sn[j] * sm.g[j];
+  sm.g[j] = sm.cs[j] * sm.g[j];
+  if (j < kRtCap)
+    for (int i = 0; i <= j; i++) sm.Rt[j * (j + 1) / 2 + i] = c[i];
+  if (blockIdx.x == 0)
+    for (int i = 0; i <= j; i++) a.R[(size_t)i * a.maxl + j] = c[i];
 }
 
 // The step controller as a call on the record in place: inlined, or on a
@@ -323,7 +464,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) voi
   const int s0 = blockIdx.x * a.spw, s1 = min(a.nslices, s0 + a.spw);
   const int r0 = 64 * s0, r1 = min(a.n, 64 * s1);
   const int e0 = 3 * r0, e1 = 3 * r1;
-  const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
+  const int tid = threadIdx.x;
   const double tol = a.tol;
   unsigned ph = 0;
   bool ok = true;
@@ -340,7 +481,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) voi
         const double Hi[3] = {a.Ht[3 * i], a.Ht[3 * i + 1], a.Ht[3 * i + 2]};
         bjac_block(cf.c1, cf.c2, a.gamma, a.kappa[i], mi, Hi, a.Minv + 9 * i);
       }
-    publish_direction(a, sm, nullptr, true, 0, s0, s1, r0, r1);
+    publish_direction(a, sm, nullptr, true, 0, s0, s1, r0, r1, -1, 0.0);
     double s = 0.0;
     for (int e = e0 + tid; e < e1; e += 256) {
       double r = 1.0 * a.avec[e];
@@ -364,46 +505,166 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) voi
     }
     __syncthreads();
   }
-  for (int j = 0; ok && !solved && j < a.maxl; j++) {
-    double* Vn = a.V + (size_t)(j + 1) * a.ld;
-    const __amdgpu_buffer_rsrc_t rz = z_rsrc(a, j);
-    const double sc = sm.zs[j];
-    // Vn = (z - gamma J z) ewt, z = sc Z_j
-    for (int s = s0; s < s1; s++) {
-      const int row = 64 * s + lane;
-      const bool live = row < a.n;
-      const int b = a.off[s], w = (a.off[s + 1] - b) >> 6;
-      double hv[3] = {0.0, 0.0, 0.0};
-      if (live)
-        for (int q = wv; q < w; q += kSplit) {
-          const int e = b + (q << 6) + lane;
-          const int c = a.col[e];
-          const double kv = (double)a.K[e];
-          hv[0] += kv * load_coh(rz, 3 * c);
-          hv[1] += kv * load_coh(rz, 3 * c + 1);
-          hv[2] += kv * load_coh(rz, 3 * c + 2);
+  int j = 0;
+  bool conv = !ok || solved;  // no (further) Arnoldi steps
+  // One exchange per Arnoldi step while j < kOneExMax: step j applies the
+  // operator to z_j, publishes the next direction
+  //   z_{j+1} = M^-1 (w_j / ewt) - z_j,  w_j = (z_j - gamma J z_j) ewt,
+  // and in one exchange gets the second Gram-Schmidt pass of the pending
+  // u_j (the once-projected w_{j-1}): a = V^T u_j and |u_j|^2, together with
+  // the first pass of w_j: b = V^T w_j, u_j . w_j and |w_j|^2.  Then v_j =
+  // (u_j - V a) / alpha finishes Hessenberg column j - 1 (its first-pass
+  // entries plus a, and alpha below them), whose Givens rotation gives the
+  // residual after j steps, and u_{j+1} = w_j - V b - h_jj v_j waits for the
+  // next step.  The directions form a Newton basis of the same Krylov space
+  // as the two-exchange steps below (with M fixed, exactly the iterates of
+  // right-preconditioned GMRES), whose conditioning grows with the step
+  // count, so after a.onex (at most kOneExMax) steps the two-exchange steps
+  // take over.
+  if (a.onex > 0 && !conv) {
+    for (;; j++) {
+      const bool pend = j > 0;  // u_j waits in V_j for its second pass
+      const bool havew = j < a.onex && j < a.maxl;
+      double* Vj = a.V + (size_t)j * a.ld;
+      double* Vn = a.V + (size_t)(j + 1) * a.ld;  // w_j, then u_{j+1}
+      double zz = 0.0;
+      if (havew) {
+        apply_op(a, sm, j, Vn, s0, s1);
+        if (j + 1 < a.maxl)
+          zz = publish_direction(a, sm, Vn, false, j + 1, s0, s1, r0, r1, j, sm.zs[j]);
+      }
+      pair_dots(a, sm, Vj, havew ? Vn : nullptr, j, e0, e1);
+      {
+        double uu = 0.0, cc = 0.0, ww = 0.0;
+        for (int e = e0 + tid; e < e1; e += 256) {
+          const double u = Vj[e];
+          uu += u * u;
+          if (havew) {
+            const double w = Vn[e];
+            cc += u * w;
+            ww += w * w;
+          }
         }
-      for (int q = 0; q < 3; q++) sm.gp[wv * 3 + q][lane] = hv[q];
+        block_sum2(uu, cc, sm.red);
+        block_sum2(zz, ww, sm.red);
+        if (tid == 0) {
+          sm.loc[2 * j] = uu;
+          sm.loc[2 * j + 1] = cc;
+          sm.loc[2 * j + 2] = zz;
+          sm.loc[2 * j + 3] = ww;
+        }
+      }
+      ok = exchange(a, sm, ++ph, 2 * j + 4, false);
+      if (!ok) break;
+      // everything needed from the totals, before another exchange may
+      // overwrite them: column j - 1 = pending entries + a, the new pending
+      // entries b, and a.a, a.b, b.b
+      const double uu = sm.tot[2 * j], cc = sm.tot[2 * j + 1], zt = sm.tot[2 * j + 2];
+      const double ww = sm.tot[2 * j + 3];
+      double aa = 0.0, ab = 0.0, bb = 0.0;
+      for (int i = 0; i < j; i++) {
+        aa += sm.tot[i] * sm.tot[i];
+        ab += sm.tot[i] * sm.tot[j + i];
+        bb += sm.tot[j + i] * sm.tot[j + i];
+      }
+      for (int i = tid; i < j; i += 256) sm.col[i] = sm.colp[i] + sm.tot[i];
       __syncthreads();
-      if (wv == 0 && live) {
-        for (int q = 0; q < 3; q++) {
-          double acc = sm.gp[q][lane];
-          for (int u = 1; u < kSplit; u++) acc += sm.gp[u * 3 + q][lane];
-          hv[q] = acc;
+      for (int i = tid; i < j; i += 256) sm.colp[i] = sm.tot[j + i];
+      double alpha = 1.0;
+      bool vdone = false;  // V_j already holds u_j - V a
+      if (pend) {
+        double al2 = uu - aa;
+        if (!(al2 > 1e-2 * uu)) {  // cancellation: form u_j - V a and measure it directly
+          for (int e = e0 + tid; e < e1; e += 256) {
+            double v = Vj[e];
+            for (int i = 0; i < j; i++) v -= sm.tot[i] * a.V[(size_t)i * a.ld + e];
+            Vj[e] = v;
+          }
+          vdone = true;
+          double sq = 0.0;
+          for (int e = e0 + tid; e < e1; e += 256) sq += Vj[e] * Vj[e];
+          block_total(sq, sm, 0);
+          ok = exchange(a, sm, ++ph, 1, false);
+          if (!ok) break;
+          al2 = sm.tot[0];
         }
-        const Coeffs& cf = a.ct[a.inst ? a.inst[row] : 0];
-        const double fct = cf.negJ * a.Vinv[row] * sc;
-        const double h[3] = {fct * hv[0], fct * hv[1], fct * hv[2]};
-        const double mi[3] = {a.m[3 * row], a.m[3 * row + 1], a.m[3 * row + 2]};
-        const double Hi[3] = {a.Ht[3 * row], a.Ht[3 * row + 1], a.Ht[3 * row + 2]};
-        double zi[3], jv[3];
-        for (int d = 0; d < 3; d++) zi[d] = sc * load_coh(rz, 3 * row + d);
-        llg_jv_row(cf.c1, cf.c2, h, mi, zi, Hi, jv);
-        for (int d = 0; d < 3; d++)
-          Vn[3 * row + d] = (zi[d] - a.gamma * jv[d]) * a.ewt[3 * row + d];
+        alpha = sqrt(fmax(al2, 0.0));
+        if (tid == 0) {
+          sm.col[j] = alpha;
+          rotate_column(a, sm, j - 1);
+        }
+        __syncthreads();
+        k = j;
+        if (fabs(sm.g[j]) <= tol || !(alpha > 0.0) || !isfinite(alpha)) {
+          conv = true;
+          break;
+        }
+      }
+      const double inv = pend ? 1.0 / alpha : 1.0;
+      const double hjj = pend ? (cc - ab) / alpha : cc;
+      if (havew) {
+        // look-ahead: |u_{j+1}|^2 = |w_j|^2 - |b|^2 - h_jj^2 (v_0 .. v_j
+        // orthonormal) completes column j provisionally.  While the
+        // difference keeps 8 of the 16 digits, the estimate is good to about
+        // 1e-8 relative, far finer than the stopping test needs; if that
+        // column's rotation already meets the tolerance, the solve stops here
+        // with it (its second Gram-Schmidt pass would change it at rounding
+        // level), and the step that would finish it is not taken.
+        const double un2 = ww - bb - hjj * hjj;
+        if (tid == 0) {
+          sm.stop = 0;
+          if (un2 > 1e-8 * ww) {
+            double* c = sm.col;
+            for (int i = 0; i < j; i++) c[i] = sm.colp[i];
+            c[j] = hjj;
+            c[j + 1] = sqrt(un2);
+            // row j of the column after the earlier rotations, without
+            // applying them yet
+            double p = c[0];
+            for (int i = 0; i < j; i++) p = -sm.sn[i] * p + sm.cs[i] * c[i + 1];
+            const double rr = hypot(p, c[j + 1]);
+            if (rr > 0.0 && fabs(c[j + 1] / rr * sm.g[j]) <= tol) {
+              rotate_column(a, sm, j);
+              sm.stop = 1;
+            }
+          }
+        }
+        __syncthreads();
+        if (sm.stop) {
+          k = j + 1;
+          conv = true;
+          break;
+        }
+      }
+      // v_j = (u_j - V a) / alpha and, while the steps go on, u_{j+1} =
+      // w_j - V b - h_jj v_j, in one pass over V_0 .. V_{j-1}
+      for (int e = e0 + tid; e < e1; e += 256) {
+        double v = Vj[e], w = havew ? Vn[e] : 0.0;
+        for (int i = 0; i < j; i++) {
+          const double vi = a.V[(size_t)i * a.ld + e];
+          if (!vdone) v -= sm.tot[i] * vi;
+          if (havew) w -= sm.colp[i] * vi;
+        }
+        v *= inv;
+        if (pend) Vj[e] = v;
+        if (havew) Vn[e] = w - hjj * v;
+      }
+      if (!havew) {  // the last column is finished, v_j normalised for what follows
+        __syncthreads();
+        break;
+      }
+      if (tid == 0) {
+        sm.colp[j] = hjj;
+        sm.zs[j + 1] = zt > 0.0 ? 1.0 / sqrt(zt) : 0.0;
       }
       __syncthreads();
     }
+    if (j >= a.maxl) conv = true;
+  }
+  // two exchanges per Arnoldi step (classical Gram-Schmidt applied twice)
+  for (; ok && !conv && j < a.maxl; j++) {
+    double* Vn = a.V + (size_t)(j + 1) * a.ld;
+    apply_op(a, sm, j, Vn, s0, s1);
     // Gram-Schmidt pass 1
     local_dots(a, sm, Vn, j, e0, e1);
     ok = exchange(a, sm, ++ph, j + 1, false);
@@ -420,7 +681,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) voi
     {
       double s = 0.0;
       for (int e = e0 + tid; e < e1; e += 256) s += Vn[e] * Vn[e];
-      if (j + 1 < a.maxl) publish_direction(a, sm, Vn, false, j + 1, s0, s1, r0, r1);
+      if (j + 1 < a.maxl) publish_direction(a, sm, Vn, false, j + 1, s0, s1, r0, r1, -1, 0.0);
       block_total(s, sm, j + 1);
     }
     ok = exchange(a, sm, ++ph, j + 2, false);
@@ -455,25 +716,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) voi
       for (int e = e0 + tid; e < e1; e += 256) Vn[e] *= inv;
     }
     if (tid == 0) {
-      double* c = sm.col;
-      c[j + 1] = hn;
+      sm.col[j + 1] = hn;
       if (j + 1 < a.maxl) sm.zs[j + 1] = hn > 0.0 ? 1.0 / hn : 0.0;
-      for (int i = 0; i < j; i++) {  // previous rotations
-        const double p = c[i], q = c[i + 1];
-        c[i] = sm.cs[i] * p + sm.sn[i] * q;
-        c[i + 1] = -sm.sn[i] * p + sm.cs[i] * q;
-      }
-      const double p = c[j], q = c[j + 1];
-      const double rr = hypot(p, q);
-      sm.cs[j] = rr > 0.0 ? p / rr : 1.0;
-      sm.sn[j] = rr > 0.0 ? q / rr : 0.0;
-      c[j] = rr;
-      sm.g[j + 1] = -sm.sn[j] * sm.g[j];
-      sm.g[j] = sm.cs[j] * sm.g[j];
-      if (j < kRtCap)
-        for (int i = 0; i <= j; i++) sm.Rt[j * (j + 1) / 2 + i] = c[i];
-      if (blockIdx.x == 0)
-        for (int i = 0; i <= j; i++) a.R[(size_t)i * a.maxl + j] = c[i];
+      rotate_column(a, sm, j);
     }
     __syncthreads();
     k = j + 1;
@@ -587,6 +832,7 @@ template <typename TV>
 hipError_t launch_typed(const DeviceOps& d, const TV* K, GmresWork& w, int grid, int spw,
                         int maxl, const NewtonSystem& s, hipStream_t st) {
   GmresArgs<TV> a{d.n,     d.nslices, spw,      maxl,     w.gmax,  w.pstride, s.sweeps,
+                  s.onex_steps,
                   d.sell_off, d.sell_col, K,  d.Vinv,   d.coefs,  d.inst,  s.ycur,    s.Ht,
                   s.ewt,   s.kappa,   s.Minv,   s.gamma,  s.tol,   s.avec,    s.f,
                   s.yp,    s.x,       w.V,      w.Z,      3 * (size_t)d.n,    w.part,
@@ -627,7 +873,8 @@ bool gmres_grid(int nslices, int maxl, int device, int gmax, int min_spw, int* g
 hipError_t launch_newton_gmres(const DeviceOps& d, bool mixed, GmresWork& w, int grid, int spw,
                                int maxl, const NewtonSystem& s, hipStream_t st) {
   if (grid < 1 || grid > w.gmax || grid > w.resident || maxl < 1 || maxl > kGmresMaxl ||
-      w.pstride < maxl + 2 ||
+      s.onex_steps < 0 || s.onex_steps > kGmresOneExMax ||
+      w.pstride < gmres_pstride(maxl) ||
       (long long)grid * spw < d.nslices || (long long)(grid - 1) * spw >= d.nslices ||
       3LL * d.n * 8 >= (1LL << 32) || (long long)w.gmax * w.pstride * 8 >= (1LL << 32))
     return hipErrorInvalidConfiguration;

@@ -89,7 +89,7 @@ Integrator::Integrator(nsim_handle* h, const TsOptions& o) : h_(h), opt_(o) {
     gw_.V = V_;
     gw_.gmax = grid;
     gw_.resident = resident;
-    gw_.pstride = (opt_.krylov_max + 2 + 7) & ~7;
+    gw_.pstride = gmres_pstride(opt_.krylov_max);
     const size_t maxl = opt_.krylov_max;
     bool g = alloc(&gw_.Z, n * maxl) && alloc(&gw_.part, 2 * (size_t)grid * gw_.pstride) &&
              alloc(&gw_.R, maxl * maxl) && alloc(&gw_.y, maxl) && alloc(&gw_.status, 8);
@@ -302,6 +302,7 @@ int Integrator::enqueue_cycle() {
   sys.halt = &ds_->halt;
   sys.tol = kLinCoef * kNewtonCoef * std::sqrt((double)n3_);
   sys.sweeps = opt_.gmres_pc_sweeps;
+  sys.onex_steps = opt_.gmres_onex_steps;
   sys.ts = ds_;
   sys.slots = slots_dev_;
   gw_.cooperative = opt_.cooperative;
@@ -454,6 +455,7 @@ int Integrator::advance(double t_final, int max_steps, bool exact_tstop, double*
     st_.rhs = hs_.rhs;
     st_.newton = hs_.newton;
     st_.linear = hs_.linear;
+    st_.linear_max = hs_.linear_max;
     st_.err_fails = hs_.err_fails;
     st_.conv_fails = hs_.conv_fails;
     st_.h_last = hs_.h_last;

@@ -53,6 +53,8 @@ struct TsOptions {
   int gmres_spw = 1;                // least SELL slices per GMRES workgroup
   int gmres_pc_sweeps = 1;          // local Jacobi sweeps of the persistent GMRES's
                                     // preconditioner (0: 3x3 block Jacobi only)
+  int gmres_onex_steps = 32;        // persistent GMRES: Arnoldi steps with one
+                                    // grid exchange each (0 .. 32; the rest take two)
   int max_batch = 16;               // Newton cycles queued per batch
   bool fixed_order = false;         // orders 3-5: rise to max_order as the history
                                     // allows instead of choosing the order
@@ -61,6 +63,7 @@ struct TsOptions {
 
 struct TsStats {
   long steps = 0, rhs = 0, newton = 0, linear = 0, err_fails = 0, conv_fails = 0;
+  long linear_max = 0;  // most Krylov iterations of one Newton iteration
   double h_last = 0.0, t = 0.0, max_dm = 0.0;
   int q_last = 0;
   long long order_steps[kMaxOrder + 1] = {};  // accepted steps per order

@@ -265,12 +265,6 @@ constexpr int kBdfKrylov = 40;  // restart length of the two GMRES solves
 
 using BdfStep = AdjointState::BdfStep;
 
-int bdf_window(const nsim_bdf_traj& tr, double t0, double h) {
-  if (tr.n_pulses <= 0) return -1;
-  const int w = (int)std::floor((t0 + 0.5 * h) / tr.pulse_len);
-  return w < tr.n_pulses ? w : -1;
-}
-
 // H_ext = the static part + amps[w] along the pulse direction (w < 0: none)
 int bdf_set_window(nsim_handle* h, const double* amps, int w) {
   AdjointState* A = h->adj;

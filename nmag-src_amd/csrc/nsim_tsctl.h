@@ -69,6 +69,7 @@ struct TsState {
   int exact_tstop, max_steps, steps_call;
   // statistics
   long long steps, rhs, newton, linear, err_fails, conv_fails;
+  long long linear_max;  // most Krylov iterations of one Newton iteration
   long long order_steps[kMaxOrder + 1];  // accepted steps per order
   double h_last;
   int q_last;
@@ -344,6 +345,7 @@ __host__ __device__ inline void ts_after_newton(TsState& s, double d, double err
   s.rhs++;
   s.newton++;
   s.linear += its;
+  if (its > s.linear_max) s.linear_max = its;
   bool converged = false, failed = false;
   if (!ts_finite(d)) {
     failed = true;

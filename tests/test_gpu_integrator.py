@@ -291,3 +291,42 @@ def test_orders_up_to_five_against_scipy(device):
         if q == 5:
             assert sum(by_order[2:]) > 0.5 * steps[q], by_order
     assert steps[5] < 0.8 * steps[2], steps
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_one_exchange_gmres_matches_two_exchange_gmres(precision):
+    """The persistent GMRES's one-exchange Arnoldi steps (second Gram-Schmidt
+    pass delayed to the next step, Newton-basis directions, look-ahead stop)
+    and its two-exchange steps build the same Krylov spaces.  Whether every
+    step takes two exchanges (0), or the first 1, 2, 8 or all of them (32)
+    take one, switching to two-exchange steps mid-solve, the trajectories agree
+    far below the integration tolerance, with the same step counts and nearly
+    the same Krylov iteration counts."""
+    # a smooth state, damped, at rtol 1e-5: steps long enough for up to about
+    # 20 Krylov iterations per Newton solve
+    mat = material(alpha=0.5)
+    coords, tets = kuhn_cuboid(10, 6, 3, 2.5)
+    g = NativeMagnet(coords, tets, mat.su_coeffs(), precision=precision, potential="dense")
+    n = len(coords)
+    x, y = coords[:, 0], coords[:, 1]
+    m0 = np.stack([np.ones_like(x), 0.3 * np.sin(x / 7.0), 0.2 * np.cos(y / 5.0)], axis=1)
+    m0 /= np.linalg.norm(m0, axis=1, keepdims=True)
+    g.set_field("H_ext", np.tile([0.0, 0.02, 0.01], (n, 1)))
+    res = {}
+    for steps in (0, 1, 2, 8, 32):
+        g.tune("gmres_onex_steps", steps)
+        g.set_field("m", m0)
+        g.ts_init(0.0, rtol=1e-5, atol=1e-5)
+        assert g.advance(100.0) == pytest.approx(100.0)
+        res[steps] = (g.get_field("m"), {k: g.stat("ts_" + k) for k in (
+            "steps", "newton", "linear", "linear_max")})
+    s0 = res[0][1]
+    assert s0["steps"] > 10 and s0["linear_max"] >= 10, s0
+    for steps in (1, 2, 8, 32):
+        s1 = res[steps][1]
+        assert abs(s1["steps"] - s0["steps"]) <= max(2, 0.02 * s0["steps"]), (steps, s1, s0)
+        assert abs(s1["linear"] - s0["linear"]) <= 0.05 * s0["linear"] + 5, (steps, s1, s0)
+        assert np.abs(res[steps][0] - res[0][0]).max() < 1e-6, steps
+    with pytest.raises(Exception):
+        g.tune("gmres_onex_steps", 33)
+    g.close()
