@@ -76,6 +76,9 @@ def graphs(g):
 # workload only.
 MEASURED_TRAFFIC = {(4, 19926): 777846 * 2048 + 179 * 1024,
                     (8, 19926): 1554955 * 2048 + 185 * 1024,
+                    # the 24-bit operator, one handle per profiled process
+                    # (profiles/potential_gemv_f24_c2_slab_hbm_counters.md)
+                    (3, 19926): int(583578.2 * 2048 + 194.9 * 1024),
                     # both k_panel_gemv<float> launches of the compressed
                     # operator (eps 1e-7, eta 3: 292,995,808 stored bytes;
                     # profiles/potential_hmatrix_c2_slab.md); any other
@@ -545,7 +548,8 @@ def main():
     ap.add_argument("--dynamics-routes", default="dense,hmatrix",
                     help="demag routes timed in the dynamics leg")
     ap.add_argument("--dense-precs", default="fp64,mixed,mixed24",
-                    help="precisions timed on the dense route (fp64 first)")
+                    help="precisions timed on the dense route (fp64 first when given; "
+                         "one alone builds one operator handle, as the counter passes need)")
     ap.add_argument("--routes", default="dense,iterative,hmatrix",
                     help="demag routes timed in the field-evaluation leg")
     ap.add_argument("--no-autotune", action="store_true",
@@ -569,9 +573,10 @@ def main():
     global AUTOTUNE, GRAPHS
     AUTOTUNE = not args.no_autotune
     GRAPHS = not args.no_graph
-    if not args.dense_precs.startswith("fp64") or not set(args.dense_precs.split(",")) <= {
-            "fp64", "mixed", "mixed24"}:
-        print("--dense-precs: fp64 first, then any of mixed, mixed24", file=sys.stderr)
+    dp = args.dense_precs.split(",")
+    if not dp or not set(dp) <= {"fp64", "mixed", "mixed24"} or ("fp64" in dp and dp[0] != "fp64"):
+        print("--dense-precs: any of fp64, mixed, mixed24 (fp64 first when given)",
+              file=sys.stderr)
         sys.exit(2)
     if not args.routes or not set(args.routes.split(",")) <= {"dense", "iterative", "hmatrix"}:
         print("--routes: any of dense, iterative, hmatrix", file=sys.stderr)
