@@ -194,7 +194,8 @@ struct TVec<uint16_t> {
 
 // part[chunk][j] = sum over the chunk's rows i of B_ij e_i; each thread owns
 // V consecutive columns (one 16-byte load per row), rows of a chunk are
-// streamed in order, so a wavefront reads 1 KiB of a row per load.
+// streamed in order (eight rows' loads in flight), so a wavefront reads
+// 1 KiB of a row per load.
 template <typename TB>
 __global__ __launch_bounds__(256) void k_bem_gemv_t(int nb, int ldb, int rows_per_chunk,
                                                     const TB* __restrict__ B,
@@ -207,6 +208,7 @@ __global__ __launch_bounds__(256) void k_bem_gemv_t(int nb, int ldb, int rows_pe
   const int r1 = min(r0 + rows_per_chunk, nb);
   double acc[V];
   for (int v = 0; v < V; v++) acc[v] = 0.0;
+#pragma unroll 8
   for (int i = r0; i < r1; i++) {
     double bv[V];
     TVec<TB>::load(B + (size_t)i * ldb, j0, ldb, bv);
@@ -216,16 +218,30 @@ __global__ __launch_bounds__(256) void k_bem_gemv_t(int nb, int ldb, int rows_pe
   for (int v = 0; v < V; v++) part[(size_t)blockIdx.y * ldb + j0 + v] = acc[v];
 }
 
-__global__ __launch_bounds__(256) void k_bem_t_reduce(int nb, int ldb, int chunks,
-                                                      const double* __restrict__ part,
-                                                      const int* __restrict__ bnode,
-                                                      double* __restrict__ chi, int accumulate) {
-  const int j = blockIdx.x * 256 + threadIdx.x;
-  if (j >= nb) return;
+// chi[bnode[j]] (+)= sum over the chunks c of part[c][j], 64 columns per
+// workgroup: wavefront w sums the chunks w, w + 16, ... in increasing order
+// (eight loads in flight), then the 16 wavefront sums meet in LDS in a fixed
+// order, so the result does not depend on timing.
+__global__ __launch_bounds__(1024) void k_bem_t_reduce(int nb, int ldb, int chunks,
+                                                       const double* __restrict__ part,
+                                                       const int* __restrict__ bnode,
+                                                       double* __restrict__ chi, int accumulate) {
+  __shared__ double sh[16][64];
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  const int j = blockIdx.x * 64 + l;
   double s = 0.0;
-  for (int c = 0; c < chunks; c++) s += part[(size_t)c * ldb + j];
-  if (accumulate) s += chi[bnode[j]];
-  chi[bnode[j]] = s;
+  if (j < nb) {
+#pragma unroll 8
+    for (int c = w; c < chunks; c += 16) s += part[(size_t)c * ldb + j];
+  }
+  sh[w][l] = s;
+  __syncthreads();
+  if (w == 0 && j < nb) {
+    double t = 0.0;
+    for (int q = 0; q < 16; q++) t += sh[q][l];
+    if (accumulate) t += chi[bnode[j]];
+    chi[bnode[j]] = t;
+  }
 }
 
 __global__ __launch_bounds__(1024) void k_mean_project(int n, double* __restrict__ x) {
@@ -279,8 +295,8 @@ hipError_t gemv_t_typed(const DenseOp& op, const double* e, double* part, double
   const int rpc = (nb + chunks - 1) / chunks;
   const int cols = (ldb / V + 255) / 256;
   k_bem_gemv_t<TB><<<dim3(cols, chunks), 256, 0, st>>>(nb, ldb, rpc, (const TB*)op.A, e, part);
-  k_bem_t_reduce<<<blocks_for(nb), 256, 0, st>>>(nb, ldb, chunks, part, op.map, chi,
-                                                 accumulate ? 1 : 0);
+  k_bem_t_reduce<<<(nb + 63) / 64, 1024, 0, st>>>(nb, ldb, chunks, part, op.map, chi,
+                                                   accumulate ? 1 : 0);
   return hipGetLastError();
 }
 
@@ -316,9 +332,10 @@ hipError_t launch_adj_boundary(const DeviceOps& d, const double* psi, const doub
 
 int dense_t_chunks(const DenseOp& op) {
   if (op.h_n.empty() || op.h_n[0] == 0) return 1;
-  // about 2048 workgroups in all, each chunk at least 8 rows
+  // about 512 workgroups in all (each thread then streams a few tens of rows
+  // with its loads in flight together), each chunk at least 8 rows
   const int cols = (op.h_ld[0] / 2 + 255) / 256;
-  int chunks = std::max(1, 2048 / std::max(1, cols));
+  int chunks = std::max(1, 512 / std::max(1, cols));
   chunks = std::min(chunks, std::max(1, op.h_n[0] / 8));
   return chunks;
 }
