@@ -40,6 +40,9 @@ struct AdjointState {
   double* ckpt = nullptr;
   int ckpt_count = 0;
   double* stages = nullptr;  // checkpoint_every x 4 stage inputs
+  // checkpoint_every x 4 stages x the fields the vector-Jacobian product reads
+  // there (H_exch, H_demag, H_anis, H_total), kept by the recompute pass
+  double* sfields = nullptr;
   int stage_steps = 0;
   bool forward_done = false;
   nsim_traj last{};
@@ -121,16 +124,25 @@ FieldArgs field_args_of(nsim_handle* h, const double* m) { return field_args(h, 
 
 // out = J(y)^T lam, plus the parameter accumulations of `acc`.  Evaluates the
 // right-hand side at y first, so the handle's fields are those of y.
+// saved: the fields of y (H_exch, H_demag, H_anis, H_total, 3n each) kept
+// when y's right-hand side was evaluated; then it is not evaluated again.
 int vjp(nsim_handle* h, const double* y, const double* lam, double* out, const AdjAccum& acc,
-        bool eval_rhs = true) {
+        bool eval_rhs = true, const double* saved = nullptr) {
   AdjointState* A = h->adj;
   const DeviceOps& d = h->d;
   hipStream_t st = h->stream;
   // eval_rhs false: the handle's fields are those of y already
-  if (eval_rhs) TRY(device_rhs(h, y, h->dmdt));
+  if (eval_rhs && !saved) TRY(device_rhs(h, y, h->dmdt));
   const bool demag = h->opt.do_demag != 0 && d.nb > 0;
+  const size_t n3 = 3 * (size_t)d.n;
   FieldBufs b{h->Hexch, demag ? h->Hdemag : nullptr, h->Hanis, h->Htotal, h->rho,
               h->phi1,  h->phi2, h->phi, h->phi1b};
+  if (saved) {
+    b.Hexch = const_cast<double*>(saved);
+    b.Hdemag = demag ? const_cast<double*>(saved + n3) : nullptr;
+    b.Hanis = const_cast<double*>(saved + 2 * n3);
+    b.Htotal = const_cast<double*>(saved + 3 * n3);
+  }
   HIPCHK(launch_adj_local(d, field_args_of(h, y), b, lam, A->g, A->x, out, acc, st));
   HIPCHK(launch_adj_exch_div(d, h->mixed, A->x, out, A->psi, st));
   if (demag && d.pot.A) {
@@ -194,6 +206,7 @@ int ensure_traj(nsim_handle* h, const nsim_traj* tr) {
   }
   if (tr->checkpoint_every > A->stage_steps) {
     TRY(dalloc(A, &A->stages, (size_t)tr->checkpoint_every * 4 * n3));
+    TRY(dalloc(A, &A->sfields, (size_t)tr->checkpoint_every * 16 * n3));
     A->stage_steps = tr->checkpoint_every;
   }
   if (tr->n_pulses > A->hbar_windows) {
@@ -229,24 +242,36 @@ int restore_hext(nsim_handle* h) {
   return 0;
 }
 
-// One RK4 step y -> y (stage inputs copied to `stage` when not null).
-int rk4_step(nsim_handle* h, double dt, double* y, double* stage) {
+// One RK4 step y -> y (stage inputs copied to `stage` and the fields of each
+// stage's right-hand side to `fields`, 4 x 3n per stage, when not null).
+int rk4_step(nsim_handle* h, double dt, double* y, double* stage, double* fields = nullptr) {
   AdjointState* A = h->adj;
   hipStream_t st = h->stream;
   const int n3 = 3 * A->n;
   const size_t bytes = (size_t)n3 * sizeof(double);
   double** k = A->kv;
+  const bool demag = h->opt.do_demag != 0 && h->d.nb > 0;
+  auto keep = [&](int i) -> int {
+    if (fields)
+      HIPCHK(vec_copy4(st, n3, h->Hexch, demag ? h->Hdemag : nullptr, h->Hanis, h->Htotal,
+                       fields + (size_t)i * 4 * n3));
+    return 0;
+  };
   if (stage) HIPCHK(hipMemcpyAsync(stage, y, bytes, hipMemcpyDeviceToDevice, st));
   TRY(device_rhs(h, y, k[0]));
+  TRY(keep(0));
   HIPCHK(vec_lincomb(st, n3, 1.0, y, 0.5 * dt, k[0], 0, nullptr, 0, nullptr, A->tmp));
   if (stage) HIPCHK(hipMemcpyAsync(stage + n3, A->tmp, bytes, hipMemcpyDeviceToDevice, st));
   TRY(device_rhs(h, A->tmp, k[1]));
+  TRY(keep(1));
   HIPCHK(vec_lincomb(st, n3, 1.0, y, 0.5 * dt, k[1], 0, nullptr, 0, nullptr, A->tmp));
   if (stage) HIPCHK(hipMemcpyAsync(stage + 2 * n3, A->tmp, bytes, hipMemcpyDeviceToDevice, st));
   TRY(device_rhs(h, A->tmp, k[2]));
+  TRY(keep(2));
   HIPCHK(vec_lincomb(st, n3, 1.0, y, dt, k[2], 0, nullptr, 0, nullptr, A->tmp));
   if (stage) HIPCHK(hipMemcpyAsync(stage + 3 * n3, A->tmp, bytes, hipMemcpyDeviceToDevice, st));
   TRY(device_rhs(h, A->tmp, k[3]));
+  TRY(keep(3));
   HIPCHK(vec_lincomb(st, n3, 1.0, k[0], 2.0, k[1], 2.0, k[2], 1.0, k[3], A->tmp));
   HIPCHK(vec_lincomb(st, n3, 1.0, y, dt / 6.0, A->tmp, 0, nullptr, 0, nullptr, y));
   return 0;
@@ -625,7 +650,8 @@ int nsim_traj_adjoint(nsim_handle* h, const nsim_traj* tr, const double* amps,
     HIPCHK(hipMemcpyAsync(A->y, A->ckpt + (size_t)seg * n3, bytes, hipMemcpyDeviceToDevice, st));
     for (int s = s0; s < s1; s++) {
       TRY(set_window(h, tr, amps, s));
-      TRY(rk4_step(h, dt, A->y, A->stages + (size_t)(s - s0) * 4 * n3));
+      TRY(rk4_step(h, dt, A->y, A->stages + (size_t)(s - s0) * 4 * n3,
+                   A->sfields + (size_t)(s - s0) * 16 * n3));
     }
     // backward through the segment
     for (int s = s1 - 1; s >= s0; s--) {
@@ -634,19 +660,21 @@ int nsim_traj_adjoint(nsim_handle* h, const nsim_traj* tr, const double* amps,
       AdjAccum acc{A->k1bar, A->msbar, w >= 0 ? A->hbar + (size_t)w * n : nullptr,
                    {tr->pulse_dir[0], tr->pulse_dir[1], tr->pulse_dir[2]}};
       const double* Y = A->stages + (size_t)(s - s0) * 4 * n3;
+      // the fields of each stage, kept by the recompute pass above
+      const double* F = A->sfields + (size_t)(s - s0) * 16 * n3;
       HIPCHK(vec_scale4(st, n3, A->ybar, dt / 6.0, kb[0], dt / 3.0, kb[1], dt / 3.0, kb[2],
                         dt / 6.0, kb[3]));
       // stage 4: Y4 = y + dt k3
-      TRY(vjp(h, Y + 3 * (size_t)n3, kb[3], A->out, acc));
+      TRY(vjp(h, Y + 3 * (size_t)n3, kb[3], A->out, acc, true, F + 12 * (size_t)n3));
       HIPCHK(vec_update2(st, n3, A->out, 1.0, A->ybar, dt, kb[2]));
       // stage 3: Y3 = y + dt/2 k2
-      TRY(vjp(h, Y + 2 * (size_t)n3, kb[2], A->out, acc));
+      TRY(vjp(h, Y + 2 * (size_t)n3, kb[2], A->out, acc, true, F + 8 * (size_t)n3));
       HIPCHK(vec_update2(st, n3, A->out, 1.0, A->ybar, 0.5 * dt, kb[1]));
       // stage 2: Y2 = y + dt/2 k1
-      TRY(vjp(h, Y + (size_t)n3, kb[1], A->out, acc));
+      TRY(vjp(h, Y + (size_t)n3, kb[1], A->out, acc, true, F + 4 * (size_t)n3));
       HIPCHK(vec_update2(st, n3, A->out, 1.0, A->ybar, 0.5 * dt, kb[0]));
       // stage 1: Y1 = y
-      TRY(vjp(h, Y, kb[0], A->out, acc));
+      TRY(vjp(h, Y, kb[0], A->out, acc, true, F));
       HIPCHK(vec_lincomb(st, n3, 1.0, A->ybar, 1.0, A->out, 0, nullptr, 0, nullptr, A->ybar));
     }
   }

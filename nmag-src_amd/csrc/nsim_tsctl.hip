@@ -21,7 +21,10 @@ __global__ __launch_bounds__(256) void k_ts_begin(int n3, const TsState* __restr
                                                   double* __restrict__ yp,
                                                   double* __restrict__ avec,
                                                   double* __restrict__ ewt) {
-  if (s->mode != kTsBegin || s->halt) return;
+  // every field of the record and the slot pointers are read before the
+  // mode is tested, so their loads are in flight together (the record is
+  // always valid to read; the slot indices are always in range or negative)
+  const int mode = s->mode, halt = s->halt;
   const int copy = s->copy;
   const double* yn = slots[s->sh[0]];
   double* dst = copy >= 0 ? slots[copy] : nullptr;
@@ -41,6 +44,7 @@ __global__ __launch_bounds__(256) void k_ts_begin(int n3, const TsState* __restr
     A[j] = s->A[j];
   }
   const double rtol = s->rtol, atol = s->atol;
+  if (mode != kTsBegin || halt) return;
   for (int i = blockIdx.x * 256 + threadIdx.x; i < n3; i += gridDim.x * 256) {
     // the accepted iterate becomes y_n (same values as the ring slot gets)
     const double y = dst ? ycur[i] : yn[i];

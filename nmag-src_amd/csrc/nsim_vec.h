@@ -19,6 +19,10 @@ hipError_t vec_mul(hipStream_t st, int n, const double* x, const double* w, bool
 // y_k = c_k x, k = 0..3, in one launch.
 hipError_t vec_scale4(hipStream_t st, int n, const double* x, double c0, double* y0, double c1,
                       double* y1, double c2, double* y2, double c3, double* y3);
+// dst[k * n ...] = src_k (k = 0..3; a null source leaves its slice alone),
+// in one launch.
+hipError_t vec_copy4(hipStream_t st, int n, const double* s0, const double* s1,
+                     const double* s2, const double* s3, double* dst);
 // y1 += a1 x; y2 += a2 x, in one launch.
 hipError_t vec_update2(hipStream_t st, int n, const double* x, double a1, double* y1, double a2,
                        double* y2);

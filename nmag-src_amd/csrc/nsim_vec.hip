@@ -48,6 +48,17 @@ __global__ __launch_bounds__(256) void k_scale4(int n, const double* x, double c
   }
 }
 
+__global__ __launch_bounds__(256) void k_copy4(int n, const double* s0, const double* s1,
+                                               const double* s2, const double* s3,
+                                               double* dst) {
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+    if (s0) dst[i] = s0[i];
+    if (s1) dst[n + i] = s1[i];
+    if (s2) dst[2 * (size_t)n + i] = s2[i];
+    if (s3) dst[3 * (size_t)n + i] = s3[i];
+  }
+}
+
 // y1 += a1 x and y2 += a2 x in one pass.
 __global__ __launch_bounds__(256) void k_update2(int n, const double* x, double a1, double* y1,
                                                  double a2, double* y2) {
@@ -208,6 +219,12 @@ hipError_t vec_lincomb(hipStream_t st, int n, double c0, const double* x0, doubl
 hipError_t vec_scale4(hipStream_t st, int n, const double* x, double c0, double* y0, double c1,
                       double* y1, double c2, double* y2, double c3, double* y3) {
   k_scale4<<<grid_for(n), 256, 0, st>>>(n, x, c0, y0, c1, y1, c2, y2, c3, y3);
+  return hipGetLastError();
+}
+
+hipError_t vec_copy4(hipStream_t st, int n, const double* s0, const double* s1,
+                     const double* s2, const double* s3, double* dst) {
+  k_copy4<<<grid_for(n), 256, 0, st>>>(n, s0, s1, s2, s3, dst);
   return hipGetLastError();
 }
 
