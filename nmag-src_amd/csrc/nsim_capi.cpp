@@ -122,14 +122,24 @@ int setup_gemv(nsim_handle* h, DenseOp& op, int rows_per_pass, int wg_per_cu, in
 // (Re)builds the CG operator with `rpt` rows per thread (parts of 256*rpt
 // nodes) and uploads it; fails if the parts do not fit one co-resident grid.
 int setup_cg(nsim_handle* h, int rpt) {
+  // the fine coarse space (sub-part aggregates) when the parts allow it and
+  // it fits a co-resident grid, else aggregates of whole parts
   CgOperator op;
-  std::string err = build_cg_operator(h->host, 256 * rpt, op);
+  std::string err;
+  int maxw = 0, resident = 0;
+  bool fine = false;
+  for (int pass = h->cg_fine ? 0 : 1; pass < 2; pass++) {
+    op = CgOperator();
+    err = build_cg_operator(h->host, 256 * rpt, op, pass == 0);
+    if (!err.empty()) continue;
+    fine = op.sub_shift > 0;
+    maxw = cg_slot_variant(op.max_width, rpt);
+    resident = cg_max_workgroups(rpt, maxw, h->device, fine);
+    if (op.parts <= 1024 && op.parts <= resident) break;
+    err = "CG parts do not fit one co-resident grid at " + std::to_string(rpt) +
+          " rows per thread";
+  }
   if (!err.empty()) return fail(err);
-  const int maxw = cg_slot_variant(op.max_width, rpt);
-  const int resident = cg_max_workgroups(rpt, maxw, h->device);
-  if (op.parts > 1024 || op.parts > resident)
-    return fail("CG parts do not fit one co-resident grid at " + std::to_string(rpt) +
-                " rows per thread");
   for (void* p : h->cg_allocs) (void)hipFree(p);
   h->cg_allocs.clear();
   CgDevice c;
@@ -138,6 +148,8 @@ int setup_cg(nsim_handle* h, int rpt) {
   c.rpt = rpt;
   c.cap_shift = rpt == 1 ? 8 : rpt == 2 ? 9 : 10;
   c.agg_shift = op.agg_shift;
+  c.sub_shift = op.sub_shift;
+  c.fine = fine;
   c.naggs = op.naggs;
   c.maxw = maxw;
   c.max_width = op.max_width;
@@ -151,6 +163,12 @@ int setup_cg(nsim_handle* h, int rpt) {
             up(&c.p2o, op.p2o) && up(&c.dinv_nbc, op.dinv_nbc) &&
             up(&c.dinv_dbc, op.dinv_dbc) && up(&c.ainv_nbc, op.ainv_nbc) &&
             up(&c.ainv_dbc, op.ainv_dbc);
+  if (ok && fine) {
+    // the fine coarse inverses in FP32 (the kernel keeps them in LDS)
+    const std::vector<float> fn(op.ainv_nbc.begin(), op.ainv_nbc.end());
+    const std::vector<float> fd(op.ainv_dbc.begin(), op.ainv_dbc.end());
+    ok = up(&c.ainvf_nbc, fn) && up(&c.ainvf_dbc, fd);
+  }
   if (!ok) return fail("CG operator upload failed");
   h->cg = c;
   return 0;
@@ -622,7 +640,7 @@ int create_impl(const char* who, int ninst, const nsim_mesh* meshes,
   h->cgw.gmax = 1024;
   // every layout pads by less than one part of at most 1024 rows
   ZEROS(h->cgw.pub, 2 * (n + 1024));
-  ZEROS(h->cgw.part, 8 * (size_t)h->cgw.gmax);
+  ZEROS(h->cgw.part, 2 * (size_t)kCgRecord * h->cgw.gmax);
   ZEROS(h->cgw.tag, 2 * (size_t)h->cgw.gmax * dev::kTagStride);
   ZEROS(h->sync_nbc, 8);
   ZEROS(h->sync_dbc, 8);
@@ -766,6 +784,11 @@ int nsim_set_tuning(nsim_handle* h, const char* key, int value) {
   if (k == "cg_coarse") {
     h->cg_coarse = value != 0;
     return 0;
+  }
+  if (k == "cg_fine") {  // sub-part aggregates of the coarse space (rebuilds the CG)
+    h->cg_fine = value != 0;
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return setup_cg(h, h->cg.rpt);
   }
   if (k == "cooperative_launch") {  // persistent CG / GMRES launches
     h->cgw.cooperative = value != 0;

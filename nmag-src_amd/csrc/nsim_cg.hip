@@ -26,6 +26,7 @@
 
 #include "nsim_dev_common.h"
 #include "nsim_device.h"
+#include "nsim_internal.h"
 
 namespace nsim {
 
@@ -35,6 +36,12 @@ using namespace dev;
 
 constexpr int kMaxAggs = 64;
 constexpr int kMaxParts = 1024;
+constexpr int kSubs = 4;  // aggregates per part at most (fine coarse spaces)
+static_assert(2 + 2 * kSubs <= kCgRecord, "workgroup record");
+// restriction slots per workgroup: fixed per variant so that they stay in
+// registers (indexed by compile-time constants only)
+template <bool FINE>
+constexpr int kSlots = FINE ? kSubs : 1;
 
 template <typename TV>
 struct CgArgs {
@@ -42,6 +49,7 @@ struct CgArgs {
   int gmax, maxit;
   int cap_shift;   // log2(rows per part)
   int agg_shift;   // log2(parts per aggregate)
+  int sub_shift;   // log2(aggregates per part); then agg_shift is 0
   int naggs;       // 0: Jacobi only
   const int* off;
   const int* col;
@@ -49,10 +57,11 @@ struct CgArgs {
   const int* p2o;      // row -> original node, -1 on padding rows
   const double* dinv;  // 0 on padding and fixed (Dirichlet) rows
   const double* ainv;  // naggs x naggs coarse inverse
+  const float* ainvf;  // the same in FP32 (fine coarse spaces)
   const double* b;     // original order, may be null (zero load)
   double* x;           // original order; in: warm start (+ Dirichlet values), out: solution
   double* pub;         // 2n published vector (by phase parity)
-  double* part;        // 2 * gmax * 4 partial sums (by phase parity)
+  double* part;        // 2 * gmax records of kCgRecord partial sums (by phase parity)
   unsigned long long* tag;  // 2*gmax phase tags
   unsigned* sync;           // [2] error flag, [3] last its, [4] total its, [5] solves,
                             // [6] solves stopped at maxit
@@ -65,16 +74,37 @@ struct CgArgs {
   const int* halt;          // nonzero word: every workgroup returns at once
 };
 
-struct Shared {
+// LDS of a workgroup: the FP64 coarse inverse of at most 64 aggregates (three
+// workgroups per CU), or, with a fine coarse space, the FP32 inverse of at
+// most kCgFineAggs sub-part aggregates (one workgroup per CU).
+template <bool FINE>
+struct Shared;
+template <>
+struct Shared<false> {
   double red[8];
+  double wred[4][kCgRecord];
+  double tot[kCgRecord];
   double ainv[kMaxAggs * kMaxAggs];
   double cpart[kMaxParts];
   double cin[kMaxAggs];
   double zc[kMaxAggs];
 };
-// three workgroups per CU, with 1 KiB for the compiler's own LDS use
-static_assert(3 * lds_granules((int)sizeof(Shared) + 1024) * kLdsGranule <= kLdsPerCu,
+template <>
+struct Shared<true> {
+  double red[8];
+  double wred[4][kCgRecord];
+  double tot[kCgRecord];
+  float ainv[kCgFineAggs * kCgFineAggs];
+  double cpart[kCgFineAggs];
+  double cin[kCgFineAggs];
+  double zc[kCgFineAggs];
+};
+// three workgroups per CU (one with a fine coarse space), with 1 KiB for the
+// compiler's own LDS use
+static_assert(3 * lds_granules((int)sizeof(Shared<false>) + 1024) * kLdsGranule <= kLdsPerCu,
               "CG workgroup LDS too large for three workgroups per CU");
+static_assert(lds_granules((int)sizeof(Shared<true>) + 1024) * kLdsGranule <= kLdsPerCu,
+              "CG workgroup LDS (fine coarse space) too large for a CU");
 
 constexpr unsigned kSpinLimit = 1u << 20;
 
@@ -113,7 +143,7 @@ struct RowSlots {
   // sum_j K_ij (v_j + [j free] zc[agg(j)]), v read device-coherently at `shift`
   __device__ double gather(const CgArgs<TV>& a, __amdgpu_buffer_rsrc_t r, int shift,
                            const double* zc, bool coarse) const {
-    const int sh = a.cap_shift + a.agg_shift;
+    const int sh = a.cap_shift + a.agg_shift - a.sub_shift;  // row -> aggregate
     double acc = 0.0;
     if constexpr (MAXW > 0) {
       double v[MAXW];
@@ -150,27 +180,96 @@ struct RowSlots {
   }
 };
 
-// Publishes this workgroup's partial sums (s1, s2, coarse restriction c and,
-// in the first phase, cb) of phase `ph`; returns the grid totals of s1, s2 and
-// leaves in sm.zc the coarse correction A_c^-1 (sum of the c's per aggregate).
-// Every thread must have issued its sc1
-// stores of the phase's published vector before the call.  Returns false if
-// the exchange timed out or another workgroup reported an error; the spin is
-// bounded so that every workgroup leaves the solver and the grid drains.
-template <typename TV>
+// Sums the entries v[i], i < Q with used(i), over the workgroup into
+// sm.tot[i]: a butterfly per wavefront, then the four wavefronts as
+// (w0 + w1) + (w2 + w3), the order of block_sum2, so every workgroup forms
+// its totals the same way.  `used` is uniform over the workgroup.
+template <int Q, bool FINE, typename Used>
+__device__ void block_sums(const double (&v)[Q], Used used, Shared<FINE>& sm) {
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+#pragma unroll
+  for (int i = 0; i < Q; i++) {
+    if (!used(i)) continue;
+    const double t = wave_sum(v[i]);
+    if (l == 0) sm.wred[w][i] = t;
+  }
+  __syncthreads();
+  if (threadIdx.x < Q && used((int)threadIdx.x)) {
+    const int i = threadIdx.x;
+    sm.tot[i] = (sm.wred[0][i] + sm.wred[1][i]) + (sm.wred[2][i] + sm.wred[3][i]);
+  }
+  __syncthreads();
+}
+
+// sm.cin = the aggregates' restrictions, from the per-part (or, with a fine
+// coarse space, per-aggregate) ones in sm.cpart.
+template <typename TV, bool FINE>
+__device__ void gather_cin(const CgArgs<TV>& a, Shared<FINE>& sm, int G) {
+  const int na = a.naggs;
+  if (threadIdx.x < na) {
+    double sacc = 0.0;
+    if (FINE) {
+      sacc = sm.cpart[threadIdx.x];
+    } else {
+      const int g0 = threadIdx.x << a.agg_shift;
+      const int g1 = min(G, (threadIdx.x + 1) << a.agg_shift);
+      for (int g = g0; g < g1; g++) sacc += sm.cpart[g];
+    }
+    sm.cin[threadIdx.x] = sacc;
+  }
+  __syncthreads();
+}
+
+// (A_c^-1 sm.cin)_i for aggregate i (threads i < naggs)
+template <typename TV, bool FINE>
+__device__ double coarse_row(const CgArgs<TV>& a, const Shared<FINE>& sm, int i) {
+  const int na = a.naggs;
+  double z = 0.0;
+  for (int b = 0; b < na; b++) z += (double)sm.ainv[i * na + b] * sm.cin[b];
+  return z;
+}
+
+// sm.zc = A_c^-1 cin over the aggregates (cin gathered as above).
+template <typename TV, bool FINE>
+__device__ void coarse_solve(const CgArgs<TV>& a, Shared<FINE>& sm, int G) {
+  gather_cin(a, sm, G);
+  if (threadIdx.x < a.naggs) sm.zc[threadIdx.x] = coarse_row(a, sm, threadIdx.x);
+  __syncthreads();
+}
+
+// Publishes this workgroup's record of phase `ph`, [s1, s2, c_0 .. c_{L-1},
+// cb_0 .. cb_{L-1}] (L = kSlots<FINE>) with c and cb the coarse restrictions
+// of its S <= L aggregates (one per part without a fine coarse space; slots
+// from S on are not written or read); returns the grid
+// totals of s1, s2 and leaves in sm.zc the coarse correction A_c^-1 (the
+// restrictions c gathered per aggregate).  Every thread must have issued its
+// sc1 stores of the phase's published vector before the call.  Returns false
+// if the exchange timed out or another workgroup reported an error; the spin
+// is bounded so that every workgroup leaves the solver and the grid drains.
+template <typename TV, bool FINE>
 __device__ bool exchange(const CgArgs<TV>& a, __amdgpu_buffer_rsrc_t rpart, unsigned ph,
-                         double& s1, double& s2, double c, double cb, Shared& sm) {
+                         double& s1, double& s2, const double (&c)[kSlots<FINE>],
+                         const double (&cb)[kSlots<FINE>], Shared<FINE>& sm) {
   wait_all_memory();
-  block_sum2(s1, s2, sm.red);
-  block_sum2(c, cb, sm.red);
+  constexpr int L = kSlots<FINE>, Q = 2 + 2 * L;
+  static_assert(Q <= kCgRecord, "workgroup record");
+  const int S = 1 << a.sub_shift;
+  const auto used = [S](int i) { return i < 2 || (i - 2) % L < S; };
+  double v[Q];
+  v[0] = s1;
+  v[1] = s2;
+#pragma unroll
+  for (int i = 0; i < L; i++) {
+    v[2 + i] = c[i];
+    v[2 + L + i] = cb[i];
+  }
+  block_sums(v, used, sm);
   const int par = ph & 1, G = gridDim.x;
   const unsigned long long tg = (a.epoch << 24) | ph;
   if (threadIdx.x == 0) {
-    double* slot = &a.part[4 * (par * a.gmax + blockIdx.x)];
-    st_coh(slot, s1);
-    st_coh(slot + 1, s2);
-    st_coh(slot + 2, c);
-    st_coh(slot + 3, cb);
+    double* slot = &a.part[kCgRecord * (par * a.gmax + blockIdx.x)];
+    for (int i = 0; i < Q; i++)
+      if (used(i)) st_coh(slot + i, sm.tot[i]);
     wait_all_memory();
     st_coh(&a.tag[tag_slot(par, a.gmax, blockIdx.x)], tg);
   }
@@ -186,60 +285,35 @@ __device__ bool exchange(const CgArgs<TV>& a, __amdgpu_buffer_rsrc_t rpart, unsi
       }
     }
     if (!bad) {
-      const int i = 4 * (par * a.gmax + t);
+      const int i = kCgRecord * (par * a.gmax + t);
       const double p1 = load_coh(rpart, i), p2 = load_coh(rpart, i + 1);
-      const double p3 = load_coh(rpart, i + 2);
+      for (int k = 0; k < S; k++) sm.cpart[t * S + k] = load_coh(rpart, i + 2 + k);
       v1 += p1;
       v2 += p2;
-      sm.cpart[t] = p3;
     }
   }
   if (__syncthreads_or(bad)) return false;
   block_sum2(v1, v2, sm.red);
   s1 = v1;
   s2 = v2;
-  if (a.naggs > 0) {
-    const int na = a.naggs;
-    if (threadIdx.x < na) {
-      double sacc = 0.0;
-      const int g0 = threadIdx.x << a.agg_shift;
-      const int g1 = min(G, (threadIdx.x + 1) << a.agg_shift);
-      for (int g = g0; g < g1; g++) sacc += sm.cpart[g];
-      sm.cin[threadIdx.x] = sacc;
-    }
-    __syncthreads();
-    if (threadIdx.x < na) {
-      double z = 0.0;
-      for (int b = 0; b < na; b++) z += sm.ainv[threadIdx.x * na + b] * sm.cin[b];
-      sm.zc[threadIdx.x] = z;
-    }
-    __syncthreads();
-  }
+  if (a.naggs > 0) coarse_solve(a, sm, G);
   return true;
 }
 
-// Reads the fourth partial (the load's coarse restriction) of phase 1 into
-// sm.cin and returns cb^T A_c^-1 cb.
-template <typename TV>
+// Reads the load's coarse restrictions cb of phase 1 into sm.cin and returns
+// cb^T A_c^-1 cb (sm.zc, the correction of the exchange, is left alone).
+template <typename TV, bool FINE>
 __device__ double load_norm_coarse(const CgArgs<TV>& a, __amdgpu_buffer_rsrc_t rpart,
-                                   Shared& sm) {
-  const int G = gridDim.x, na = a.naggs;
-  for (int t = threadIdx.x; t < G; t += 256) sm.cpart[t] = load_coh(rpart, 4 * (1 * a.gmax + t) + 3);
+                                   Shared<FINE>& sm) {
+  constexpr int L = kSlots<FINE>;
+  const int G = gridDim.x, na = a.naggs, S = 1 << a.sub_shift;
+  for (int t = threadIdx.x; t < G; t += 256)
+    for (int k = 0; k < S; k++)
+      sm.cpart[t * S + k] = load_coh(rpart, kCgRecord * (1 * a.gmax + t) + 2 + L + k);
   __syncthreads();
-  if (threadIdx.x < na) {
-    double sacc = 0.0;
-    const int g0 = threadIdx.x << a.agg_shift;
-    const int g1 = min(G, (threadIdx.x + 1) << a.agg_shift);
-    for (int g = g0; g < g1; g++) sacc += sm.cpart[g];
-    sm.cin[threadIdx.x] = sacc;
-  }
-  __syncthreads();
+  gather_cin(a, sm, G);
   double q = 0.0, dummy = 0.0;
-  if (threadIdx.x < na) {
-    double z = 0.0;
-    for (int b = 0; b < na; b++) z += sm.ainv[threadIdx.x * na + b] * sm.cin[b];
-    q = z * sm.cin[threadIdx.x];
-  }
+  if (threadIdx.x < na) q = coarse_row(a, sm, threadIdx.x) * sm.cin[threadIdx.x];
   block_sum2(q, dummy, sm.red);
   return q;
 }
@@ -249,21 +323,36 @@ __device__ double load_norm_coarse(const CgArgs<TV>& a, __amdgpu_buffer_rsrc_t r
 // travel in the same exchange.  Each thread owns RPT rows of its workgroup's
 // part and keeps their recurrence vectors in registers.  Stops when
 // (r, M r) <= rtol^2 (b_eff, M b_eff), b_eff = b - K x_fixed: the KSP rtol
-// measured against the load vector.
-template <typename TV, int RPT, int MAXW>
+// measured against the load vector.  FINE: the coarse space of sub-part
+// aggregates (several per workgroup, FP32 inverse in LDS).
+template <typename TV, int RPT, int MAXW, bool FINE>
 __global__ __launch_bounds__(256) void k_cg(CgArgs<TV> a) {
   if (a.halt && *a.halt) return;  // queued by a finished integrator run
-  __shared__ Shared sm;
+  __shared__ Shared<FINE> sm;
   const int n = a.n;
   const bool coarse = a.naggs > 0;
   const __amdgpu_buffer_rsrc_t rpub = make_rsrc(a.pub, 2u * (unsigned)n * 8u);
-  const __amdgpu_buffer_rsrc_t rpart = make_rsrc(a.part, 8u * (unsigned)a.gmax * 8u);
-  for (int i = threadIdx.x; i < a.naggs * a.naggs; i += 256) sm.ainv[i] = a.ainv[i];
-  const int agg_own = blockIdx.x >> a.agg_shift;
+  const __amdgpu_buffer_rsrc_t rpart =
+      make_rsrc(a.part, 2u * kCgRecord * (unsigned)a.gmax * 8u);
+  for (int i = threadIdx.x; i < a.naggs * a.naggs; i += 256) {
+    if constexpr (FINE)
+      sm.ainv[i] = a.ainvf[i];
+    else
+      sm.ainv[i] = a.ainv[i];
+  }
+  // row -> aggregate, and each of this thread's rows' aggregate within its
+  // part (the restrictions a workgroup publishes are per such aggregate)
+  const int ash = a.cap_shift + a.agg_shift - a.sub_shift;
+  int sub[RPT];
+#pragma unroll
+  for (int k = 0; k < RPT; k++)
+    sub[k] = (((k << 8) + (int)threadIdx.x) >> (a.cap_shift - a.sub_shift)) &
+             ((1 << a.sub_shift) - 1);
   RowSlots<TV, MAXW> rs[RPT];
   double x[RPT], r[RPT], u[RPT], w[RPT], z[RPT], q[RPT], s[RPT], p[RPT], di[RPT], mk[RPT];
   int orig[RPT];
-  double lbn = 0.0, dummy = 0.0, cr = 0.0, cb = 0.0;
+  constexpr int L = kSlots<FINE>;
+  double lbn = 0.0, dummy = 0.0, cr[L] = {}, cb[L] = {};
   // phase 1: r0 = b - K x0, publish D^-1 r0 and the coarse restrictions of r0, b_eff
 #pragma unroll
   for (int k = 0; k < RPT; k++) {
@@ -287,8 +376,11 @@ __global__ __launch_bounds__(256) void k_cg(CgArgs<TV> a) {
         r[k] = bi - ax;
         const double be = bi - af;
         lbn += be * be * di[k];
-        cr += r[k];
-        cb += be;
+#pragma unroll
+        for (int j = 0; j < L; j++) {
+          cr[j] += sub[k] == j ? r[k] : 0.0;
+          cb[j] += sub[k] == j ? be : 0.0;
+        }
       }
     }
     if (row < n) st_coh(&a.pub[1 * n + row], di[k] * r[k]);
@@ -299,30 +391,30 @@ __global__ __launch_bounds__(256) void k_cg(CgArgs<TV> a) {
   if (alive) {
     const double bnorm = lbn + (coarse ? load_norm_coarse(a, rpart, sm) : 0.0);
     const double stop = a.rtol2 * bnorm;
-    const double zown = coarse ? sm.zc[agg_own] : 0.0;
 #pragma unroll
     for (int k = 0; k < RPT; k++) {
       const int row = (blockIdx.x * RPT + k) * 256 + threadIdx.x;
       if (di[k] != 0.0) {
-        u[k] = di[k] * r[k] + zown;
+        u[k] = di[k] * r[k] + (coarse ? sm.zc[row >> ash] : 0.0);
         w[k] = rs[k].gather(a, rpub, n, sm.zc, coarse);
       }
-      (void)row;
     }
     double gam_old = 1.0, alpha_old = 1.0;
     for (;; it++) {
       const unsigned ph = (unsigned)it + 2u;
       const int par = ph & 1;
-      double gam = 0.0, del = 0.0, cw = 0.0;
+      double gam = 0.0, del = 0.0, cw[L] = {};
+      const double none[L] = {};
 #pragma unroll
       for (int k = 0; k < RPT; k++) {
         const int row = (blockIdx.x * RPT + k) * 256 + threadIdx.x;
         gam += r[k] * u[k];
         del += w[k] * u[k];
-        if (di[k] != 0.0) cw += w[k];
+#pragma unroll
+        for (int j = 0; j < L; j++) cw[j] += di[k] != 0.0 && sub[k] == j ? w[k] : 0.0;
         if (row < n) st_coh(&a.pub[par * n + row], di[k] * w[k]);
       }
-      if (!exchange(a, rpart, ph, gam, del, cw, 0.0, sm)) break;
+      if (!exchange(a, rpart, ph, gam, del, cw, none, sm)) break;
       if (!(gam > stop) || gam <= 1e-300) break;
       if (it >= a.maxit) {
         unconverged = 1u;
@@ -334,12 +426,12 @@ __global__ __launch_bounds__(256) void k_cg(CgArgs<TV> a) {
       }
       const double beta = it > 0 ? gam / gam_old : 0.0;
       const double alpha = it > 0 ? gam / (del - beta * gam / alpha_old) : gam / del;
-      const double zw = coarse ? sm.zc[agg_own] : 0.0;
 #pragma unroll
       for (int k = 0; k < RPT; k++) {
+        const int row = (blockIdx.x * RPT + k) * 256 + threadIdx.x;
         double nk = 0.0;
         if (di[k] != 0.0) {
-          mk[k] = di[k] * w[k] + zw;
+          mk[k] = di[k] * w[k] + (coarse ? sm.zc[row >> ash] : 0.0);
           nk = rs[k].gather(a, rpub, par * n, sm.zc, coarse);
         } else {
           mk[k] = 0.0;
@@ -377,25 +469,30 @@ __global__ __launch_bounds__(256) void k_cg(CgArgs<TV> a) {
   }
 }
 
-template <typename TV, int RPT, int MAXW>
+template <typename TV, int RPT, int MAXW, bool FINE>
 const void* kernel_ptr() {
-  return reinterpret_cast<const void*>(&k_cg<TV, RPT, MAXW>);
+  return reinterpret_cast<const void*>(&k_cg<TV, RPT, MAXW, FINE>);
+}
+
+template <typename TV, bool FINE>
+const void* select_kernel_f(int rpt, int maxw) {
+  if (maxw == 16) {
+    if (rpt == 1) return kernel_ptr<TV, 1, 16, FINE>();
+    if (rpt == 2) return kernel_ptr<TV, 2, 16, FINE>();
+  } else if (maxw == 32) {
+    if (rpt == 1) return kernel_ptr<TV, 1, 32, FINE>();
+    if (rpt == 2) return kernel_ptr<TV, 2, 32, FINE>();
+  } else if (maxw == 0) {
+    if (rpt == 1) return kernel_ptr<TV, 1, 0, FINE>();
+    if (rpt == 2) return kernel_ptr<TV, 2, 0, FINE>();
+    if (rpt == 4) return kernel_ptr<TV, 4, 0, FINE>();
+  }
+  return nullptr;
 }
 
 template <typename TV>
-const void* select_kernel(int rpt, int maxw) {
-  if (maxw == 16) {
-    if (rpt == 1) return kernel_ptr<TV, 1, 16>();
-    if (rpt == 2) return kernel_ptr<TV, 2, 16>();
-  } else if (maxw == 32) {
-    if (rpt == 1) return kernel_ptr<TV, 1, 32>();
-    if (rpt == 2) return kernel_ptr<TV, 2, 32>();
-  } else if (maxw == 0) {
-    if (rpt == 1) return kernel_ptr<TV, 1, 0>();
-    if (rpt == 2) return kernel_ptr<TV, 2, 0>();
-    if (rpt == 4) return kernel_ptr<TV, 4, 0>();
-  }
-  return nullptr;
+const void* select_kernel(int rpt, int maxw, bool fine) {
+  return fine ? select_kernel_f<TV, true>(rpt, maxw) : select_kernel_f<TV, false>(rpt, maxw);
 }
 
 }  // namespace
@@ -423,8 +520,8 @@ int resident_per_cu(const void* kernel, int threads, int device) {
   return occ > 0 ? occ : 0;
 }
 
-int cg_max_workgroups(int rpt, int maxw, int device) {
-  const void* k = select_kernel<double>(rpt, maxw);
+int cg_max_workgroups(int rpt, int maxw, int device, bool fine) {
+  const void* k = select_kernel<double>(rpt, maxw, fine);
   if (!k) return 0;
   int cus = 0;
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
@@ -441,18 +538,24 @@ hipError_t launch_cg(const CgDevice& c, CgWork& w, bool dirichlet, bool coarse,
   // for ones that cannot start
   if (grid <= 0 || grid > w.gmax || grid > kMaxParts || grid > c.resident ||
       maxit >= (1 << 23) ||
-      (c.maxw > 0 && c.max_width > c.maxw) || c.naggs > kMaxAggs ||
+      (c.maxw > 0 && c.max_width > c.maxw) ||
+      c.naggs > (c.fine ? kCgFineAggs : kMaxAggs) || (c.fine && grid > kCgFineParts) ||
+      (c.fine ? c.agg_shift != 0 || c.sub_shift < 1 || c.sub_shift > 2 || !c.ainvf_nbc ||
+                    c.naggs != grid << c.sub_shift
+              : c.sub_shift != 0) ||
       2LL * c.npad * 8 >= 0x7fffffffLL || (256 << 0) * c.rpt != (1 << c.cap_shift))
     return hipErrorInvalidConfiguration;
   const unsigned long long epoch = ++w.epoch;
-  const void* k = select_kernel<double>(c.rpt, c.maxw);
+  const void* k = select_kernel<double>(c.rpt, c.maxw, c.fine);
   if (!k) return hipErrorInvalidConfiguration;
   const double* dinv = dirichlet ? c.dinv_dbc : c.dinv_nbc;
   const double* ainv = dirichlet ? c.ainv_dbc : c.ainv_nbc;
+  const float* ainvf = dirichlet ? c.ainvf_dbc : c.ainvf_nbc;
   const int naggs = coarse ? c.naggs : 0;
-  CgArgs<double> a{c.npad, w.gmax, maxit, c.cap_shift, c.agg_shift, naggs, c.sell_off,
-                   c.sell_col, c.K, c.p2o, dinv, ainv, b, x, w.pub, w.part, w.tag, w.sync,
-                   epoch, rtol * rtol, xadd, xsum, restrict_out, blocal, w.halt};
+  CgArgs<double> a{c.npad,  w.gmax, maxit, c.cap_shift, c.agg_shift, c.sub_shift, naggs,
+                   c.sell_off, c.sell_col, c.K, c.p2o, dinv, ainv, ainvf, b, x, w.pub, w.part,
+                   w.tag,   w.sync, epoch, rtol * rtol, xadd, xsum, restrict_out, blocal,
+                   w.halt};
   void* args[] = {&a};
   if (w.cooperative) return hipLaunchCooperativeKernel(k, dim3(grid), dim3(256), args, 0, st);
   return hipLaunchKernel(k, dim3(grid), dim3(256), args, 0, st);

@@ -156,32 +156,51 @@ std::vector<double> bisection_coords(const HostOperators& o) {
 
 }  // namespace
 
-std::string build_cg_operator(const HostOperators& o, int cap, CgOperator& cg) {
+std::string build_cg_operator(const HostOperators& o, int cap, CgOperator& cg, bool fine) {
   const int n = o.n;
   cg.cap = cap;
   cg.parts = (n + cap - 1) / cap;
   cg.npad = cg.parts * cap;
+  // sub-parts per part: the most (at most 4, at least 64 rows each) that keep
+  // the aggregates within kCgFineAggs
+  cg.sub_shift = 0;
+  if (fine && cg.parts <= kCgFineParts)
+    while (cg.sub_shift < 2 && (cap >> (cg.sub_shift + 1)) >= 64 &&
+           (cg.parts << (cg.sub_shift + 1)) <= kCgFineAggs)
+      cg.sub_shift++;
+  const int S = 1 << cg.sub_shift, sub = cap >> cg.sub_shift;
   std::vector<int> idx(n);
   std::iota(idx.begin(), idx.end(), 0);
+  // parts x S leaves in tree order: leaf s of part g fills rows
+  // g cap + s sub ... (each leaf holds at most n / (parts S) rounded up <= sub)
   std::vector<std::pair<int, int>> parts;
-  rcb(bisection_coords(o), idx, 0, n, cg.parts, parts);
-  if ((int)parts.size() != cg.parts) return "partitioning failed";
+  rcb(bisection_coords(o), idx, 0, n, cg.parts * S, parts);
+  if ((int)parts.size() != cg.parts * S) return "partitioning failed";
   cg.p2o.assign(cg.npad, -1);
-  std::vector<int> o2p(n, -1), part_of(n, -1);
-  for (int g = 0; g < cg.parts; g++) {
-    const int sz = parts[g].second - parts[g].first;
-    if (sz > cap) return "partition part exceeds its capacity";
+  std::vector<int> o2p(n, -1), part_of(n, -1), agg_of(n, -1);
+  for (int l = 0; l < cg.parts * S; l++) {
+    const int sz = parts[l].second - parts[l].first;
+    if (sz > sub) return "partition part exceeds its capacity";
+    const int g = l / S, s = l % S;
     for (int i = 0; i < sz; i++) {
-      const int node = idx[parts[g].first + i];
-      cg.p2o[g * cap + i] = node;
-      o2p[node] = g * cap + i;
+      const int node = idx[parts[l].first + i];
+      const int row = g * cap + s * sub + i;
+      cg.p2o[row] = node;
+      o2p[node] = row;
       part_of[node] = g;
+      agg_of[node] = l;
     }
   }
-  // coarse aggregates: runs of 2^agg_shift consecutive parts, at most 64
+  // coarse aggregates: the sub-parts, or runs of 2^agg_shift consecutive
+  // parts, at most 64
   cg.agg_shift = 0;
-  while (((cg.parts + (1 << cg.agg_shift) - 1) >> cg.agg_shift) > 64) cg.agg_shift++;
-  cg.naggs = (cg.parts + (1 << cg.agg_shift) - 1) >> cg.agg_shift;
+  if (S > 1) {
+    cg.naggs = cg.parts * S;
+  } else {
+    while (((cg.parts + (1 << cg.agg_shift) - 1) >> cg.agg_shift) > 64) cg.agg_shift++;
+    cg.naggs = (cg.parts + (1 << cg.agg_shift) - 1) >> cg.agg_shift;
+    for (int i = 0; i < n; i++) agg_of[i] = part_of[i] >> cg.agg_shift;
+  }
 
   // SELL-64 operator in the permuted order
   cg.nslices = cg.npad / kSlice;
@@ -229,11 +248,11 @@ std::string build_cg_operator(const HostOperators& o, int cap, CgOperator& cg) {
   std::vector<double> An(na * na, 0.0), Ad(na * na, 0.0);
   std::vector<char> has_free(na, 0);
   for (int i = 0; i < n; i++) {
-    const int a = part_of[i] >> cg.agg_shift;
+    const int a = agg_of[i];
     const bool fi = o.blocal[i] < 0;
     if (fi) has_free[a] = 1;
     for (int k = o.rp[i]; k < o.rp[i + 1]; k++) {
-      const int j = o.ci[k], b = part_of[j] >> cg.agg_shift;
+      const int j = o.ci[k], b = agg_of[j];
       An[a * na + b] += o.K[k];
       if (fi && o.blocal[j] < 0) Ad[a * na + b] += o.K[k];
     }

@@ -122,7 +122,7 @@ struct DeviceOps {
 // Communication buffers of the persistent CG kernel.
 struct CgWork {
   double* pub = nullptr;            // 2*n published vector (by phase parity)
-  double* part = nullptr;           // 2*gmax*2 per-workgroup partial sums
+  double* part = nullptr;           // 2*gmax records of kCgRecord partial sums
   unsigned long long* tag = nullptr;  // 2*gmax phase tags, dev::kTagStride words apart
   unsigned* sync = nullptr;         // [2] error flag [3] last iterations
                                     // [4] total iterations [5] solves
@@ -169,8 +169,10 @@ hipError_t launch_exch_div(const DeviceOps& d, bool mixed, const double* m, doub
                            double* rho, bool want_rho, hipStream_t st);
 // The CG's reordered stiffness operator in HBM (see CgOperator).
 struct CgDevice {
-  int parts = 0, npad = 0, rpt = 1, cap_shift = 8, agg_shift = 0, naggs = 0;
+  int parts = 0, npad = 0, rpt = 1, cap_shift = 8, agg_shift = 0, sub_shift = 0, naggs = 0;
   int maxw = 0, max_width = 0;
+  bool fine = false;  // sub-part aggregates, FP32 coarse inverse in LDS (ainvf_*)
+  float *ainvf_nbc = nullptr, *ainvf_dbc = nullptr;
   int* sell_off = nullptr;
   int* sell_col = nullptr;
   double* K = nullptr;
@@ -216,7 +218,7 @@ hipError_t launch_jtimes(const DeviceOps& d, bool mixed, const double* m, const 
 // `width` slots at `rpt` rows per thread (0: slots streamed from memory).
 int cg_slot_variant(int width, int rpt);
 // Cooperative-launch capacity of the CG kernel (workgroups of 256 threads).
-int cg_max_workgroups(int rpt, int maxw, int device);
+int cg_max_workgroups(int rpt, int maxw, int device, bool fine);
 // Workgroups of `kernel` (`threads` threads each) that one CU holds at once:
 // the runtime's occupancy figure, capped by kLdsPerCu over the kernel's LDS
 // rounded up to whole allocation granules (the runtime's figure divides by

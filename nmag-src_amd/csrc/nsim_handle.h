@@ -57,6 +57,7 @@ struct nsim_handle {
   nsim::CgDevice cg;             // the CG's reordered operator
   std::vector<void*> cg_allocs;  // its device buffers
   bool cg_coarse = true;         // two-level preconditioner on
+  bool cg_fine = true;           // its coarse space of sub-part aggregates when it fits
   // field vectors (site-major 3-vectors or scalars)
   double *m = nullptr, *Hext = nullptr, *k1 = nullptr, *k2 = nullptr, *axis = nullptr,
          *pin = nullptr;

@@ -56,15 +56,23 @@ std::string build_host_operators(int n, const double* coords, int T, const int* 
 // space holds the constants of every magnet; Dirichlet: free rows and columns
 // only).  The magnets of a batch are laid side by side for the bisection, so
 // parts rarely straddle two magnets.
+// Fine coarse spaces: a part is split into 2^sub_shift sub-parts, each its own
+// aggregate, while parts x sub-parts stays within kCgFineAggs (their coarse
+// inverse then lives in LDS in FP32, one workgroup per CU, so at most
+// kCgFineParts parts); beyond that, runs of parts form at most 64 aggregates.
+constexpr int kCgFineAggs = 160;
+constexpr int kCgFineParts = 256;
+constexpr int kCgRecord = 12;  // doubles a CG workgroup publishes per exchange
 struct CgOperator {
   int cap = 256, parts = 0, npad = 0, nslices = 0, max_width = 0;
-  int agg_shift = 0, naggs = 0;
+  int agg_shift = 0, sub_shift = 0, naggs = 0;
   std::vector<int> p2o;        // npad: original node or -1
   std::vector<int> sell_off, sell_col;
   std::vector<double> sK;
   std::vector<double> dinv_nbc, dinv_dbc;  // npad, 0 on inactive / fixed rows
   std::vector<double> ainv_nbc, ainv_dbc;  // naggs x naggs
 };
-std::string build_cg_operator(const HostOperators& o, int cap, CgOperator& cg);
+// fine: allow sub-part aggregates (see kCgFineAggs).
+std::string build_cg_operator(const HostOperators& o, int cap, CgOperator& cg, bool fine);
 
 }  // namespace nsim

@@ -118,8 +118,10 @@ def test_warm_start_reduces_iterations():
 
 @pytest.mark.parametrize("dims", [(1, 1, 1, 2.5), (6, 5, 3, 2.5), (40, 20, 2, 2.5)])
 def test_two_level_preconditioner_matches_jacobi_solution(dims):
-    """The coarse correction changes the iteration, not the solution; on one
-    part (a single aggregate) it must vanish for the Neumann problem."""
+    """The coarse correction changes the iteration, not the solution: the
+    two-level solve (sub-part aggregates, one part holds two or four) gives
+    the field of the Jacobi-only solve, in fewer iterations on three parts
+    or more."""
     nx, ny, nz, h = dims
     _, _, orc, gpu, mat = make_pair(nx, ny, nz, h, rtol=1e-12, potential="iterative")
     m = random_m(orc.n, 13)
@@ -134,6 +136,31 @@ def test_two_level_preconditioner_matches_jacobi_solution(dims):
     assert gpu.stat("cg_unconverged") == 0
     if gpu.stat("cg_grid") > 2:
         assert its2 < its1
+
+
+@pytest.mark.parametrize("dims", [(6, 5, 3, 2.5), (40, 20, 2, 2.5), (30, 24, 6, 2.5)])
+def test_fine_coarse_space_matches_part_aggregates(dims):
+    """Sub-part aggregates (two or four per part, their coarse inverse in FP32)
+    change the iteration, not the solution: both Poisson solves at 1e-12 give
+    the field of the coarse space of whole parts, in no more iterations."""
+    nx, ny, nz, h = dims
+    _, _, orc, gpu, mat = make_pair(nx, ny, nz, h, rtol=1e-12, potential="iterative")
+    parts = gpu.stat("cg_grid")
+    assert gpu.stat("cg_aggregates") in (2 * parts, 4 * parts)
+    m = random_m(orc.n, 17)
+    a = _fields_gpu(gpu, m)
+    its_fine = (gpu.stat("cg_iters_nbc"), gpu.stat("cg_iters_dbc"))
+    gpu.tune("cg_fine", 0)
+    assert gpu.stat("cg_aggregates") == parts
+    gpu.set_field("phi1", np.zeros(orc.n))
+    gpu.set_field("phi2", np.zeros(orc.n))
+    b = _fields_gpu(gpu, m)
+    its_part = (gpu.stat("cg_iters_nbc"), gpu.stat("cg_iters_dbc"))
+    assert rel_err(a["H_demag"], b["H_demag"]) < 1e-10
+    assert gpu.stat("cg_unconverged") == 0 and gpu.stat("cg_error") == 0
+    assert its_fine[0] <= its_part[0] and its_fine[1] <= its_part[1]
+    ref = orc.fields(orc_params(mat), m, rtol=1e-13)
+    assert rel_err(a["H_demag"], ref["H_demag"]) < FP64_TOL
 
 
 def test_demag_linear_and_uniform_cuboid_factors():
