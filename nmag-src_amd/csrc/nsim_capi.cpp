@@ -167,7 +167,8 @@ int setup_cg(nsim_handle* h, int rpt) {
     // the fine coarse inverses in FP32 (the kernel keeps them in LDS)
     const std::vector<float> fn(op.ainv_nbc.begin(), op.ainv_nbc.end());
     const std::vector<float> fd(op.ainv_dbc.begin(), op.ainv_dbc.end());
-    ok = up(&c.ainvf_nbc, fn) && up(&c.ainvf_dbc, fd);
+    ok = up(&c.ainvf_nbc, fn) && up(&c.ainvf_dbc, fd) && up(&c.need, op.need) &&
+         up(&c.need_n, op.need_n);
   }
   if (!ok) return fail("CG operator upload failed");
   h->cg = c;

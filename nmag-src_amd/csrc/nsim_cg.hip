@@ -58,6 +58,8 @@ struct CgArgs {
   const double* dinv;  // 0 on padding and fixed (Dirichlet) rows
   const double* ainv;  // naggs x naggs coarse inverse
   const float* ainvf;  // the same in FP32 (fine coarse spaces)
+  const int* need;     // fine coarse spaces: per part, kCgNeed aggregate slots
+  const int* need_n;   // and how many of them are used
   const double* b;     // original order, may be null (zero load)
   double* x;           // original order; in: warm start (+ Dirichlet values), out: solution
   double* pub;         // 2n published vector (by phase parity)
@@ -75,8 +77,9 @@ struct CgArgs {
 };
 
 // LDS of a workgroup: the FP64 coarse inverse of at most 64 aggregates (three
-// workgroups per CU), or, with a fine coarse space, the FP32 inverse of at
-// most kCgFineAggs sub-part aggregates (one workgroup per CU).
+// workgroups per CU), or, with a fine coarse space, the rows of the FP32
+// inverse of at most kCgFineAggs sub-part aggregates that the workgroup's
+// rows touch (at most kCgNeed of them; one workgroup per CU).
 template <bool FINE>
 struct Shared;
 template <>
@@ -94,10 +97,11 @@ struct Shared<true> {
   double red[8];
   double wred[4][kCgRecord];
   double tot[kCgRecord];
-  float ainv[kCgFineAggs * kCgFineAggs];
-  double cpart[kCgFineAggs];
+  float ainv[kCgNeed * kCgFineAggs];  // row r: the inverse's row need[r]
   double cin[kCgFineAggs];
-  double zc[kCgFineAggs];
+  double zc[kCgFineAggs];             // valid on need[0, nneed) only
+  int need[kCgNeed];
+  int nneed;
 };
 // three workgroups per CU (one with a fine coarse space), with 1 KiB for the
 // compiler's own LDS use
@@ -201,61 +205,93 @@ __device__ void block_sums(const double (&v)[Q], Used used, Shared<FINE>& sm) {
   __syncthreads();
 }
 
-// sm.cin = the aggregates' restrictions, from the per-part (or, with a fine
-// coarse space, per-aggregate) ones in sm.cpart.
-template <typename TV, bool FINE>
-__device__ void gather_cin(const CgArgs<TV>& a, Shared<FINE>& sm, int G) {
+// Where an exchange leaves the restrictions it reads, one slot per workgroup
+// and aggregate of it: sm.cpart (aggregates of whole parts, summed per
+// aggregate by gather_cin) or, with a fine coarse space, sm.cin directly.
+__device__ inline double* coarse_in(Shared<false>& sm) { return sm.cpart; }
+__device__ inline double* coarse_in(Shared<true>& sm) { return sm.cin; }
+
+// sm.cin = the aggregates' restrictions, from the per-part ones in sm.cpart
+// (aggregates of whole parts).
+template <typename TV>
+__device__ void gather_cin(const CgArgs<TV>& a, Shared<false>& sm, int G) {
   const int na = a.naggs;
   if (threadIdx.x < na) {
     double sacc = 0.0;
-    if (FINE) {
-      sacc = sm.cpart[threadIdx.x];
-    } else {
-      const int g0 = threadIdx.x << a.agg_shift;
-      const int g1 = min(G, (threadIdx.x + 1) << a.agg_shift);
-      for (int g = g0; g < g1; g++) sacc += sm.cpart[g];
-    }
+    const int g0 = threadIdx.x << a.agg_shift;
+    const int g1 = min(G, (threadIdx.x + 1) << a.agg_shift);
+    for (int g = g0; g < g1; g++) sacc += sm.cpart[g];
     sm.cin[threadIdx.x] = sacc;
   }
   __syncthreads();
 }
 
 // (A_c^-1 sm.cin)_i for aggregate i (threads i < naggs)
-template <typename TV, bool FINE>
-__device__ double coarse_row(const CgArgs<TV>& a, const Shared<FINE>& sm, int i) {
+template <typename TV>
+__device__ double coarse_row(const CgArgs<TV>& a, const Shared<false>& sm, int i) {
   const int na = a.naggs;
   double z = 0.0;
-  for (int b = 0; b < na; b++) z += (double)sm.ainv[i * na + b] * sm.cin[b];
+  for (int b = 0; b < na; b++) z += sm.ainv[i * na + b] * sm.cin[b];
   return z;
 }
 
-// sm.zc = A_c^-1 cin over the aggregates (cin gathered as above).
+// Fine coarse space: (A_c^-1 sm.cin)_{need[r]} for r < nr, four threads per
+// row r = threadIdx.x / 4 (every lane must call: the four partial sums are
+// combined by lane shuffles, in an order that depends on the row alone, so
+// every workgroup gets the same value for an aggregate).
+template <typename TV>
+__device__ double coarse_need_row(const CgArgs<TV>& a, const Shared<true>& sm, int nr) {
+  const int na = a.naggs, r = threadIdx.x >> 2, q = threadIdx.x & 3;
+  double z = 0.0;
+  if (r < nr) {
+    const float* row = &sm.ainv[r * na];
+    for (int b = q; b < na; b += 4) z += (double)row[b] * sm.cin[b];
+  }
+  z += __shfl_xor(z, 1, 64);
+  z += __shfl_xor(z, 2, 64);
+  return z;
+}
+
+// sm.zc = A_c^-1 cin over the aggregates (cin gathered as above); with a
+// fine coarse space, on the aggregates this workgroup's rows touch.
 template <typename TV, bool FINE>
 __device__ void coarse_solve(const CgArgs<TV>& a, Shared<FINE>& sm, int G) {
-  gather_cin(a, sm, G);
-  if (threadIdx.x < a.naggs) sm.zc[threadIdx.x] = coarse_row(a, sm, threadIdx.x);
+  if constexpr (FINE) {
+    const int nr = sm.nneed, r = threadIdx.x >> 2;
+    const double z = coarse_need_row(a, sm, nr);
+    if (r < nr && (threadIdx.x & 3) == 0) sm.zc[sm.need[r]] = z;
+  } else {
+    gather_cin(a, sm, G);
+    if (threadIdx.x < a.naggs) sm.zc[threadIdx.x] = coarse_row(a, sm, threadIdx.x);
+  }
   __syncthreads();
 }
 
 // Publishes this workgroup's record of phase `ph`, [s1, s2, c_0 .. c_{L-1},
 // cb_0 .. cb_{L-1}] (L = kSlots<FINE>) with c and cb the coarse restrictions
 // of its S <= L aggregates (one per part without a fine coarse space; slots
-// from S on are not written or read); returns the grid
-// totals of s1, s2 and leaves in sm.zc the coarse correction A_c^-1 (the
-// restrictions c gathered per aggregate).  Every thread must have issued its
+// from S on are not written or read), and, if `with3`, s3 after them;
+// returns the grid totals of s1, s2 (and s3) and leaves in sm.zc the coarse
+// correction A_c^-1 (the restrictions c gathered per aggregate; with a fine
+// coarse space they are read straight into sm.cin, one slot per aggregate,
+// and zc is formed on the aggregates this workgroup's rows touch).  Every
+// thread must have issued its
 // sc1 stores of the phase's published vector before the call.  Returns false
 // if the exchange timed out or another workgroup reported an error; the spin
 // is bounded so that every workgroup leaves the solver and the grid drains.
 template <typename TV, bool FINE>
 __device__ bool exchange(const CgArgs<TV>& a, __amdgpu_buffer_rsrc_t rpart, unsigned ph,
                          double& s1, double& s2, const double (&c)[kSlots<FINE>],
-                         const double (&cb)[kSlots<FINE>], Shared<FINE>& sm) {
+                         const double (&cb)[kSlots<FINE>], double& s3, bool with3,
+                         Shared<FINE>& sm) {
   wait_all_memory();
   constexpr int L = kSlots<FINE>, Q = 2 + 2 * L;
-  static_assert(Q <= kCgRecord, "workgroup record");
+  static_assert(Q + 1 <= kCgRecord, "workgroup record");
   const int S = 1 << a.sub_shift;
-  const auto used = [S](int i) { return i < 2 || (i - 2) % L < S; };
-  double v[Q];
+  const auto used = [S, with3](int i) {
+    return i < 2 || (i < Q && (i - 2) % L < S) || (i == Q && with3);
+  };
+  double v[Q + 1];
   v[0] = s1;
   v[1] = s2;
 #pragma unroll
@@ -263,17 +299,19 @@ __device__ bool exchange(const CgArgs<TV>& a, __amdgpu_buffer_rsrc_t rpart, unsi
     v[2 + i] = c[i];
     v[2 + L + i] = cb[i];
   }
+  v[Q] = s3;
   block_sums(v, used, sm);
   const int par = ph & 1, G = gridDim.x;
   const unsigned long long tg = (a.epoch << 24) | ph;
   if (threadIdx.x == 0) {
     double* slot = &a.part[kCgRecord * (par * a.gmax + blockIdx.x)];
-    for (int i = 0; i < Q; i++)
+    for (int i = 0; i <= Q; i++)
       if (used(i)) st_coh(slot + i, sm.tot[i]);
     wait_all_memory();
     st_coh(&a.tag[tag_slot(par, a.gmax, blockIdx.x)], tg);
   }
-  double v1 = 0.0, v2 = 0.0;
+  double v1 = 0.0, v2 = 0.0, v3 = 0.0;
+  double* const cdst = coarse_in(sm);
   int bad = 0;
   for (int t = threadIdx.x; t < G && !bad; t += 256) {
     unsigned spins = 0;
@@ -287,7 +325,8 @@ __device__ bool exchange(const CgArgs<TV>& a, __amdgpu_buffer_rsrc_t rpart, unsi
     if (!bad) {
       const int i = kCgRecord * (par * a.gmax + t);
       const double p1 = load_coh(rpart, i), p2 = load_coh(rpart, i + 1);
-      for (int k = 0; k < S; k++) sm.cpart[t * S + k] = load_coh(rpart, i + 2 + k);
+      for (int k = 0; k < S; k++) cdst[t * S + k] = load_coh(rpart, i + 2 + k);
+      if (with3) v3 += load_coh(rpart, i + Q);
       v1 += p1;
       v2 += p2;
     }
@@ -296,24 +335,38 @@ __device__ bool exchange(const CgArgs<TV>& a, __amdgpu_buffer_rsrc_t rpart, unsi
   block_sum2(v1, v2, sm.red);
   s1 = v1;
   s2 = v2;
+  if (with3) {
+    double none = 0.0;
+    block_sum2(v3, none, sm.red);
+    s3 = v3;
+  }
   if (a.naggs > 0) coarse_solve(a, sm, G);
   return true;
 }
 
 // Reads the load's coarse restrictions cb of phase 1 into sm.cin and returns
 // cb^T A_c^-1 cb (sm.zc, the correction of the exchange, is left alone).
+// With a fine coarse space, only this workgroup's own aggregates' terms
+// (need[0, S)): the grid sums them in the next exchange.
 template <typename TV, bool FINE>
 __device__ double load_norm_coarse(const CgArgs<TV>& a, __amdgpu_buffer_rsrc_t rpart,
                                    Shared<FINE>& sm) {
   constexpr int L = kSlots<FINE>;
   const int G = gridDim.x, na = a.naggs, S = 1 << a.sub_shift;
+  double* const dst = coarse_in(sm);
   for (int t = threadIdx.x; t < G; t += 256)
     for (int k = 0; k < S; k++)
-      sm.cpart[t * S + k] = load_coh(rpart, kCgRecord * (1 * a.gmax + t) + 2 + L + k);
+      dst[t * S + k] = load_coh(rpart, kCgRecord * (1 * a.gmax + t) + 2 + L + k);
   __syncthreads();
-  gather_cin(a, sm, G);
   double q = 0.0, dummy = 0.0;
-  if (threadIdx.x < na) q = coarse_row(a, sm, threadIdx.x) * sm.cin[threadIdx.x];
+  if constexpr (FINE) {
+    const int r = threadIdx.x >> 2;
+    const double z = coarse_need_row(a, sm, S);
+    if (r < S && (threadIdx.x & 3) == 0) q = z * sm.cin[sm.need[r]];
+  } else {
+    gather_cin(a, sm, G);
+    if (threadIdx.x < na) q = coarse_row(a, sm, threadIdx.x) * sm.cin[threadIdx.x];
+  }
   block_sum2(q, dummy, sm.red);
   return q;
 }
@@ -334,11 +387,18 @@ __global__ __launch_bounds__(256) void k_cg(CgArgs<TV> a) {
   const __amdgpu_buffer_rsrc_t rpub = make_rsrc(a.pub, 2u * (unsigned)n * 8u);
   const __amdgpu_buffer_rsrc_t rpart =
       make_rsrc(a.part, 2u * kCgRecord * (unsigned)a.gmax * 8u);
-  for (int i = threadIdx.x; i < a.naggs * a.naggs; i += 256) {
-    if constexpr (FINE)
-      sm.ainv[i] = a.ainvf[i];
-    else
-      sm.ainv[i] = a.ainv[i];
+  if constexpr (FINE) {
+    // the rows of the coarse inverse for the aggregates this part touches
+    const int nr = a.naggs > 0 ? a.need_n[blockIdx.x] : 0, na = a.naggs;
+    const int* nd = a.need + (size_t)blockIdx.x * kCgNeed;
+    if (threadIdx.x < kCgNeed) sm.need[threadIdx.x] = (int)threadIdx.x < nr ? nd[threadIdx.x] : 0;
+    if (threadIdx.x == 0) sm.nneed = nr;
+    for (int i = threadIdx.x; i < nr * na; i += 256) {
+      const int r = i / na, b = i - r * na;
+      sm.ainv[i] = a.ainvf[(size_t)nd[r] * na + b];
+    }
+  } else {
+    for (int i = threadIdx.x; i < a.naggs * a.naggs; i += 256) sm.ainv[i] = a.ainv[i];
   }
   // row -> aggregate, and each of this thread's rows' aggregate within its
   // part (the restrictions a workgroup publishes are per such aggregate)
@@ -387,10 +447,12 @@ __global__ __launch_bounds__(256) void k_cg(CgArgs<TV> a) {
   }
   int it = 0;
   unsigned unconverged = 0u;
-  const bool alive = exchange(a, rpart, 1u, lbn, dummy, cr, cb, sm);
+  double bq = 0.0;  // the load's coarse norm (a fine space's own terms until summed)
+  const bool alive = exchange(a, rpart, 1u, lbn, dummy, cr, cb, bq, false, sm);
   if (alive) {
-    const double bnorm = lbn + (coarse ? load_norm_coarse(a, rpart, sm) : 0.0);
-    const double stop = a.rtol2 * bnorm;
+    if (coarse) bq = load_norm_coarse(a, rpart, sm);
+    // with a fine coarse space the first exchange of the loop completes bq
+    double stop = a.rtol2 * (lbn + bq);
 #pragma unroll
     for (int k = 0; k < RPT; k++) {
       const int row = (blockIdx.x * RPT + k) * 256 + threadIdx.x;
@@ -414,7 +476,9 @@ __global__ __launch_bounds__(256) void k_cg(CgArgs<TV> a) {
         for (int j = 0; j < L; j++) cw[j] += di[k] != 0.0 && sub[k] == j ? w[k] : 0.0;
         if (row < n) st_coh(&a.pub[par * n + row], di[k] * w[k]);
       }
-      if (!exchange(a, rpart, ph, gam, del, cw, none, sm)) break;
+      const bool sum_bq = FINE && coarse && it == 0;
+      if (!exchange(a, rpart, ph, gam, del, cw, none, bq, sum_bq, sm)) break;
+      if (sum_bq) stop = a.rtol2 * (lbn + bq);
       if (!(gam > stop) || gam <= 1e-300) break;
       if (it >= a.maxit) {
         unconverged = 1u;
@@ -541,7 +605,7 @@ hipError_t launch_cg(const CgDevice& c, CgWork& w, bool dirichlet, bool coarse,
       (c.maxw > 0 && c.max_width > c.maxw) ||
       c.naggs > (c.fine ? kCgFineAggs : kMaxAggs) || (c.fine && grid > kCgFineParts) ||
       (c.fine ? c.agg_shift != 0 || c.sub_shift < 1 || c.sub_shift > 2 || !c.ainvf_nbc ||
-                    c.naggs != grid << c.sub_shift
+                    !c.ainvf_dbc || !c.need || !c.need_n || c.naggs != grid << c.sub_shift
               : c.sub_shift != 0) ||
       2LL * c.npad * 8 >= 0x7fffffffLL || (256 << 0) * c.rpt != (1 << c.cap_shift))
     return hipErrorInvalidConfiguration;
@@ -553,7 +617,8 @@ hipError_t launch_cg(const CgDevice& c, CgWork& w, bool dirichlet, bool coarse,
   const float* ainvf = dirichlet ? c.ainvf_dbc : c.ainvf_nbc;
   const int naggs = coarse ? c.naggs : 0;
   CgArgs<double> a{c.npad,  w.gmax, maxit, c.cap_shift, c.agg_shift, c.sub_shift, naggs,
-                   c.sell_off, c.sell_col, c.K, c.p2o, dinv, ainv, ainvf, b, x, w.pub, w.part,
+                   c.sell_off, c.sell_col, c.K, c.p2o, dinv, ainv, ainvf, c.need, c.need_n,
+                   b, x, w.pub, w.part,
                    w.tag,   w.sync, epoch, rtol * rtol, xadd, xsum, restrict_out, blocal,
                    w.halt};
   void* args[] = {&a};

@@ -121,6 +121,40 @@ void sym_pinv(std::vector<double>& A, int n, double rel, double scale) {
   A.swap(inv);
 }
 
+// Pseudo-inverse of the Neumann coarse matrix A by deflation.  Its null space
+// is spanned by the vectors constant on each connected set of aggregates (the
+// aggregates that one mesh component covers, linked through the edges of the
+// stiffness pattern; an aggregate without nodes is a set of its own).  With Z
+// their normalised indicators, A + c Z Z^T is SPD and
+//     A^+ = (A + c Z Z^T)^-1 - Z Z^T / c.
+// False (A unchanged) if the shifted matrix is not positive definite.
+bool neumann_pinv(const HostOperators& o, const std::vector<int>& agg_of, std::vector<double>& A,
+                  int na, double c) {
+  std::vector<int> up(na);
+  std::iota(up.begin(), up.end(), 0);
+  auto find = [&](int a) {
+    while (up[a] != a) a = up[a] = up[up[a]];
+    return a;
+  };
+  for (int i = 0; i < o.n; i++)
+    for (int k = o.rp[i]; k < o.rp[i + 1]; k++) {
+      const int a = find(agg_of[i]), b = find(agg_of[o.ci[k]]);
+      if (a != b) up[std::max(a, b)] = std::min(a, b);
+    }
+  std::vector<int> root(na), size(na, 0);
+  for (int a = 0; a < na; a++) size[root[a] = find(a)]++;
+  std::vector<double> B(A);
+  for (int a = 0; a < na; a++)
+    for (int b = 0; b < na; b++)
+      if (root[a] == root[b]) B[a * na + b] += c / size[root[a]];
+  if (!spd_inverse(B, na)) return false;
+  for (int a = 0; a < na; a++)
+    for (int b = 0; b < na; b++)
+      if (root[a] == root[b]) B[a * na + b] -= 1.0 / (c * size[root[a]]);
+  A.swap(B);
+  return true;
+}
+
 // Coordinates for the bisection: a batch's magnets are shifted to lie side by
 // side along x with gaps, so that bisection separates magnets before it cuts
 // one; a single magnet keeps its own coordinates.
@@ -259,13 +293,15 @@ std::string build_cg_operator(const HostOperators& o, int cap, CgOperator& cg, b
   }
   // The Neumann coarse matrix is singular along every vector that is constant
   // on each magnet (one such direction per magnet whose aggregates it does not
-  // share); its pseudo-inverse gives no coarse correction there.  Roundoff
-  // leaves those eigenvalues near 1e-16 of the operator scale (the mean of
-  // sum K_ii over an aggregate); the smallest genuine one of 64 aggregates is
-  // above 1e-4 of the largest.
+  // share), and along each aggregate without nodes; its pseudo-inverse gives
+  // no coarse correction there.  It is formed by deflating those directions
+  // with the operator scale c (the mean of sum K_ii over an aggregate), in
+  // O(naggs^3); the eigen-decomposition is the fallback.  Roundoff leaves the
+  // null eigenvalues near 1e-16 of c; the smallest genuine one is above 1e-4
+  // of the largest at 64 aggregates and falls roughly as naggs^(-2/3) beyond.
   double kd = 0.0;
   for (int i = 0; i < n; i++) kd += o.Kdiag[i];
-  sym_pinv(An, na, 1e-9, kd / na);
+  if (!neumann_pinv(o, agg_of, An, na, kd / na)) sym_pinv(An, na, 1e-9, kd / na);
   for (int a = 0; a < na; a++)
     if (!has_free[a]) {
       for (int b = 0; b < na; b++) Ad[a * na + b] = Ad[b * na + a] = 0.0;
@@ -276,6 +312,38 @@ std::string build_cg_operator(const HostOperators& o, int cap, CgOperator& cg, b
     if (!has_free[a]) Ad[a * na + a] = 0.0;  // no coarse correction there
   cg.ainv_nbc.swap(An);
   cg.ainv_dbc.swap(Ad);
+
+  // fine coarse spaces: the aggregates each part's rows touch, its own first
+  if (S > 1) {
+    cg.need.assign((size_t)cg.parts * kCgNeed, 0);
+    cg.need_n.assign(cg.parts, 0);
+    std::vector<int> mark(na, -1), other;
+    for (int g = 0; g < cg.parts; g++) {
+      int* nd = &cg.need[(size_t)g * kCgNeed];
+      int c = 0;
+      for (int s = 0; s < S; s++) {
+        nd[c++] = g * S + s;
+        mark[g * S + s] = g;
+      }
+      other.clear();
+      for (int r = g * cap; r < (g + 1) * cap; r++) {
+        const int node = cg.p2o[r];
+        if (node < 0) continue;
+        for (int k = o.rp[node]; k < o.rp[node + 1]; k++) {
+          const int b = agg_of[o.ci[k]];
+          if (mark[b] != g) {
+            mark[b] = g;
+            other.push_back(b);
+          }
+        }
+      }
+      if (c + (int)other.size() > kCgNeed)
+        return "a CG part touches more coarse aggregates than a workgroup holds";
+      std::sort(other.begin(), other.end());
+      for (int b : other) nd[c++] = b;
+      cg.need_n[g] = c;
+    }
+  }
   return "";
 }
 

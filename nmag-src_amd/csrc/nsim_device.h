@@ -173,6 +173,7 @@ struct CgDevice {
   int maxw = 0, max_width = 0;
   bool fine = false;  // sub-part aggregates, FP32 coarse inverse in LDS (ainvf_*)
   float *ainvf_nbc = nullptr, *ainvf_dbc = nullptr;
+  int *need = nullptr, *need_n = nullptr;  // per part: the aggregates its rows touch
   int* sell_off = nullptr;
   int* sell_col = nullptr;
   double* K = nullptr;

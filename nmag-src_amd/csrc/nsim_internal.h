@@ -57,11 +57,14 @@ std::string build_host_operators(int n, const double* coords, int T, const int* 
 // only).  The magnets of a batch are laid side by side for the bisection, so
 // parts rarely straddle two magnets.
 // Fine coarse spaces: a part is split into 2^sub_shift sub-parts, each its own
-// aggregate, while parts x sub-parts stays within kCgFineAggs (their coarse
-// inverse then lives in LDS in FP32, one workgroup per CU, so at most
-// kCgFineParts parts); beyond that, runs of parts form at most 64 aggregates.
-constexpr int kCgFineAggs = 160;
+// aggregate, while parts x sub-parts stays within kCgFineAggs and every part's
+// rows touch at most kCgNeed aggregates (its workgroup keeps those rows of the
+// coarse inverse in LDS in FP32 and evaluates only them; one workgroup per CU,
+// so at most kCgFineParts parts); otherwise runs of parts form at most 64
+// aggregates.
+constexpr int kCgFineAggs = 320;
 constexpr int kCgFineParts = 256;
+constexpr int kCgNeed = 64;
 constexpr int kCgRecord = 12;  // doubles a CG workgroup publishes per exchange
 struct CgOperator {
   int cap = 256, parts = 0, npad = 0, nslices = 0, max_width = 0;
@@ -71,6 +74,9 @@ struct CgOperator {
   std::vector<double> sK;
   std::vector<double> dinv_nbc, dinv_dbc;  // npad, 0 on inactive / fixed rows
   std::vector<double> ainv_nbc, ainv_dbc;  // naggs x naggs
+  // fine coarse spaces: per part, kCgNeed slots listing the aggregates its
+  // rows' columns touch (its own sub-parts first), need_n of them used
+  std::vector<int> need, need_n;
 };
 // fine: allow sub-part aggregates (see kCgFineAggs).
 std::string build_cg_operator(const HostOperators& o, int cap, CgOperator& cg, bool fine);
