@@ -149,6 +149,8 @@ int setup_cg(nsim_handle* h, int rpt) {
   c.cap_shift = rpt == 1 ? 8 : rpt == 2 ? 9 : 10;
   c.agg_shift = op.agg_shift;
   c.sub_shift = op.sub_shift;
+  c.need_max = op.need_max;
+  c.need_lg = op.need_lg;
   c.fine = fine;
   c.naggs = op.naggs;
   c.maxw = maxw;

@@ -60,6 +60,7 @@ struct CgArgs {
   const float* ainvf;  // the same in FP32 (fine coarse spaces)
   const int* need;     // fine coarse spaces: per part, kCgNeed aggregate slots
   const int* need_n;   // and how many of them are used
+  int need_lg;         // log2 threads per row of the coarse inverse
   const double* b;     // original order, may be null (zero load)
   double* x;           // original order; in: warm start (+ Dirichlet values), out: solution
   double* pub;         // 2n published vector (by phase parity)
@@ -129,16 +130,81 @@ struct RowSlots {
     width = (a.off[s + 1] - base) >> 6;
     free_mask = 0;
     if constexpr (MAXW > 0) {
+      // every slot's loads issued at once (slots past the width re-read the
+      // last one, then become the row itself with value 0); a slice of
+      // padding rows only has no slots
+      if (width > 0) {
+        double dv[MAXW];
+#pragma unroll
+        for (int k = 0; k < MAXW; k++) {
+          const int e = base + (min(k, width - 1) << 6) + lane;
+          col[k] = a.col[e];
+          val[k] = a.val[e];
+        }
+#pragma unroll
+        for (int k = 0; k < MAXW; k++) dv[k] = a.dinv[col[k]];
+#pragma unroll
+        for (int k = 0; k < MAXW; k++) {
+          if (k >= width) {
+            col[k] = row;
+            val[k] = (TV)0;
+          } else if (dv[k] != 0.0) {
+            free_mask |= 1ull << k;
+          }
+        }
+      } else {
+#pragma unroll
+        for (int k = 0; k < MAXW; k++) {
+          col[k] = row;
+          val[k] = (TV)0;
+        }
+      }
+    }
+  }
+
+  // sum_j K_ij x_j over all columns (ax) and over the fixed ones (af), x in
+  // the original order (the warm start)
+  __device__ void residual_terms(const CgArgs<TV>& a, double& ax, double& af) const {
+    ax = af = 0.0;
+    if constexpr (MAXW > 0) {
+      int o[MAXW];
+      double xv[MAXW];
+#pragma unroll
+      for (int k = 0; k < MAXW; k++) o[k] = a.p2o[col[k]];
+#pragma unroll
+      for (int k = 0; k < MAXW; k++) xv[k] = a.x[max(o[k], 0)];
 #pragma unroll
       for (int k = 0; k < MAXW; k++) {
         if (k < width) {
-          const int e = base + (k << 6) + lane;
-          col[k] = a.col[e];
-          val[k] = a.val[e];
-          if (a.dinv[col[k]] != 0.0) free_mask |= 1ull << k;
-        } else {
-          col[k] = row;
-          val[k] = (TV)0;
+          const double pr = (double)val[k] * xv[k];
+          ax += pr;
+          if (!((free_mask >> k) & 1ull)) af += pr;
+        }
+      }
+    } else {
+      for (int k0 = 0; k0 < width; k0 += 8) {
+        int c[8], o[8];
+        double w8[8], xv[8], dv[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+          const int e = base + (min(k0 + k, width - 1) << 6) + lane;
+          c[k] = a.col[e];
+          w8[k] = (double)a.val[e];
+        }
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+          o[k] = a.p2o[c[k]];
+          dv[k] = a.dinv[c[k]];
+        }
+#pragma unroll
+        for (int k = 0; k < 8; k++) xv[k] = a.x[max(o[k], 0)];
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+          if (k0 + k < width) {
+            const double pr = w8[k] * xv[k];
+            ax += pr;
+            if (dv[k] == 0.0) af += pr;
+          }
         }
       }
     }
@@ -183,6 +249,22 @@ struct RowSlots {
     return acc;
   }
 };
+
+// dst[i] = src(i) for i < total over the workgroup, eight loads in flight per
+// thread (src is called on in-range indices only)
+template <typename T, typename Src>
+__device__ void lds_fill(T* dst, int total, Src src) {
+  for (int i0 = 0; i0 < total; i0 += 8 * 256) {
+    T v[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) v[k] = src(min(i0 + (k << 8) + (int)threadIdx.x, total - 1));
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      const int i = i0 + (k << 8) + threadIdx.x;
+      if (i < total) dst[i] = v[k];
+    }
+  }
+}
 
 // Sums the entries v[i], i < Q with used(i), over the workgroup into
 // sm.tot[i]: a butterfly per wavefront, then the four wavefronts as
@@ -235,20 +317,22 @@ __device__ double coarse_row(const CgArgs<TV>& a, const Shared<false>& sm, int i
   return z;
 }
 
-// Fine coarse space: (A_c^-1 sm.cin)_{need[r]} for r < nr, four threads per
-// row r = threadIdx.x / 4 (every lane must call: the four partial sums are
-// combined by lane shuffles, in an order that depends on the row alone, so
-// every workgroup gets the same value for an aggregate).
+// Fine coarse space: (A_c^-1 sm.cin)_{need[r]} for r < nr, with
+// T = 2^need_lg threads per row r = threadIdx.x / T (T is the same in every
+// workgroup, at most 64, and T times the longest need list is at most 256).
+// Every lane must call: the T partial sums are combined by lane shuffles in
+// an order that depends on the row alone, so every workgroup gets the same
+// value for an aggregate.
 template <typename TV>
 __device__ double coarse_need_row(const CgArgs<TV>& a, const Shared<true>& sm, int nr) {
-  const int na = a.naggs, r = threadIdx.x >> 2, q = threadIdx.x & 3;
+  const int na = a.naggs, T = 1 << a.need_lg;
+  const int r = threadIdx.x >> a.need_lg, q = threadIdx.x & (T - 1);
   double z = 0.0;
   if (r < nr) {
     const float* row = &sm.ainv[r * na];
-    for (int b = q; b < na; b += 4) z += (double)row[b] * sm.cin[b];
+    for (int b = q; b < na; b += T) z += (double)row[b] * sm.cin[b];
   }
-  z += __shfl_xor(z, 1, 64);
-  z += __shfl_xor(z, 2, 64);
+  for (int o = 1; o < T; o <<= 1) z += __shfl_xor(z, o, 64);
   return z;
 }
 
@@ -257,9 +341,9 @@ __device__ double coarse_need_row(const CgArgs<TV>& a, const Shared<true>& sm, i
 template <typename TV, bool FINE>
 __device__ void coarse_solve(const CgArgs<TV>& a, Shared<FINE>& sm, int G) {
   if constexpr (FINE) {
-    const int nr = sm.nneed, r = threadIdx.x >> 2;
+    const int nr = sm.nneed, r = threadIdx.x >> a.need_lg;
     const double z = coarse_need_row(a, sm, nr);
-    if (r < nr && (threadIdx.x & 3) == 0) sm.zc[sm.need[r]] = z;
+    if (r < nr && (threadIdx.x & ((1 << a.need_lg) - 1)) == 0) sm.zc[sm.need[r]] = z;
   } else {
     gather_cin(a, sm, G);
     if (threadIdx.x < a.naggs) sm.zc[threadIdx.x] = coarse_row(a, sm, threadIdx.x);
@@ -360,9 +444,9 @@ __device__ double load_norm_coarse(const CgArgs<TV>& a, __amdgpu_buffer_rsrc_t r
   __syncthreads();
   double q = 0.0, dummy = 0.0;
   if constexpr (FINE) {
-    const int r = threadIdx.x >> 2;
+    const int r = threadIdx.x >> a.need_lg;
     const double z = coarse_need_row(a, sm, S);
-    if (r < S && (threadIdx.x & 3) == 0) q = z * sm.cin[sm.need[r]];
+    if (r < S && (threadIdx.x & ((1 << a.need_lg) - 1)) == 0) q = z * sm.cin[sm.need[r]];
   } else {
     gather_cin(a, sm, G);
     if (threadIdx.x < na) q = coarse_row(a, sm, threadIdx.x) * sm.cin[threadIdx.x];
@@ -393,12 +477,13 @@ __global__ __launch_bounds__(256) void k_cg(CgArgs<TV> a) {
     const int* nd = a.need + (size_t)blockIdx.x * kCgNeed;
     if (threadIdx.x < kCgNeed) sm.need[threadIdx.x] = (int)threadIdx.x < nr ? nd[threadIdx.x] : 0;
     if (threadIdx.x == 0) sm.nneed = nr;
-    for (int i = threadIdx.x; i < nr * na; i += 256) {
-      const int r = i / na, b = i - r * na;
-      sm.ainv[i] = a.ainvf[(size_t)nd[r] * na + b];
-    }
+    __syncthreads();
+    lds_fill(sm.ainv, nr * na, [&](int i) {
+      const int r = i / na;
+      return a.ainvf[(size_t)sm.need[r] * na + (i - r * na)];
+    });
   } else {
-    for (int i = threadIdx.x; i < a.naggs * a.naggs; i += 256) sm.ainv[i] = a.ainv[i];
+    lds_fill(sm.ainv, a.naggs * a.naggs, [&](int i) { return a.ainv[i]; });
   }
   // row -> aggregate, and each of this thread's rows' aggregate within its
   // part (the restrictions a workgroup publishes are per such aggregate)
@@ -424,14 +509,8 @@ __global__ __launch_bounds__(256) void k_cg(CgArgs<TV> a) {
       di[k] = a.dinv[row];
       x[k] = a.x[orig[k]];
       if (di[k] != 0.0) {
-        double ax = 0.0, af = 0.0;
-        for (int t = 0; t < rs[k].width; t++) {
-          const int e = rs[k].base + (t << 6) + rs[k].lane;
-          const int j = a.col[e];
-          const double pr = (double)a.val[e] * a.x[a.p2o[j]];
-          ax += pr;
-          if (a.dinv[j] == 0.0) af += pr;
-        }
+        double ax, af;
+        rs[k].residual_terms(a, ax, af);
         const double bi = a.b ? a.b[orig[k]] : 0.0;
         r[k] = bi - ax;
         const double be = bi - af;
@@ -605,7 +684,8 @@ hipError_t launch_cg(const CgDevice& c, CgWork& w, bool dirichlet, bool coarse,
       (c.maxw > 0 && c.max_width > c.maxw) ||
       c.naggs > (c.fine ? kCgFineAggs : kMaxAggs) || (c.fine && grid > kCgFineParts) ||
       (c.fine ? c.agg_shift != 0 || c.sub_shift < 1 || c.sub_shift > 2 || !c.ainvf_nbc ||
-                    !c.ainvf_dbc || !c.need || !c.need_n || c.naggs != grid << c.sub_shift
+                    !c.ainvf_dbc || !c.need || !c.need_n || c.naggs != grid << c.sub_shift ||
+                    c.need_lg < 2 || c.need_lg > 6 || c.need_max > (256 >> c.need_lg)
               : c.sub_shift != 0) ||
       2LL * c.npad * 8 >= 0x7fffffffLL || (256 << 0) * c.rpt != (1 << c.cap_shift))
     return hipErrorInvalidConfiguration;
@@ -618,7 +698,7 @@ hipError_t launch_cg(const CgDevice& c, CgWork& w, bool dirichlet, bool coarse,
   const int naggs = coarse ? c.naggs : 0;
   CgArgs<double> a{c.npad,  w.gmax, maxit, c.cap_shift, c.agg_shift, c.sub_shift, naggs,
                    c.sell_off, c.sell_col, c.K, c.p2o, dinv, ainv, ainvf, c.need, c.need_n,
-                   b, x, w.pub, w.part,
+                   c.need_lg, b, x, w.pub, w.part,
                    w.tag,   w.sync, epoch, rtol * rtol, xadd, xsum, restrict_out, blocal,
                    w.halt};
   void* args[] = {&a};

@@ -342,7 +342,12 @@ std::string build_cg_operator(const HostOperators& o, int cap, CgOperator& cg, b
       std::sort(other.begin(), other.end());
       for (int b : other) nd[c++] = b;
       cg.need_n[g] = c;
+      cg.need_max = std::max(cg.need_max, c);
     }
+    // as many threads per coarse row (at most a wavefront) as the longest
+    // list leaves
+    cg.need_lg = 6;
+    while ((cg.need_max << cg.need_lg) > 256) cg.need_lg--;
   }
   return "";
 }

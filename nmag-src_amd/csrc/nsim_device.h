@@ -174,6 +174,7 @@ struct CgDevice {
   bool fine = false;  // sub-part aggregates, FP32 coarse inverse in LDS (ainvf_*)
   float *ainvf_nbc = nullptr, *ainvf_dbc = nullptr;
   int *need = nullptr, *need_n = nullptr;  // per part: the aggregates its rows touch
+  int need_max = 0, need_lg = 2;            // see CgOperator
   int* sell_off = nullptr;
   int* sell_col = nullptr;
   double* K = nullptr;

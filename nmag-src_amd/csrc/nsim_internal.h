@@ -77,6 +77,7 @@ struct CgOperator {
   // fine coarse spaces: per part, kCgNeed slots listing the aggregates its
   // rows' columns touch (its own sub-parts first), need_n of them used
   std::vector<int> need, need_n;
+  int need_max = 0, need_lg = 2;  // longest list; log2 threads per coarse row (2^need_lg * need_max <= 256)
 };
 // fine: allow sub-part aggregates (see kCgFineAggs).
 std::string build_cg_operator(const HostOperators& o, int cap, CgOperator& cg, bool fine);
