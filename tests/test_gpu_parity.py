@@ -163,6 +163,34 @@ def test_fine_coarse_space_matches_part_aggregates(dims):
     assert rel_err(a["H_demag"], ref["H_demag"]) < FP64_TOL
 
 
+def test_fine_coarse_space_at_30k_nodes():
+    """A cube of 30^3 cells (29,791 nodes): 117 parts of 256 rows with two
+    sub-part aggregates each, and 59 parts of 512 rows with four; the field
+    matches the coarse space of whole parts (59 aggregates of two parts) with
+    both solves at 1e-11, in fewer iterations."""
+    _, _, orc, gpu, mat = make_pair(30, 30, 30, 2.0, rtol=1e-11, build_bem=False,
+                                    potential="iterative")
+    m = random_m(orc.n, 23)
+    got, its = {}, {}
+    for rpt, subs in ((1, 2), (2, 4)):
+        gpu.tune("cg_rows_per_thread", rpt)
+        assert gpu.stat("cg_aggregates") == subs * gpu.stat("cg_grid")
+        gpu.set_field("phi1", np.zeros(orc.n))
+        gpu.set_field("phi2", np.zeros(orc.n))
+        got[rpt] = _fields_gpu(gpu, m)["H_demag"]
+        its[rpt] = gpu.stat("cg_iters_nbc")
+    gpu.tune("cg_rows_per_thread", 1)
+    gpu.tune("cg_fine", 0)
+    assert gpu.stat("cg_aggregates") == (gpu.stat("cg_grid") + 1) // 2
+    gpu.set_field("phi1", np.zeros(orc.n))
+    gpu.set_field("phi2", np.zeros(orc.n))
+    ref = _fields_gpu(gpu, m)["H_demag"]
+    assert gpu.stat("cg_unconverged") == 0 and gpu.stat("cg_error") == 0
+    for rpt in (1, 2):
+        assert rel_err(got[rpt], ref) < 1e-9
+    assert its[1] < gpu.stat("cg_iters_nbc")
+
+
 def test_demag_linear_and_uniform_cuboid_factors():
     """H_demag(-m) = -H_demag(m); uniform m gives Aharoni's factors (coarse mesh)."""
     from test_oracle import aharoni_nz
