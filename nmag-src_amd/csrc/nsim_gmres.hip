@@ -43,7 +43,7 @@ namespace {
 
 using namespace dev;
 
-constexpr int kSplit = 4;                  // wavefronts per slice in the J.z gather
+constexpr int kSplit = 4;                  // wavefronts of a workgroup (J.z gather partials)
 constexpr unsigned kSpinLimit = 1u << 20;  // polls before an exchange gives up
 constexpr int kMaxGrid = 1024;
 constexpr int kPcNodes = 256;              // largest node block of the local sweeps
@@ -120,6 +120,21 @@ struct GShared {
 // the occupancy query still reports three.
 static_assert(3 * lds_granules((int)sizeof(GShared) + 1024) * kLdsGranule <= kLdsPerCu,
               "GMRES workgroup LDS too large for three workgroups per CU");
+
+// A workgroup takes its slices [s0, s1) in rounds of up to four, all of a
+// round's slices at once: with ns slices in the round, 4 / ns wavefronts (one
+// when ns is 3) share a slice, wavefront wv taking slice sb + wv / per and the
+// column slots part, part + per, ... of its rows.  One slice per workgroup
+// splits every row over the four wavefronts.
+struct SliceRound {
+  int s, part, per;
+  bool active;
+};
+__device__ inline SliceRound slice_round(int sb, int s1, int wv) {
+  const int ns = min(4, s1 - sb);
+  const int per = ns == 1 ? 4 : ns == 2 ? 2 : 1;
+  return SliceRound{sb + wv / per, wv % per, per, wv / per < ns};
+}
 
 // z = M^-1 t on one node (3x3 block).
 __device__ inline void pc_node(const double* __restrict__ Minv, int i, const double* t,
@@ -284,13 +299,14 @@ __device__ double publish_direction(const GmresArgs<TV>& a, GShared& sm, const d
   int cur = 0;
   for (int sw = 0; sw < a.sweeps; sw++) {
     __syncthreads();
-    for (int s = s0; s < s1; s++) {
-      const int row = 64 * s + lane;
-      const bool live = row < a.n;
-      const int b = a.off[s], w = (a.off[s + 1] - b) >> 6;
+    for (int sb = s0; sb < s1; sb += 4) {
+      const SliceRound sr = slice_round(sb, s1, wv);
+      const int row = 64 * sr.s + lane;
+      const bool live = sr.active && row < a.n;
       double hv[3] = {0.0, 0.0, 0.0};
-      if (live)
-        for (int q = wv; q < w; q += kSplit) {
+      if (live) {
+        const int b = a.off[sr.s], w = (a.off[sr.s + 1] - b) >> 6;
+        for (int q = sr.part; q < w; q += sr.per) {
           const int e = b + (q << 6) + lane;
           const int c = a.col[e];
           if (c == row || c < r0 || c >= r1) continue;
@@ -299,12 +315,13 @@ __device__ double publish_direction(const GmresArgs<TV>& a, GShared& sm, const d
           hv[1] += kv * sm.pz[cur][3 * (c - r0) + 1];
           hv[2] += kv * sm.pz[cur][3 * (c - r0) + 2];
         }
+      }
       for (int q = 0; q < 3; q++) sm.gp[wv * 3 + q][lane] = hv[q];
       __syncthreads();
-      if (wv == 0 && live) {
+      if (sr.part == 0 && live) {
         for (int q = 0; q < 3; q++) {
-          double acc = sm.gp[q][lane];
-          for (int u = 1; u < kSplit; u++) acc += sm.gp[u * 3 + q][lane];
+          double acc = sm.gp[wv * 3 + q][lane];
+          for (int u = 1; u < sr.per; u++) acc += sm.gp[(wv + u) * 3 + q][lane];
           hv[q] = acc;
         }
         const Coeffs& cf = a.ct[a.inst ? a.inst[row] : 0];
@@ -355,13 +372,14 @@ __device__ void apply_op(const GmresArgs<TV>& a, GShared& sm, int j, double* Vn,
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const __amdgpu_buffer_rsrc_t rz = z_rsrc(a, j);
   const double sc = sm.zs[j];
-  for (int s = s0; s < s1; s++) {
-    const int row = 64 * s + lane;
-    const bool live = row < a.n;
-    const int b = a.off[s], w = (a.off[s + 1] - b) >> 6;
+  for (int sb = s0; sb < s1; sb += 4) {
+    const SliceRound sr = slice_round(sb, s1, wv);
+    const int row = 64 * sr.s + lane;
+    const bool live = sr.active && row < a.n;
     double hv[3] = {0.0, 0.0, 0.0};
-    if (live)
-      for (int q = wv; q < w; q += kSplit) {
+    if (live) {
+      const int b = a.off[sr.s], w = (a.off[sr.s + 1] - b) >> 6;
+      for (int q = sr.part; q < w; q += sr.per) {
         const int e = b + (q << 6) + lane;
         const int c = a.col[e];
         const double kv = (double)a.K[e];
@@ -369,12 +387,13 @@ __device__ void apply_op(const GmresArgs<TV>& a, GShared& sm, int j, double* Vn,
         hv[1] += kv * load_coh(rz, 3 * c + 1);
         hv[2] += kv * load_coh(rz, 3 * c + 2);
       }
+    }
     for (int q = 0; q < 3; q++) sm.gp[wv * 3 + q][lane] = hv[q];
     __syncthreads();
-    if (wv == 0 && live) {
+    if (sr.part == 0 && live) {
       for (int q = 0; q < 3; q++) {
-        double acc = sm.gp[q][lane];
-        for (int u = 1; u < kSplit; u++) acc += sm.gp[u * 3 + q][lane];
+        double acc = sm.gp[wv * 3 + q][lane];
+        for (int u = 1; u < sr.per; u++) acc += sm.gp[(wv + u) * 3 + q][lane];
         hv[q] = acc;
       }
       const Coeffs& cf = a.ct[a.inst ? a.inst[row] : 0];
@@ -392,27 +411,75 @@ __device__ void apply_op(const GmresArgs<TV>& a, GShared& sm, int j, double* Vn,
   }
 }
 
-// sm.loc[i] = this workgroup's part of V_i . u and, when w is given,
-// sm.loc[j + i] = its part of V_i . w, for i < j.
+// The sums of a one-exchange step over this workgroup's elements, in one
+// pass: sm.loc[i] = V_i . u and sm.loc[j + i] = V_i . w for i < j, then
+// sm.loc[2j] = u . u, [2j + 1] = u . w, [2j + 2] = the threads' zz (their
+// parts of |z_{j+1}|^2), [2j + 3] = w . w; the w terms are zero when w is
+// null.  Thread t takes the elements t, t + 256, ..., the basis vectors go
+// eight at a time with all their loads in flight, and each sum is a
+// wavefront butterfly followed by the four wavefronts in a fixed order (so
+// every workgroup forms its parts the same way).  sm.gp holds the wavefronts'
+// partials (4 (2j + 4) <= 4 (2 kOneExMax + 4) doubles).
 template <typename TV>
-__device__ void pair_dots(const GmresArgs<TV>& a, GShared& sm, const double* u, const double* w,
-                          int j, int e0, int e1) {
-  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  for (int i = wv; i < j; i += 4) {
-    const double* Vi = a.V + (size_t)i * a.ld;
-    double sa = 0.0, sb = 0.0;
-    for (int e = e0 + lane; e < e1; e += 64) {
-      const double vi = Vi[e];
-      sa += vi * u[e];
-      if (w) sb += vi * w[e];
+__device__ void step_dots(const GmresArgs<TV>& a, GShared& sm, const double* u, const double* w,
+                          int j, int e0, int e1, double zz) {
+  constexpr int C = 8;
+  static_assert(4 * (2 * kOneExMax + 4) <= 3 * kSplit * 64, "wavefront partials");
+  const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
+  const int cnt = 2 * j + 4;
+  double* part = &sm.gp[0][0] + wv * cnt;
+  {
+    double uu = 0.0, cc = 0.0, ww = 0.0;
+    for (int e = e0 + tid; e < e1; e += 256) {
+      const double ue = u[e];
+      uu += ue * ue;
+      if (w) {
+        const double we = w[e];
+        cc += ue * we;
+        ww += we * we;
+      }
     }
-    sa = wave_sum(sa);
-    sb = wave_sum(sb);
+    uu = wave_sum(uu);
+    cc = wave_sum(cc);
+    zz = wave_sum(zz);
+    ww = wave_sum(ww);
     if (lane == 0) {
-      sm.loc[i] = sa;
-      sm.loc[j + i] = sb;
+      part[2 * j] = uu;
+      part[2 * j + 1] = cc;
+      part[2 * j + 2] = zz;
+      part[2 * j + 3] = ww;
     }
   }
+  for (int i0 = 0; i0 < j; i0 += C) {
+    double sa[C], sb[C];
+#pragma unroll
+    for (int c = 0; c < C; c++) sa[c] = sb[c] = 0.0;
+    for (int e = e0 + tid; e < e1; e += 256) {
+      double vi[C];
+#pragma unroll
+      for (int c = 0; c < C; c++) vi[c] = i0 + c < j ? a.V[(size_t)(i0 + c) * a.ld + e] : 0.0;
+      const double ue = u[e], we = w ? w[e] : 0.0;
+#pragma unroll
+      for (int c = 0; c < C; c++) {
+        sa[c] += vi[c] * ue;
+        sb[c] += vi[c] * we;
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < C; c++) {
+      if (i0 + c < j) {  // uniform over the workgroup
+        const double ta = wave_sum(sa[c]), tb = wave_sum(sb[c]);
+        if (lane == 0) {
+          part[i0 + c] = ta;
+          part[j + i0 + c] = tb;
+        }
+      }
+    }
+  }
+  __syncthreads();
+  const double* p = &sm.gp[0][0];
+  for (int i = tid; i < cnt; i += 256)
+    sm.loc[i] = (p[i] + p[cnt + i]) + (p[2 * cnt + i] + p[3 * cnt + i]);
 }
 
 // Thread 0: Hessenberg column j (sm.col[0 .. j + 1]) through the earlier
@@ -533,27 +600,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) voi
         if (j + 1 < a.maxl)
           zz = publish_direction(a, sm, Vn, false, j + 1, s0, s1, r0, r1, j, sm.zs[j]);
       }
-      pair_dots(a, sm, Vj, havew ? Vn : nullptr, j, e0, e1);
-      {
-        double uu = 0.0, cc = 0.0, ww = 0.0;
-        for (int e = e0 + tid; e < e1; e += 256) {
-          const double u = Vj[e];
-          uu += u * u;
-          if (havew) {
-            const double w = Vn[e];
-            cc += u * w;
-            ww += w * w;
-          }
-        }
-        block_sum2(uu, cc, sm.red);
-        block_sum2(zz, ww, sm.red);
-        if (tid == 0) {
-          sm.loc[2 * j] = uu;
-          sm.loc[2 * j + 1] = cc;
-          sm.loc[2 * j + 2] = zz;
-          sm.loc[2 * j + 3] = ww;
-        }
-      }
+      step_dots(a, sm, Vj, havew ? Vn : nullptr, j, e0, e1, zz);
       ok = exchange(a, sm, ++ph, 2 * j + 4, false);
       if (!ok) break;
       // everything needed from the totals, before another exchange may
