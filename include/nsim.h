@@ -344,6 +344,10 @@ int nsim_get_stat(nsim_handle* h, const char* name, double* out);
 int nsim_timers_enable(nsim_handle* h, int on);
 /* Launch-shape knobs (used by the tile autotuner); key "cg_rows_per_thread"
  * takes 1, 2, 4 or 8.  No counterpart in the reference (PETSc picks its own).
+ * "cg_coarse" (default 1) turns the CG's coarse correction on; "cg_fine"
+ * (default 1) lets it use two or four sub-part aggregates per part (at most
+ * 256 parts and 320 aggregates), else aggregates of whole parts (setting it
+ * rebuilds the CG operator).
  * Integrator keys: "ts_fixed_order" (see nsim_ts_init), "gmres_device",
  * "gmres_pc_sweeps", "gmres_slices_per_wg", "cooperative_launch",
  * "gmres_onex_steps" (0 .. 32, default 32: the persistent GMRES's Arnoldi
