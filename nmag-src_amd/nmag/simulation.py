@@ -527,7 +527,7 @@ class Simulation:
         for field, sub in zip(FIELD_UNITS, self._subfield_names(FIELD_UNITS)):
             si = FIELD_UNITS[field]
             vals = f[field] * su_factor(si.dims)
-            avg = (vals * (V[:, None] if vals.ndim > 1 else V)).sum(0) / V.sum()
+            avg = (V @ vals) / V.sum()
             names.append(sub)
             units.append(si)
             values.append([float(a) for a in avg] if vals.ndim > 1 else float(avg))
@@ -650,8 +650,8 @@ class Simulation:
         vals = self._field_su(field_name) * su_factor(FIELD_UNITS[field_name].dims)
         V = self._volumes
         if vals.ndim == 1:
-            return float((vals * V).sum() / V.sum())
-        return list((vals * V[:, None]).sum(0) / V.sum())
+            return float(V @ vals / V.sum())
+        return list((V @ vals) / V.sum())
 
     def get_timers(self):
         """Per-kernel device time of the last timed evaluations (ms, launches)."""
