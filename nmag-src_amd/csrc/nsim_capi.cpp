@@ -1248,8 +1248,10 @@ int nsim_advance(nsim_handle* h, double t_final, int max_steps, int exact_tstop,
   if (!h->ts) return fail("nsim_advance: call nsim_ts_init first");
   const int rc = h->ts->advance(t_final, max_steps, exact_tstop != 0, t_reached);
   if (rc) return rc;
+  // the Poisson CG runs only on the iterative route: with the potential
+  // operator there is no flag to read back (and no extra synchronisation)
   unsigned flags = 0;
-  if (handle_error_flags(h, &flags) == 0 && flags != 0)
+  if (!h->use_potential && handle_error_flags(h, &flags) == 0 && flags != 0)
     return fail("nsim_advance: a Poisson solve reported an error (flag " +
                 std::to_string(flags) + ")");
   return 0;
