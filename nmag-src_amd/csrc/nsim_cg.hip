@@ -3,8 +3,11 @@
 // interface/nmag/nmag_lam.py:614-633, 744-754).
 //
 // Rows are reordered so that each workgroup owns one spatially compact part of
-// the mesh (recursive coordinate bisection); consecutive parts form up to 64
-// coarse aggregates.  The preconditioner is additive two-level:
+// the mesh (recursive coordinate bisection).  The coarse aggregates are two
+// or four sub-parts of every part (the part bisected again; at most 320, each
+// workgroup evaluating only the rows of the coarse inverse its rows touch),
+// or, where that does not fit, runs of consecutive parts (at most 64).  The
+// preconditioner is additive two-level:
 //     M = D^-1 + P A_c^-1 P^T,   A_c = P^T K P,
 // with P the piecewise-constant prolongation from the aggregates.  P^T w is a
 // per-workgroup partial sum, so it travels with the inner products: one
