@@ -170,6 +170,36 @@ def test_batch_integration_matches_single_magnets():
         assert np.max(np.abs(mb[k] - ms)) < 1e-4, k
 
 
+def test_iterative_route_batch_matches_oracle_with_either_coarse_space():
+    """The batch on the iterative route: the CG's sub-part aggregates (a few
+    of them straddle two magnets, whose constants the coarse pseudo-inverse
+    then shares) leave every magnet's demag field that of the oracle for the
+    magnet alone, as the aggregates of whole parts do."""
+    meshes, mats, fields = _batch_inputs()
+    b = NativeBatch(meshes, [m.su_coeffs() for m in mats], precision="fp64", nbc_rtol=1e-13,
+                    dbc_rtol=1e-13, nbc_maxits=20000, dbc_maxits=20000, potential="iterative")
+    assert not b.dense_potential
+    grid = b.stat("cg_grid")
+    assert b.stat("cg_aggregates") in (2 * grid, 4 * grid)
+    _set_all(b, fields)
+    got = {}
+    for fine in (1, 0):
+        b.tune("cg_fine", fine)
+        b.set_field("phi1", np.zeros(b.n))
+        b.set_field("phi2", np.zeros(b.n))
+        b.execute("update_dmdt")
+        got[fine] = b.split(b.get_field("H_demag"))
+    assert b.stat("cg_aggregates") <= grid
+    for k, (coords, tets) in enumerate(meshes):
+        f = fields[k]
+        ref = oracle.OracleMesh(coords, tets).fields(orc_params(mats[k]), f["m"],
+                                                     H_ext=f["H_ext"], k1=f["K1"],
+                                                     axis=f["axis"], rtol=1e-13)
+        for fine in (1, 0):
+            assert rel_err(got[fine][k], ref["H_demag"]) < FP64_TOL, (k, fine)
+    assert b.stat("cg_unconverged") == 0 and b.stat("cg_error") == 0
+
+
 @pytest.mark.parametrize("route", ["iterative", "dense"])
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_autotuner_keeps_the_fields(precision, route):
