@@ -706,6 +706,12 @@ def main():
                                cg_iter_bytes * max(it_m["cg_iters_dbc"], 1), kit["cg_dbc"]),
             "bem": roofline("k_dense_gemv<float> BEM", gemv_bytes(it_m["nb"], 4), kit["bem"]),
         }
+        # the CG's bytes are priced against HBM for comparison only: an
+        # iteration is one grid-wide exchange and a gather, so latency bounds it
+        for key, its in (("cg_nbc", it_m["cg_iters_nbc"]), ("cg_dbc", it_m["cg_iters_dbc"])):
+            rooflines_iterative[key]["limited_by"] = "latency (one grid exchange per iteration)"
+            rooflines_iterative[key]["us_per_iteration"] = round(
+                1e3 * kit[key] / max(its, 1), 2)
 
     dyn = None
     if not args.no_dynamics:
